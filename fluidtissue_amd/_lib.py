@@ -1,0 +1,111 @@
+"""ctypes binding of libftreinit.so (include/ftreinit.h).
+
+The library is the product: there is no fallback.  If the shared object is
+missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libftreinit.so")
+
+FT_OK = 0
+FT_ERR_ARG = -1
+FT_ERR_NODEVICE = -2
+FT_ERR_HIP = -3
+FT_ERR_NOMEM = -4
+FT_ERR_T2E = -5
+FT_ERR_DEGENERATE = -6
+FT_ERR_STATE = -7
+FT_ERR_BUDGET = -8
+
+FT_DIST_POINTS = 0
+FT_DIST_TRIANGLES = 1
+FT_BLEND_INIT = 0
+FT_BLEND_UPDATE = 1
+
+FT_PHI_SPHERE_WAVE = 1
+FT_PHI_STAR = 2
+FT_PHI_SPHERES = 3
+FT_PHI_ELLIPSOIDS = 4
+
+
+class FtOpts(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("blend", C.c_int32),
+        ("w_phi", C.c_double),
+        ("w_ifn", C.c_double),
+        ("band", C.c_double),
+        ("bin_scale", C.c_float),
+        ("want_nearest", C.c_int32),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
+class FtCounts(C.Structure):
+    _fields_ = [
+        ("n_ifp", C.c_int64),
+        ("n_tri", C.c_int64),
+        ("n_if_elems", C.c_int64),
+        ("n_clamped", C.c_int64),
+        ("bad_tet", C.c_int64),
+    ]
+
+
+class FtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ftreinit error {code}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+
+lib = C.CDLL(LIB_PATH)
+
+_P = C.c_void_p
+_H = C.c_void_p
+_SIGS = {
+    "ft_opts_default": (None, [C.POINTER(FtOpts)]),
+    "ft_last_error": (C.c_char_p, []),
+    "ft_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "ft_create": (C.c_int, [C.c_int32, C.POINTER(_H)]),
+    "ft_destroy": (C.c_int, [_H]),
+    "ft_set_nodes": (C.c_int, [_H, _P, C.c_int64]),
+    "ft_set_mesh": (C.c_int, [_H, _P, C.c_int64, _P, C.c_int64, _P, C.c_int64, _P]),
+    "ft_reset_phi_points": (C.c_int, [_H, _P, C.c_int64, _P]),
+    "ft_reinit": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), C.POINTER(FtCounts)]),
+    "ft_reinit_device": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), C.POINTER(FtCounts), _P]),
+    "ft_get_interface": (C.c_int, [_H, _P, _P, _P, _P, _P, _P, _P]),
+    "ft_get_nearest": (C.c_int, [_H, _P]),
+    "ft_interface_device": (C.c_int, [_H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "ft_set_timing": (C.c_int, [_H, C.c_int32]),
+    "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "ft_kuhn_counts": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ft_kuhn_mesh": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_double, C.c_uint64,
+                               _P, _P, _P, _P]),
+    "ft_phi_field": (C.c_int, [C.c_int32, C.c_int64, _P, C.c_uint64, _P]),
+}
+OPTIONAL = set()
+
+
+def _bind():
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+_bind()
+
+
+def check(rc):
+    if rc != FT_OK:
+        raise FtError(rc, lib.ft_last_error().decode(errors="replace"))
+    return rc
+
+
+def exported_symbols():
+    return sorted(_SIGS)

@@ -1,0 +1,220 @@
+// ft_device.hpp -- device-side building blocks shared by the reinit kernels:
+// bit tests, wave64/block scans, Morton codes, box bounds and the
+// point/triangle distance functions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+namespace ftd {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ int sign_bit(double x)
+{
+    return (int)((unsigned long long)__double_as_longlong(x) >> 63);
+}
+
+__device__ __forceinline__ int get_bit(const uint64_t* bits, int64_t i)
+{
+    return (int)((bits[i >> 6] >> (i & 63)) & 1ull);
+}
+
+// ---- scans ----------------------------------------------------------------
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d, int width = kWave)
+{
+    const unsigned lo = __shfl_up((unsigned)(v & 0xffffffffu), d, width);
+    const unsigned hi = __shfl_up((unsigned)(v >> 32), d, width);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t shfl_up_any(uint32_t v, int d, int width = kWave)
+{
+    return __shfl_up(v, d, width);
+}
+__device__ __forceinline__ uint64_t shfl_up_any(uint64_t v, int d, int width = kWave)
+{
+    return shfl_up_u64(v, d, width);
+}
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const T t = shfl_up_any(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan across a block of BS threads; *total receives the block sum.
+// lds must hold BS/64 + 1 elements.  Contains __syncthreads: call from all
+// threads of the block.
+template <class T, int BS>
+__device__ __forceinline__ T block_excl_scan(T v, T* lds, T* total)
+{
+    constexpr int NW = BS / kWave;
+    const T inc = wave_incl_scan(v);
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    if (lane == kWave - 1) lds[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        const T s = lane < NW ? lds[lane] : T(0);
+        const T si = wave_incl_scan(s);
+        if (lane < NW) lds[lane] = si - s;
+        if (lane == NW - 1) lds[NW] = si;
+    }
+    __syncthreads();
+    const T res = inc - v + lds[w];
+    *total = lds[NW];
+    __syncthreads();
+    return res;
+}
+
+// ---- Morton codes (x in the highest bit of each triple) --------------------
+__device__ __forceinline__ uint32_t part1by2(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    return (part1by2(x) << 2) | (part1by2(y) << 1) | part1by2(z);
+}
+
+// Uniform grid description (device copy of the host's choice).
+struct Grid {
+    float lo[3];
+    float inv_b;          // 1 / bin edge
+    int dim[3];
+    int pad;
+};
+
+__device__ __forceinline__ uint32_t grid_code(const Grid& g, float x, float y, float z)
+{
+    const float c[3] = {x, y, z};
+    uint32_t b[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float f = (c[k] - g.lo[k]) * g.inv_b;
+        int i = (f >= 0.0f) ? (int)f : 0;           // NaN -> 0
+        i = i < g.dim[k] ? i : g.dim[k] - 1;
+        b[k] = (uint32_t)i;
+    }
+    return morton3(b[0], b[1], b[2]);
+}
+
+// Squared distance from x to an axis-aligned box (0 inside).
+__device__ __forceinline__ float box_dist2(float x, float y, float z, const float* lo,
+                                           const float* hi)
+{
+    const float dx = fmaxf(fmaxf(lo[0] - x, x - hi[0]), 0.0f);
+    const float dy = fmaxf(fmaxf(lo[1] - y, y - hi[1]), 0.0f);
+    const float dz = fmaxf(fmaxf(lo[2] - z, z - hi[2]), 0.0f);
+    return __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, dx * dx));
+}
+
+// ---- FP64 point-triangle distance ------------------------------------------
+// The closest point of a triangle is the orthogonal projection onto its plane
+// when that falls inside (edge functions n.(e_i x (x - v_i)) >= 0), else the
+// closest point of one of its three edge segments.  Triangles thinner than
+// 1e-7 of their longest edge are measured by their segments alone.  Used for
+// the degenerate and thin triangles only.
+__device__ inline double dot3d(const double* u, const double* v)
+{
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+__device__ inline void cross3d(const double* u, const double* v, double* w)
+{
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+__device__ inline double seg_dist2_f64(const double* x, const double* a, const double* b)
+{
+    double ab[3], ax[3];
+    for (int c = 0; c < 3; c++) { ab[c] = b[c] - a[c]; ax[c] = x[c] - a[c]; }
+    const double l2 = dot3d(ab, ab);
+    double t = 0.0;
+    if (l2 > 0.0) {
+        t = dot3d(ax, ab) / l2;
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    }
+    double r[3];
+    for (int c = 0; c < 3; c++) r[c] = ax[c] - t * ab[c];
+    return dot3d(r, r);
+}
+
+__device__ inline double tri_dist2_f64(const double* x, const double* a, const double* b,
+                                       const double* c)
+{
+    double ab[3], bc[3], ca[3], n[3];
+    for (int k = 0; k < 3; k++) { ab[k] = b[k] - a[k]; bc[k] = c[k] - b[k]; ca[k] = a[k] - c[k]; }
+    cross3d(ab, bc, n);
+    const double nn = dot3d(n, n);
+    const double lmax = fmax(dot3d(ab, ab), fmax(dot3d(bc, bc), dot3d(ca, ca)));
+    if (nn > 1e-14 * lmax * lmax) {
+        double xa[3], xb[3], xc[3], t[3];
+        for (int k = 0; k < 3; k++) { xa[k] = x[k] - a[k]; xb[k] = x[k] - b[k]; xc[k] = x[k] - c[k]; }
+        cross3d(ab, xa, t);
+        const double s0 = dot3d(n, t);
+        cross3d(bc, xb, t);
+        const double s1 = dot3d(n, t);
+        cross3d(ca, xc, t);
+        const double s2 = dot3d(n, t);
+        if (s0 >= 0.0 && s1 >= 0.0 && s2 >= 0.0) {
+            const double h = dot3d(n, xa);
+            return h * h / nn;
+        }
+    }
+    return fmin(seg_dist2_f64(x, a, b), fmin(seg_dist2_f64(x, b, c), seg_dist2_f64(x, c, a)));
+}
+
+// ---- FP32 point-triangle distance -------------------------------------------
+// Triangle record (80 B), an orthonormal frame at vertex a with u along ab and
+// n the unit normal; in it the triangle is (0,0), (B,0), (Cx,Cy), Cy > 0:
+//   F0 = (a, B)  F1 = (u, Cx)  F2 = (v, Cy)  F3 = (n, 1/B)
+//   F4 = (1/|ac|^2, 1/|bc|^2, Cx - B, flag)      flag < 0: FP64 path
+// The point is projected once (X, Y, Z); the in-plane distance is 0 inside
+// the triangle, else the nearest clamped edge projection; d^2 = d2D^2 + Z^2.
+// The frame keeps the 2D problem well conditioned for any non-thin triangle.
+struct TriRec {
+    float4 F0, F1, F2, F3, F4;
+};
+
+__device__ __forceinline__ float tri_dist2_f32(float x, float y, float z, const TriRec& r)
+{
+    const float wx = x - r.F0.x, wy = y - r.F0.y, wz = z - r.F0.z;
+    const float X = __fmaf_rn(wz, r.F1.z, __fmaf_rn(wy, r.F1.y, wx * r.F1.x));
+    const float Y = __fmaf_rn(wz, r.F2.z, __fmaf_rn(wy, r.F2.y, wx * r.F2.x));
+    const float Z = __fmaf_rn(wz, r.F3.z, __fmaf_rn(wy, r.F3.y, wx * r.F3.x));
+    const float B = r.F0.w, Cx = r.F1.w, Cy = r.F2.w, dCx = r.F4.z;
+    const float xb = X - B;
+    const bool inside = (Y >= 0.0f) & (__fmaf_rn(Cy, X, -Cx * Y) >= 0.0f) &
+                        (__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f);
+    // edge ab
+    float t = fminf(fmaxf(X * r.F3.w, 0.0f), 1.0f);
+    float ex = __fmaf_rn(-t, B, X);
+    float d = __fmaf_rn(Y, Y, ex * ex);
+    // edge ac
+    t = fminf(fmaxf(__fmaf_rn(Y, Cy, X * Cx) * r.F4.x, 0.0f), 1.0f);
+    ex = __fmaf_rn(-t, Cx, X);
+    float ey = __fmaf_rn(-t, Cy, Y);
+    d = fminf(d, __fmaf_rn(ey, ey, ex * ex));
+    // edge bc
+    t = fminf(fmaxf(__fmaf_rn(Y, Cy, xb * dCx) * r.F4.y, 0.0f), 1.0f);
+    ex = __fmaf_rn(-t, dCx, xb);
+    ey = __fmaf_rn(-t, Cy, Y);
+    d = fminf(d, __fmaf_rn(ey, ey, ex * ex));
+    return __fmaf_rn(Z, Z, inside ? 0.0f : d);
+}
+
+}  // namespace ftd

@@ -1,0 +1,219 @@
+"""Host-side mirror of fluidtissue's reinit interface over libftreinit.so.
+
+  EuclideanDistance   src/fem/EuclideanDistance.hpp:13-52 (init / resetPhi)
+  Reinit              LevelSet::resetPhi_ and the blends of LevelSet::init /
+                      LevelSet::update (src/fem/LevelSet.cpp:73-74, 147-148,
+                      429-455) on a mesh resident in HBM, with the interface
+                      members of LevelSet.hpp:133-139 as outputs
+  kuhn_mesh, phi_field
+                      synthetic inputs (x-major nodes as TetraMesh.cpp:199-211)
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+MODES = {"points": _lib.FT_DIST_POINTS, "triangles": _lib.FT_DIST_TRIANGLES}
+BLENDS = {"init": _lib.FT_BLEND_INIT, "update": _lib.FT_BLEND_UPDATE}
+PHI_KINDS = {
+    "sphere_wave": _lib.FT_PHI_SPHERE_WAVE,
+    "star": _lib.FT_PHI_STAR,
+    "spheres": _lib.FT_PHI_SPHERES,
+    "ellipsoids": _lib.FT_PHI_ELLIPSOIDS,
+}
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+@dataclass
+class Mesh:
+    """Tet mesh in the reference's layout (TetraMesh.hpp:250-256)."""
+    xyz: np.ndarray     # (N, 3) float32
+    tets: np.ndarray    # (T, 4) int32
+    edges: np.ndarray   # (E, 2) int32
+    t2e: np.ndarray     # (T, 6) int32
+    shape: tuple = ()
+    spacing: float = 0.0
+
+    @property
+    def n_nodes(self):
+        return self.xyz.shape[0]
+
+
+@dataclass
+class Interface:
+    """LevelSet.hpp:133-139: ifp_, ifp2e_, ifMesh_ (t, p2t), ifElements_, ifpArea_."""
+    ifp: np.ndarray
+    ifp2e: np.ndarray
+    tris: np.ndarray
+    if_elems: np.ndarray
+    ifp_area: np.ndarray
+    p2t_offsets: np.ndarray
+    p2t: np.ndarray
+
+
+def device_count():
+    n = C.c_int32(0)
+    check(lib.ft_device_count(C.byref(n)))
+    return n.value
+
+
+def kuhn_mesh(n, origin=(0.0, 0.0, 0.0), spacing=None, jitter=0.0, seed=0):
+    """n or (nx, ny, nz) nodes; default spacing puts the grid on the unit cube."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else tuple(n)
+    if spacing is None:
+        spacing = 1.0 / (max(nx, ny, nz) - 1)
+    sp = (spacing,) * 3 if np.isscalar(spacing) else tuple(spacing)
+    N, T, E = C.c_int64(), C.c_int64(), C.c_int64()
+    check(lib.ft_kuhn_counts(nx, ny, nz, C.byref(N), C.byref(T), C.byref(E)))
+    xyz = np.empty((N.value, 3), np.float32)
+    tets = np.empty((T.value, 4), np.int32)
+    edges = np.empty((E.value, 2), np.int32)
+    t2e = np.empty((T.value, 6), np.int32)
+    o = np.array(origin, np.float64)
+    s = np.array(sp, np.float64)
+    check(lib.ft_kuhn_mesh(nx, ny, nz, _ptr(o), _ptr(s), float(jitter), int(seed) & (2**64 - 1),
+                           _ptr(xyz), _ptr(tets), _ptr(edges), _ptr(t2e)))
+    return Mesh(xyz, tets, edges, t2e, (nx, ny, nz), float(sp[0]))
+
+
+def phi_field(kind, xyz, seed=0):
+    xyz = _c(xyz, np.float32)
+    phi = np.empty(xyz.shape[0], np.float64)
+    check(lib.ft_phi_field(PHI_KINDS[kind], xyz.shape[0], _ptr(xyz), int(seed) & (2**64 - 1),
+                           _ptr(phi)))
+    return phi
+
+
+def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest=False,
+              w_phi=0.75, w_ifn=0.25):
+    o = _lib.FtOpts()
+    lib.ft_opts_default(C.byref(o))
+    o.mode = MODES[mode]
+    o.blend = BLENDS[blend]
+    o.band = float(band)
+    o.bin_scale = float(bin_scale)
+    o.want_nearest = 1 if want_nearest else 0
+    o.w_phi = float(w_phi)
+    o.w_ifn = float(w_ifn)
+    return o
+
+
+class _Handle:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib.ft_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ft_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class EuclideanDistance(_Handle):
+    """EuclideanDistance::init / resetPhi (EuclideanDistance.hpp:30-38)."""
+
+    def init(self, meshp, phi0=None, n=None, n_threads=None):
+        xyz = _c(meshp, np.float32).reshape(-1, 3)
+        if n is not None and n != xyz.shape[0]:
+            raise ValueError("n does not match meshp")
+        self._xyz = xyz
+        self.n = xyz.shape[0]
+        check(lib.ft_set_nodes(self._h, _ptr(xyz), self.n))
+
+    def reset_phi(self, ifp, n_ifp, phi_n):
+        """In place on phi_n (float64, length n), as the reference's void resetPhi."""
+        ifp = _c(ifp, np.float32).reshape(-1, 3)
+        if n_ifp != ifp.shape[0]:
+            raise ValueError("n_ifp does not match ifp")
+        if phi_n.dtype != np.float64 or not phi_n.flags.c_contiguous or phi_n.shape[0] != self.n:
+            raise ValueError("phi_n must be a contiguous float64 array of length n")
+        check(lib.ft_reset_phi_points(self._h, _ptr(ifp) if n_ifp else None, n_ifp, _ptr(phi_n)))
+
+
+class Reinit(_Handle):
+    """LevelSet::resetPhi_ + blend on a resident mesh."""
+
+    def set_mesh(self, mesh):
+        self.mesh = mesh
+        xyz = _c(mesh.xyz, np.float32)
+        tets = _c(mesh.tets, np.int32)
+        edges = _c(mesh.edges, np.int32)
+        t2e = _c(mesh.t2e, np.int32)
+        check(lib.ft_set_mesh(self._h, _ptr(xyz), xyz.shape[0], _ptr(tets), tets.shape[0],
+                              _ptr(edges), edges.shape[0], _ptr(t2e)))
+        self.n = xyz.shape[0]
+
+    def reset_phi(self, phi, mode="points", blend="init", band=0.0, bin_scale=0.0,
+                  want_ifn=True, want_nearest=False):
+        """Returns (phi_out, phi_ifn, counts); phi is not modified."""
+        out = np.array(phi, dtype=np.float64, copy=True)
+        ifn = np.empty_like(out) if want_ifn else None
+        o = make_opts(mode, blend, band, bin_scale, want_nearest)
+        cnt = _lib.FtCounts()
+        check(lib.ft_reinit(self._h, _ptr(out), _ptr(ifn), C.byref(o), C.byref(cnt)))
+        self.counts = cnt
+        return out, ifn, cnt
+
+    def reset_phi_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
+                         bin_scale=0.0, want_nearest=False, stream=None):
+        """Device-resident phi (raw device addresses, e.g. tensor.data_ptr())."""
+        o = make_opts(mode, blend, band, bin_scale, want_nearest)
+        cnt = _lib.FtCounts()
+        check(lib.ft_reinit_device(self._h, C.c_void_p(phi_ptr),
+                                   C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o),
+                                   C.byref(cnt), C.c_void_p(stream) if stream else None))
+        self.counts = cnt
+        return cnt
+
+    def set_timing(self, on=True):
+        check(lib.ft_set_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        """(marching tets, index build, query) cumulative ms, number of calls."""
+        ms = (C.c_double * 3)()
+        n = C.c_int64()
+        check(lib.ft_get_timing(self._h, ms, C.byref(n)))
+        return list(ms), n.value
+
+    def interface(self):
+        c = self.counts
+        P, F, X = c.n_ifp, c.n_tri, c.n_if_elems
+        ifp = np.empty((P, 3), np.float32)
+        ifp2e = np.empty(P, np.int32)
+        tris = np.empty((F, 4), np.int32)
+        elems = np.empty(X, np.int32)
+        area = np.empty(P, np.float32)
+        off = np.empty(P + 1, np.int32)
+        p2t = np.empty(3 * F, np.int32)
+        check(lib.ft_get_interface(self._h, _ptr(ifp), _ptr(ifp2e), _ptr(tris), _ptr(elems),
+                                   _ptr(area), _ptr(off), _ptr(p2t)))
+        return Interface(ifp, ifp2e, tris, elems, area, off, p2t)
+
+    def nearest(self):
+        out = np.empty(self.n, np.int32)
+        check(lib.ft_get_nearest(self._h, _ptr(out)))
+        return out

@@ -1,0 +1,187 @@
+/*
+ * ftreinit.h -- C ABI of the MI355X (gfx950) level-set reinitialization.
+ *
+ * Replaces the reinit hot path of fluidtissue:
+ *   EuclideanDistance::init / resetPhi     src/fem/EuclideanDistance.hpp:30-38,
+ *                                          src/fem/EuclideanDistance.cu:66-118
+ *   LevelSet::resetPhi_ and its helpers    src/fem/LevelSet.cpp:261-455
+ *   the caller blends                      src/fem/LevelSet.cpp:18,73-74,147-148
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (FT_OK = 0, negative = error class) and never throws; ft_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Array conventions follow the reference's data contract
+ * (src/fem/TetraMesh.hpp:250-256, src/fem/MathTypes.hpp:27-46):
+ *   xyz    n_nodes x 3 float, AoS (Vec3f)
+ *   phi    n_nodes double (VecXd)
+ *   tets   n_tets x 4 int32 (Vec4i)
+ *   edges  n_edges x 2 int32 (Vec2i), orientation as given by the mesher
+ *   t2e    n_tets x 6 int32 (Vec6i), local edge order {01,02,03,12,13,23}
+ * Interface outputs follow LevelSet.hpp:133-139:
+ *   ifp        P x 3 float, ordered by crossing-edge id       (ifp_)
+ *   ifp2e      P int32, the crossing edge of each point         (ifp2e_)
+ *   tris       F x 4 int32 {a,b,c,-1}, in tet order              (ifMesh_.t)
+ *   if_elems   crossing tets in increasing order                 (ifElements_)
+ *   ifp_area   P float, summed triangle areas per point          (ifpArea_)
+ *   p2t        CSR: offsets (P+1) and triangle ids, ascending    (ifMesh_.p2t)
+ * e2ifp_ is the inverse of ifp2e (ifp2e is strictly increasing).
+ */
+#ifndef FTREINIT_H
+#define FTREINIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FT_OK               0
+#define FT_ERR_ARG         -1  /* bad argument (null pointer, size, option)   */
+#define FT_ERR_NODEVICE    -2  /* no HIP device                               */
+#define FT_ERR_HIP         -3  /* HIP runtime error                           */
+#define FT_ERR_NOMEM       -4  /* device allocation failed                    */
+#define FT_ERR_T2E         -5  /* crossing tet whose t2e edges do not cross 3
+                                  or 4 times, or disagree with its node signs
+                                  (LevelSet.cpp:400-405)                      */
+#define FT_ERR_DEGENERATE  -6  /* crossing tet whose four nodes all have
+                                  |phi| <= FLT_MIN: the reference reads past
+                                  the tet there (LevelSet.cpp:338-341)        */
+#define FT_ERR_STATE       -7  /* call order (e.g. reinit before set_mesh)    */
+#define FT_ERR_BUDGET      -8  /* request exceeds the device memory budget    */
+
+/* ---- options ----------------------------------------------------------- */
+#define FT_DIST_POINTS     0   /* reference semantics: nearest crossing point
+                                  (EuclideanDistance.cu:14-35)                */
+#define FT_DIST_TRIANGLES  1   /* nearest marching-tets triangle              */
+
+#define FT_BLEND_INIT      0   /* LevelSet::init:   phi = phi_ifn (:74)       */
+#define FT_BLEND_UPDATE    1   /* LevelSet::update: phi = w_phi*retained +
+                                  w_ifn*phi_ifn (:148, weights :18)           */
+
+typedef struct ft_opts {
+    int32_t mode;        /* FT_DIST_*                                          */
+    int32_t blend;       /* FT_BLEND_*                                         */
+    double  w_phi;       /* 0.75 (LevelSet.cpp:18)                             */
+    double  w_ifn;       /* 0.25                                               */
+    double  band;        /* > 0: narrow band; nodes with no primitive within
+                            band get +-band and are counted in n_clamped.
+                            0: full domain                                      */
+    float   bin_scale;   /* bin edge in units of the mean node spacing; 0 =
+                            default                                             */
+    int32_t want_nearest;/* 1: keep the per-node nearest primitive index       */
+    int32_t reserved[6];
+} ft_opts;
+
+typedef struct ft_counts {
+    int64_t n_ifp;       /* P: crossing points                                 */
+    int64_t n_tri;       /* F: interface triangles                             */
+    int64_t n_if_elems;  /* crossing tets                                      */
+    int64_t n_clamped;   /* nodes clamped to the band                          */
+    int64_t bad_tet;     /* first offending tet on FT_ERR_T2E/_DEGENERATE      */
+} ft_counts;
+
+typedef struct ft_ctx* ft_handle;
+
+/* Defaults: points mode, init blend, weights (0.75, 0.25), full domain. */
+void        ft_opts_default(ft_opts* o);
+const char* ft_last_error(void);
+int         ft_device_count(int32_t* n);
+
+/* One handle per device and host thread (EuclideanDistance is not
+ * thread-safe either, EuclideanDistance.hpp:13-52). */
+int ft_create(int32_t device, ft_handle* out);
+int ft_destroy(ft_handle h);
+
+/* EuclideanDistance::init (EuclideanDistance.cu:66-86): uploads the node
+ * coordinates once.  phi0 is not needed (the reference's copy of it is dead
+ * data, re-uploaded by resetPhi). */
+int ft_set_nodes(ft_handle h, const float* xyz, int64_t n_nodes);
+
+/* Uploads nodes and connectivity once; they stay resident in HBM
+ * (Mesh3D accessors, TetraMesh.hpp:49-62). */
+int ft_set_mesh(ft_handle h, const float* xyz, int64_t n_nodes,
+                const int32_t* tets, int64_t n_tets, const int32_t* edges,
+                int64_t n_edges, const int32_t* t2e);
+
+/* EuclideanDistance::resetPhi (EuclideanDistance.cu:90-118), host pointers:
+ * phi_n[i] = sign(phi_n[i] < 0) * min_j |ifp_j - p_i| in float, bit-equal to
+ * the reference's CPU branch.  n_ifp == 0 gives +-FLT_MAX. */
+int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp,
+                        double* phi_n);
+
+/* LevelSet::resetPhi_ plus the caller's blend, host phi (LevelSet.cpp:429-455,
+ * 73-74, 147-148).  phi: in = phi_, out = blended phi_.  phi_ifn (nullable):
+ * the full reset (phi_ifn_).  Interface outputs stay on the device; fetch
+ * them with ft_get_interface. */
+int ft_reinit(ft_handle h, double* phi, double* phi_ifn, const ft_opts* o,
+              ft_counts* counts);
+
+/* Same, on device-resident phi (device pointers on the handle's device).
+ * stream: a hipStream_t, or NULL for the handle's own stream. */
+int ft_reinit_device(ft_handle h, double* d_phi, double* d_phi_ifn,
+                     const ft_opts* o, ft_counts* counts, void* stream);
+
+/* Copies the interface of the last reinit to host memory.  Any pointer may
+ * be NULL; sizes come from ft_counts (p2t_offsets: n_ifp+1, p2t_tris:
+ * 3*n_tri). */
+int ft_get_interface(ft_handle h, float* ifp, int32_t* ifp2e, int32_t* tris,
+                     int32_t* if_elems, float* ifp_area, int32_t* p2t_offsets,
+                     int32_t* p2t_tris);
+
+/* Per-node nearest primitive of the last reinit with want_nearest = 1 (point
+ * index in points mode, triangle index in triangles mode; -1 when none). */
+int ft_get_nearest(ft_handle h, int32_t* nearest);
+
+/* Device address of the last reinit's interface buffers (read-only, valid
+ * until the next call on h). Any pointer may be NULL. */
+int ft_interface_device(ft_handle h, const float** ifp, const int32_t** tris,
+                        const int32_t** ifp2e);
+
+/* Phase timing with HIP events on the reinit's stream (no counterpart in the
+ * reference, whose only timing is tic/toc around each phase,
+ * LevelSet.cpp:272-274,360-362,434-436).  on = 1 enables and clears it.
+ * ft_get_timing: cumulative milliseconds of [0] marching tets, [1] bin
+ * index build, [2] distance query kernel, over `calls` reinits. */
+int ft_set_timing(ft_handle h, int32_t on);
+int ft_get_timing(ft_handle h, double* phase_ms, int64_t* calls);
+
+/* ---- synthetic inputs (no counterpart in the reference; the reference
+ * meshes with CGAL, TetraMesh.cpp:23-165) ------------------------------- */
+
+/* Counts of an nx*ny*nz-node grid cut into 6 Kuhn tets per cell. */
+int ft_kuhn_counts(int32_t nx, int32_t ny, int32_t nz, int64_t* n_nodes,
+                   int64_t* n_tets, int64_t* n_edges);
+
+/* Kuhn mesh: node (i,j,k) -> index (i*ny + j)*nz + k (TetraMesh.cpp:199-211),
+ * position origin + (i,j,k)*spacing in FP64 rounded to float; interior nodes
+ * moved by U(-jitter, jitter)*spacing per axis from a splitmix64 stream of
+ * `seed` (jitter = 0: none).  Each cell's 6 tets run 000 -> 111 along the 6
+ * axis orders; edges are oriented low -> high node and numbered by
+ * (low node, direction); t2e in local order {01,02,03,12,13,23}. */
+int ft_kuhn_mesh(int32_t nx, int32_t ny, int32_t nz, const double* origin,
+                 const double* spacing, double jitter, uint64_t seed,
+                 float* xyz, int32_t* tets, int32_t* edges, int32_t* t2e);
+
+/* Level-set fields of the benchmark workloads, FP64 from the float node
+ * coordinates:
+ *   FT_PHI_SPHERE_WAVE  (|x-c|-0.15)*(1+0.5 sin4pix sin4piy sin4piz), c=0.35
+ *   FT_PHI_STAR         (|x-c|-r(theta,phi))*2.3, r=0.3+0.06cos5phi sin^3theta
+ *   FT_PHI_SPHERES      1.5*min over 64 spheres (centres U[0.1,0.9]^3,
+ *                       radii U[0.02,0.08]) of |x-c_i|-r_i
+ *   FT_PHI_ELLIPSOIDS   min over 1..16 ellipsoids (axes U[0.03,0.2], random
+ *                       rotations, centres U[0.25,0.75]^3)
+ * Seeded with splitmix64. */
+#define FT_PHI_SPHERE_WAVE  1
+#define FT_PHI_STAR         2
+#define FT_PHI_SPHERES      3
+#define FT_PHI_ELLIPSOIDS   4
+int ft_phi_field(int32_t kind, int64_t n_nodes, const float* xyz,
+                 uint64_t seed, double* phi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTREINIT_H */

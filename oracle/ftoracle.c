@@ -1,0 +1,381 @@
+/*
+ * ftoracle.c -- CPU oracle for fluidtissue's level-set reinitialization.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker, never the product.  See ftoracle.h
+ * for the reference functions restated here.  Build with -ffp-contract=off
+ * (oracle/Makefile does): each product and sum below is rounded on its own,
+ * which is how the reference's Eigen expressions evaluate.
+ *
+ * Evaluation-order notes (the bits depend on them):
+ *  - Eigen reduces a 3-vector sum (dot, squaredNorm) as x0 + (x1 + x2): its
+ *    unrolled reducer splits a length-3 range into [0] and [1,2].
+ *  - Eigen's cross product evaluates each component as a*b - c*d in one
+ *    source expression.  Whether the reference's clang fused that into an FMA
+ *    depends on its version and -march; the oracle (and the device code)
+ *    define it unfused.  Parity with the original binary is unpinned on this
+ *    point; the test suite pins everything else.
+ *  - distanceToInterface_ (EuclideanDistance.cu:28) is plain C++:
+ *    sqrt((dx*dx + dy*dy) + dz*dz) in float, left to right.
+ */
+#include "ftoracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static inline int sb(double x) { return signbit(x) ? 1 : 0; }
+
+/* ---------------------------------------------------------------------- */
+/* LevelSet::findInterfaceNodes_  (src/fem/LevelSet.cpp:261-300)           */
+/* ---------------------------------------------------------------------- */
+int64_t ora_find_interface_nodes(int64_t n_edges, const int32_t* edges,
+                                 const float* p, const double* phi,
+                                 float* ifp, int32_t* ifp2e)
+{
+    int64_t np = 0;
+    for (int64_t i = 0; i < n_edges; i++) {           /* :278 edge order */
+        const int32_t a = edges[2 * i + 0];
+        const int32_t b = edges[2 * i + 1];
+        if (sb(phi[a]) == sb(phi[b]))                  /* :281 */
+            continue;
+        double t = 0.5;                                /* :287-290 */
+        const double div = phi[a] - phi[b];
+        if (fabs(div) > (double)FLT_MIN)
+            t = phi[a] / div;
+        for (int c = 0; c < 3; c++) {                  /* :292-294, 297 */
+            const double p0 = (double)p[3 * (int64_t)a + c];
+            const double p1 = (double)p[3 * (int64_t)b + c];
+            const double v = (1.0 - t) * p0 + t * p1;
+            ifp[3 * np + c] = (float)v;
+        }
+        ifp2e[np] = (int32_t)i;                        /* :298 */
+        np++;                                          /* e2ifp_[i] = np-1 (:299) */
+    }
+    return np;
+}
+
+/* ---------------------------------------------------------------------- */
+/* LevelSet::reconstructInterface_  (src/fem/LevelSet.cpp:326-425)         */
+/* ---------------------------------------------------------------------- */
+
+/* addTriangle lambda (LevelSet.cpp:330-357). Returns 0, or -2 when no tet
+ * node has |phi| > FLT_MIN. */
+static int add_triangle(const int32_t n[3], const int32_t* tet,
+                        const float* p, const double* phi, const float* ifp,
+                        int32_t* tris, int64_t* n_tri, float* ifp_area)
+{
+    const float* q0 = ifp + 3 * (int64_t)n[0];
+    const float* q1 = ifp + 3 * (int64_t)n[1];
+    const float* q2 = ifp + 3 * (int64_t)n[2];
+    float e1[3], e2[3], nrm[3];
+    for (int c = 0; c < 3; c++) {                      /* :332-333 */
+        e1[c] = q1[c] - q0[c];
+        e2[c] = q2[c] - q0[c];
+    }
+    nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];            /* :334 Eigen cross */
+    nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
+
+    int k = 0;                                         /* :338-341 */
+    while (k < 4 && fabs(phi[tet[k]]) <= (double)FLT_MIN)
+        k++;
+    if (k == 4)
+        return -2;
+    const int32_t pref = tet[k];
+
+    float er[3];                                       /* :343 */
+    for (int c = 0; c < 3; c++)
+        er[c] = p[3 * (int64_t)pref + c] - q0[c];
+    const float dot = nrm[0] * er[0] + (nrm[1] * er[1] + nrm[2] * er[2]);
+
+    int32_t* t = tris + 4 * (*n_tri);
+    t[0] = n[0];
+    if ((signbit(dot) ? 1 : 0) != sb(phi[pref])) {     /* :344-347 */
+        t[1] = n[2];
+        t[2] = n[1];
+    } else {
+        t[1] = n[1];
+        t[2] = n[2];
+    }
+    t[3] = -1;
+    (*n_tri)++;
+
+    const float sq = nrm[0] * nrm[0] + (nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    const float norm = sqrtf(sq);                      /* normal.norm() */
+    for (int j = 0; j < 3; j++)                        /* :356 float += double */
+        ifp_area[n[j]] = (float)((double)ifp_area[n[j]] + 0.5 * (double)norm);
+    return 0;
+}
+
+int ora_reconstruct_interface(int64_t n_tets, const int32_t* tets,
+                              const int32_t* t2e, int64_t n_edges,
+                              const int32_t* edges, const float* p,
+                              const double* phi, int64_t n_ifp,
+                              const float* ifp, const int32_t* ifp2e,
+                              int32_t* tris, int32_t* if_elems,
+                              float* ifp_area, int64_t* n_tri,
+                              int64_t* n_elems, int64_t* bad_tet)
+{
+    /* e2ifp_ is a std::map edge -> ifp index; a dense table gives the same
+     * lookups (every crossing edge was inserted by findInterfaceNodes_). */
+    int32_t* e2ifp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n_edges > 0 ? n_edges : 1));
+    for (int64_t e = 0; e < n_edges; e++)
+        e2ifp[e] = -1;
+    for (int64_t i = 0; i < n_ifp; i++)
+        e2ifp[ifp2e[i]] = (int32_t)i;
+    for (int64_t i = 0; i < n_ifp; i++)                /* :366 */
+        ifp_area[i] = 0.0f;
+
+    *n_tri = 0;
+    *n_elems = 0;
+    *bad_tet = -1;
+    int rc = 0;
+    for (int64_t i = 0; i < n_tets; i++) {             /* :377 tet order */
+        const int32_t* t = tets + 4 * i;
+        const int s = sb(phi[t[0]]) + sb(phi[t[1]]) + sb(phi[t[2]]) + sb(phi[t[3]]);
+        if (s == 4 || s == 0)                          /* :380-383 */
+            continue;
+        int32_t eif[6];
+        int ne = 0;
+        for (int j = 0; j < 6; j++) {                  /* :386-391 t2e order */
+            const int32_t e = t2e[6 * i + j];
+            const int32_t a = edges[2 * (int64_t)e + 0];
+            const int32_t b = edges[2 * (int64_t)e + 1];
+            if (sb(phi[a]) != sb(phi[b]))
+                eif[ne++] = e;
+        }
+        if (ne == 3) {                                 /* :393-395 */
+            const int32_t n[3] = {e2ifp[eif[0]], e2ifp[eif[1]], e2ifp[eif[2]]};
+            rc = add_triangle(n, t, p, phi, ifp, tris, n_tri, ifp_area);
+        } else if (ne == 4) {                          /* :396-399 */
+            const int32_t na[3] = {e2ifp[eif[0]], e2ifp[eif[2]], e2ifp[eif[3]]};
+            rc = add_triangle(na, t, p, phi, ifp, tris, n_tri, ifp_area);
+            if (rc == 0) {
+                const int32_t nb[3] = {e2ifp[eif[0]], e2ifp[eif[1]], e2ifp[eif[3]]};
+                rc = add_triangle(nb, t, p, phi, ifp, tris, n_tri, ifp_area);
+            }
+        } else {                                       /* :400-405 */
+            rc = -1;
+        }
+        if (rc != 0) {
+            *bad_tet = i;
+            break;
+        }
+        if_elems[(*n_elems)++] = (int32_t)i;           /* :407 */
+    }
+    free(e2ifp);
+    return rc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* distanceToInterface_ host form (EuclideanDistance.cu:14-35) and the     */
+/* CPU branch of EuclideanDistance::resetPhi (:108-117)                    */
+/* ---------------------------------------------------------------------- */
+static inline float nearest_point(const float* x, const float* ifp,
+                                  int64_t n_ifp, int32_t* jmin)
+{
+    float dmin = FLT_MAX;
+    int32_t jm = -1;
+    for (int64_t j = 0; j < n_ifp; j++) {
+        const float dx = ifp[3 * j + 0] - x[0];
+        const float dy = ifp[3 * j + 1] - x[1];
+        const float dz = ifp[3 * j + 2] - x[2];
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (d < dmin) {                                /* :30 strict: first min */
+            dmin = d;
+            jm = (int32_t)j;
+        }
+    }
+    *jmin = jm;
+    return dmin;
+}
+
+void ora_reset_phi_points(int64_t n_nodes, const float* p, const float* ifp,
+                          int64_t n_ifp, double* phi_n, int32_t* nearest,
+                          int nthreads)
+{
+    if (nthreads < 1)
+        nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < n_nodes; i++) {
+        int32_t j;
+        float dmin = nearest_point(p + 3 * i, ifp, n_ifp, &j);
+        if (phi_n[i] < 0.0f)                           /* :113 */
+            dmin = -dmin;
+        phi_n[i] = dmin;
+        if (nearest)
+            nearest[i] = j;
+    }
+}
+
+void ora_point_distance(int64_t n_query, const int64_t* nodes, const float* p,
+                        const float* ifp, int64_t n_ifp, float* out_d,
+                        int32_t* out_j, int nthreads)
+{
+    if (nthreads < 1)
+        nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+    for (int64_t k = 0; k < n_query; k++) {
+        const int64_t i = nodes ? nodes[k] : k;
+        int32_t j;
+        const float d = nearest_point(p + 3 * i, ifp, n_ifp, &j);
+        out_d[k] = d;
+        if (out_j)
+            out_j[k] = j;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* FP64 point-triangle distance.  The closest point of a triangle is the   */
+/* orthogonal projection onto its plane when that falls inside (edge       */
+/* functions n.(e_i x (x - v_i)) >= 0 for all three edges), else the       */
+/* closest point of one of the three edge segments.  Triangles whose       */
+/* height is below 1e-7 of their longest edge have no reliable normal in   */
+/* FP64 and are measured by their edge segments alone (error <= that       */
+/* height).                                                                */
+/* ---------------------------------------------------------------------- */
+static inline double dot3(const double* u, const double* v)
+{
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+
+static inline void cross3(const double* u, const double* v, double* w)
+{
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+static double seg_dist2(const double* x, const double* a, const double* b)
+{
+    double ab[3], ax[3];
+    for (int c = 0; c < 3; c++) {
+        ab[c] = b[c] - a[c];
+        ax[c] = x[c] - a[c];
+    }
+    const double l2 = dot3(ab, ab);
+    double t = 0.0;
+    if (l2 > 0.0) {
+        t = dot3(ax, ab) / l2;
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    }
+    double r[3];
+    for (int c = 0; c < 3; c++)
+        r[c] = ax[c] - t * ab[c];
+    return dot3(r, r);
+}
+
+double ora_point_triangle_dist2(const double* x, const double* a,
+                                const double* b, const double* c)
+{
+    double ab[3], bc[3], ca[3], n[3];
+    for (int k = 0; k < 3; k++) {
+        ab[k] = b[k] - a[k];
+        bc[k] = c[k] - b[k];
+        ca[k] = a[k] - c[k];
+    }
+    cross3(ab, bc, n);
+    const double nn = dot3(n, n);
+    double lmax = dot3(ab, ab);
+    if (dot3(bc, bc) > lmax) lmax = dot3(bc, bc);
+    if (dot3(ca, ca) > lmax) lmax = dot3(ca, ca);
+    if (nn > 1e-14 * lmax * lmax) {
+        double xa[3], xb[3], xc[3], t[3];
+        for (int k = 0; k < 3; k++) {
+            xa[k] = x[k] - a[k];
+            xb[k] = x[k] - b[k];
+            xc[k] = x[k] - c[k];
+        }
+        cross3(ab, xa, t);
+        const double s0 = dot3(n, t);
+        cross3(bc, xb, t);
+        const double s1 = dot3(n, t);
+        cross3(ca, xc, t);
+        const double s2 = dot3(n, t);
+        if (s0 >= 0.0 && s1 >= 0.0 && s2 >= 0.0) {
+            const double h = dot3(n, xa);
+            return h * h / nn;
+        }
+    }
+    double d = seg_dist2(x, a, b);
+    const double d2 = seg_dist2(x, b, c);
+    const double d3 = seg_dist2(x, c, a);
+    if (d2 < d) d = d2;
+    if (d3 < d) d = d3;
+    return d;
+}
+
+static inline void load_tri(const float* ifp, const int32_t* tris, int64_t f,
+                            double* a, double* b, double* c)
+{
+    const int32_t* t = tris + 4 * f;
+    for (int k = 0; k < 3; k++) {
+        a[k] = (double)ifp[3 * (int64_t)t[0] + k];
+        b[k] = (double)ifp[3 * (int64_t)t[1] + k];
+        c[k] = (double)ifp[3 * (int64_t)t[2] + k];
+    }
+}
+
+double ora_node_triangle_distance(int64_t node, const float* p,
+                                  const float* ifp, const int32_t* tris,
+                                  int64_t f)
+{
+    double x[3], a[3], b[3], c[3];
+    for (int k = 0; k < 3; k++) x[k] = (double)p[3 * node + k];
+    load_tri(ifp, tris, f, a, b, c);
+    return sqrt(ora_point_triangle_dist2(x, a, b, c));
+}
+
+void ora_triangle_distance(int64_t n_query, const int64_t* nodes,
+                           const float* p, const float* ifp,
+                           const int32_t* tris, int64_t n_tri, double* out_d,
+                           int32_t* out_f, int nthreads)
+{
+    if (nthreads < 1)
+        nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+    for (int64_t k = 0; k < n_query; k++) {
+        const int64_t i = nodes ? nodes[k] : k;
+        double x[3], a[3], b[3], c[3];
+        for (int m = 0; m < 3; m++) x[m] = (double)p[3 * i + m];
+        double best = INFINITY;
+        int32_t bf = -1;
+        for (int64_t f = 0; f < n_tri; f++) {
+            load_tri(ifp, tris, f, a, b, c);
+            const double d2 = ora_point_triangle_dist2(x, a, b, c);
+            if (d2 < best) {
+                best = d2;
+                bf = (int32_t)f;
+            }
+        }
+        out_d[k] = bf >= 0 ? sqrt(best) : INFINITY;
+        out_f[k] = bf;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* LevelSet::resetPhi_ epilogue (LevelSet.cpp:438-448) and the blends of    */
+/* LevelSet::init (:73-74) and LevelSet::update (:147-148, weights :18).    */
+/* ---------------------------------------------------------------------- */
+void ora_finish_reset(int64_t n_nodes, const double* phi_old,
+                      const double* phi_ifn, int64_t n_ifp,
+                      const int32_t* ifp2e, const int32_t* edges, int blend,
+                      double w_phi, double w_ifn, double* retained,
+                      double* phi_out)
+{
+    memcpy(retained, phi_ifn, sizeof(double) * (size_t)n_nodes);
+    for (int64_t i = 0; i < n_ifp; i++) {              /* :443-447 */
+        const int32_t e = ifp2e[i];
+        const int32_t a = edges[2 * (int64_t)e + 0];
+        const int32_t b = edges[2 * (int64_t)e + 1];
+        retained[a] = phi_old[a];
+        retained[b] = phi_old[b];
+    }
+    for (int64_t i = 0; i < n_nodes; i++) {
+        if (blend == 0)
+            phi_out[i] = phi_ifn[i];                   /* :74 */
+        else
+            phi_out[i] = w_phi * retained[i] + w_ifn * phi_ifn[i];  /* :148 */
+    }
+}

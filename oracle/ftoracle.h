@@ -1,0 +1,111 @@
+/*
+ * ftoracle.h -- CPU oracle for fluidtissue's level-set reinitialization.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation is compared against.  Only the test suite, the smoke check
+ * and the CPU-baseline leg of the benchmark may load it; the product path
+ * (libftreinit.so) never links or calls it.
+ *
+ * It restates, in plain C, the reference's CPU reinit:
+ *   LevelSet::findInterfaceNodes_    src/fem/LevelSet.cpp:261-322
+ *   LevelSet::reconstructInterface_  src/fem/LevelSet.cpp:326-425
+ *   LevelSet::resetPhi_              src/fem/LevelSet.cpp:429-455
+ *   EuclideanDistance::resetPhi (CPU branch)  src/fem/EuclideanDistance.cu:108-117
+ *   distanceToInterface_ (host form)          src/fem/EuclideanDistance.cu:14-35
+ *   post-reset blends                         src/fem/LevelSet.cpp:18,73-74,147-148
+ * plus an FP64 nearest-triangle oracle (closed-form point-triangle distance)
+ * for the triangle-distance mode, which has no counterpart in the reference.
+ *
+ * The whole file is compiled with -ffp-contract=off: every multiply and add
+ * is rounded separately, the way Eigen's expression templates evaluate the
+ * reference's vector arithmetic.
+ *
+ * Array conventions (the reference's data contract, src/fem/TetraMesh.hpp:250-256):
+ *   p      n_nodes x 3 float, AoS (Vec3f)
+ *   phi    n_nodes double (VecXd)
+ *   tets   n_tets x 4 int32 (Vec4i)
+ *   edges  n_edges x 2 int32 (Vec2i), orientation as given by the mesher
+ *   t2e    n_tets x 6 int32 (Vec6i), local order {01,02,03,12,13,23}
+ */
+#ifndef FTORACLE_H
+#define FTORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* LevelSet::findInterfaceNodes_ (LevelSet.cpp:261-300).  Writes ifp (P x 3)
+ * and ifp2e (P) in increasing edge order; buffers must hold n_edges entries.
+ * Returns P. */
+int64_t ora_find_interface_nodes(int64_t n_edges, const int32_t* edges,
+                                 const float* p, const double* phi,
+                                 float* ifp, int32_t* ifp2e);
+
+/* LevelSet::reconstructInterface_ (LevelSet.cpp:326-425).  tris is F x 4
+ * ({a,b,c,-1}), if_elems the crossing tets, ifp_area P floats.  Capacities:
+ * tris 2*n_tets, if_elems n_tets.  Returns 0, or -1 for a tet whose t2e
+ * yields a crossing-edge count other than 3 or 4 (LevelSet.cpp:400-405), or
+ * -2 for a crossing tet whose four nodes all have |phi| <= FLT_MIN (the
+ * reference reads past the tet there, LevelSet.cpp:340-341).  On error
+ * *n_tri / *n_elems hold what was emitted before the failing tet and
+ * *bad_tet its index. */
+int ora_reconstruct_interface(int64_t n_tets, const int32_t* tets,
+                              const int32_t* t2e, int64_t n_edges,
+                              const int32_t* edges, const float* p,
+                              const double* phi, int64_t n_ifp,
+                              const float* ifp, const int32_t* ifp2e,
+                              int32_t* tris, int32_t* if_elems,
+                              float* ifp_area, int64_t* n_tri,
+                              int64_t* n_elems, int64_t* bad_tet);
+
+/* EuclideanDistance::resetPhi, CPU branch (EuclideanDistance.cu:108-117):
+ * phi_n[i] = (phi_n[i] < 0 ? -1 : 1) * min_j |ifp_j - p_i| (float, first
+ * minimum wins).  nearest (nullable) receives j, or -1 when n_ifp == 0. */
+void ora_reset_phi_points(int64_t n_nodes, const float* p, const float* ifp,
+                          int64_t n_ifp, double* phi_n, int32_t* nearest,
+                          int nthreads);
+
+/* Same distance for a subset of nodes: out_d[k] = unsigned float distance of
+ * node nodes[k] (nodes == NULL: node k), out_j[k] the nearest point. */
+void ora_point_distance(int64_t n_query, const int64_t* nodes, const float* p,
+                        const float* ifp, int64_t n_ifp, float* out_d,
+                        int32_t* out_j, int nthreads);
+
+/* FP64 squared distance from x to the closed triangle (a, b, c): plane
+ * projection when it falls inside, else the nearest edge segment; valid for
+ * degenerate (zero-area) triangles too. */
+double ora_point_triangle_dist2(const double* x, const double* a,
+                                const double* b, const double* c);
+
+/* FP64 nearest-triangle oracle by brute force over all triangles.  For each
+ * queried node: out_d = unsigned FP64 distance, out_f = nearest triangle
+ * (lowest index among exact FP64 ties), -1 when n_tri == 0. */
+void ora_triangle_distance(int64_t n_query, const int64_t* nodes,
+                           const float* p, const float* ifp,
+                           const int32_t* tris, int64_t n_tri, double* out_d,
+                           int32_t* out_f, int nthreads);
+
+/* FP64 distance of node `node` to one triangle f. */
+double ora_node_triangle_distance(int64_t node, const float* p,
+                                  const float* ifp, const int32_t* tris,
+                                  int64_t f);
+
+/* LevelSet::resetPhi_ epilogue plus the caller's blend
+ * (LevelSet.cpp:438-448, 73-74, 147-148):
+ *   phi_ifn = signed reset distance (input, already computed)
+ *   retained = phi_ifn, except phi_old at both nodes of every crossing edge
+ *   blend == 0 (init):   phi_out = phi_ifn
+ *   blend == 1 (update): phi_out = w_phi * retained + w_ifn * phi_ifn   */
+void ora_finish_reset(int64_t n_nodes, const double* phi_old,
+                      const double* phi_ifn, int64_t n_ifp,
+                      const int32_t* ifp2e, const int32_t* edges, int blend,
+                      double w_phi, double w_ifn, double* retained,
+                      double* phi_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

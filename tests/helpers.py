@@ -1,0 +1,69 @@
+"""Shared inputs and comparison helpers for the GPU parity tests."""
+import functools
+
+import numpy as np
+
+import fluidtissue_amd as fa
+import oracle
+
+
+@functools.lru_cache(maxsize=None)
+def workload(name):
+    """The benchmark workloads at test sizes: (mesh, phi, h)."""
+    if name == "cfg1":          # 32^3 sphere with a wave: sign right, phi not a distance
+        m = fa.kuhn_mesh(32)
+        return m, fa.phi_field("sphere_wave", m.xyz), 1.0 / 31
+    if name == "cfg2":          # 128^3 star-shaped surface
+        m = fa.kuhn_mesh(128)
+        return m, fa.phi_field("star", m.xyz), 1.0 / 127
+    if name == "cfg2s":         # 64^3 star: the same field on a smaller grid
+        m = fa.kuhn_mesh(64)
+        return m, fa.phi_field("star", m.xyz), 1.0 / 63
+    if name == "cfg3s":         # 96^3 jittered mesh, 64 spheres
+        m = fa.kuhn_mesh(96, jitter=0.2, seed=1)
+        return m, fa.phi_field("spheres", m.xyz, seed=0), 1.0 / 95
+    if name == "cfg3":          # 256^3 jittered mesh, 64 spheres
+        m = fa.kuhn_mesh(256, jitter=0.2, seed=1)
+        return m, fa.phi_field("spheres", m.xyz, seed=0), 1.0 / 255
+    if name == "ellipsoids":
+        m = fa.kuhn_mesh(40)
+        return m, fa.phi_field("ellipsoids", m.xyz, seed=5), 1.0 / 39
+    raise KeyError(name)
+
+
+@functools.lru_cache(maxsize=None)
+def oracle_interface(name):
+    m, phi, h = workload(name)
+    ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
+    tris, elems, area = oracle.reconstruct_interface(m.tets, m.t2e, m.edges, m.xyz, phi, ifp, ifp2e)
+    return dict(ifp=ifp, ifp2e=ifp2e, tris=tris, if_elems=elems, ifp_area=area)
+
+
+def assert_interface_bit_exact(got, ref):
+    assert got.ifp.shape == ref["ifp"].shape
+    assert np.array_equal(got.ifp.view(np.uint32), ref["ifp"].view(np.uint32))
+    assert np.array_equal(got.ifp2e, ref["ifp2e"])
+    assert np.array_equal(got.tris, ref["tris"])
+    assert np.array_equal(got.if_elems, ref["if_elems"])
+    assert np.array_equal(got.ifp_area.view(np.uint32), ref["ifp_area"].view(np.uint32))
+    off, p2t = oracle.p2t_lists(ref["tris"], len(ref["ifp"]))
+    assert np.array_equal(got.p2t_offsets, off)
+    assert np.array_equal(got.p2t, p2t)
+
+
+def triangle_errors(name, d_gpu, f_gpu, nodes, h, ref=None, eps_tie=None):
+    """Compares unsigned GPU distances / nearest triangles on `nodes` with the
+    FP64 brute-force oracle.  Returns (max_err, rms_err, n_ties, n_mismatch)."""
+    m, phi, _ = workload(name)
+    itf = ref or oracle_interface(name)
+    d64, f64 = oracle.triangle_distance(m.xyz, itf["ifp"], itf["tris"], nodes)
+    err = np.abs(d_gpu.astype(np.float64) - d64)
+    eps_tie = 1e-5 * h if eps_tie is None else eps_tie
+    ties = mismatch = 0
+    for k in np.nonzero(f_gpu != f64)[0]:
+        dg = oracle.node_triangle_distance(nodes[k], m.xyz, itf["ifp"], itf["tris"], f_gpu[k])
+        if abs(dg - d64[k]) <= eps_tie:
+            ties += 1
+        else:
+            mismatch += 1
+    return float(err.max()), float(np.sqrt((err ** 2).mean())), ties, mismatch

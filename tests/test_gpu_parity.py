@@ -1,0 +1,207 @@
+"""GPU parity: the HIP reinit through the C ABI against the CPU oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import fluidtissue_amd as fa
+import oracle
+from fluidtissue_amd import _lib
+from tests.helpers import (assert_interface_bit_exact, oracle_interface, triangle_errors,
+                           workload)
+
+pytestmark = pytest.mark.gpu
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+@pytest.fixture(scope="module")
+def rein():
+    r = fa.Reinit(0)
+    yield r
+    r.close()
+
+
+def _run(rein, name, **kw):
+    m, phi, h = workload(name)
+    if getattr(rein, "_mesh_name", None) != name:
+        rein.set_mesh(m)
+        rein._mesh_name = name
+    out, ifn, cnt = rein.reset_phi(phi, **kw)
+    return m, phi, h, out, ifn, cnt
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cfg2s", "cfg3s", "ellipsoids"])
+def test_marching_tets_bit_exact(rein, name):
+    m, phi, h, out, ifn, cnt = _run(rein, name)
+    ref = oracle_interface(name)
+    print(f"{name}: P={cnt.n_ifp} F={cnt.n_tri} Tx={cnt.n_if_elems}", flush=True)
+    assert (cnt.n_ifp, cnt.n_tri, cnt.n_if_elems) == (len(ref["ifp"]), len(ref["tris"]),
+                                                      len(ref["if_elems"]))
+    assert_interface_bit_exact(rein.interface(), ref)
+
+
+def test_marching_tets_bit_exact_cfg2(rein):
+    m, phi, h, out, ifn, cnt = _run(rein, "cfg2")
+    assert_interface_bit_exact(rein.interface(), oracle_interface("cfg2"))
+
+
+@pytest.mark.parametrize("name", ["cfg1", "ellipsoids"])
+def test_points_mode_bit_exact_full(rein, name):
+    """Reference semantics (EuclideanDistance.cu:14-35,108-117) on every node."""
+    m, phi, h, out, ifn, cnt = _run(rein, name, mode="points", blend="init", want_nearest=True)
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=0)
+    assert np.array_equal(ifn.view(np.uint64), ref["phi_ifn"].view(np.uint64))
+    assert np.array_equal(out.view(np.uint64), ref["phi"].view(np.uint64))
+    _, near = oracle.reset_phi_points(m.xyz, ref["ifp"], phi, want_nearest=True)
+    assert np.array_equal(rein.nearest(), near)
+    # LevelSet::update blend (LevelSet.cpp:147-148)
+    out_u, ifn_u, _ = rein.reset_phi(phi, mode="points", blend="update")
+    ref_u = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=1)
+    assert np.array_equal(ifn_u.view(np.uint64), ref_u["phi_ifn"].view(np.uint64))
+    assert np.array_equal(out_u.view(np.uint64), ref_u["phi"].view(np.uint64))
+
+
+@pytest.mark.parametrize("name", ["cfg2", "cfg3s"])
+def test_points_mode_bit_exact_sampled(rein, name):
+    m, phi, h, out, ifn, cnt = _run(rein, name, mode="points", want_nearest=True)
+    ref = oracle_interface(name)
+    rng = np.random.default_rng(11)
+    nodes = rng.choice(m.n_nodes, 20000, replace=False)
+    d, j = oracle.point_distance(m.xyz, ref["ifp"], nodes)
+    got = ifn[nodes]
+    assert np.array_equal(np.abs(got).astype(np.float32).view(np.uint32), d.view(np.uint32))
+    assert np.array_equal(np.abs(got), d.astype(np.float64))
+    assert np.array_equal(np.signbit(got), phi[nodes] < 0.0)
+    assert np.array_equal(rein.nearest()[nodes], j)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "ellipsoids"])
+def test_triangle_mode_full(rein, name):
+    m, phi, h, out, ifn, cnt = _run(rein, name, mode="triangles", want_nearest=True)
+    nodes = np.arange(m.n_nodes)
+    emax, erms, ties, mism = triangle_errors(name, np.abs(ifn), rein.nearest(), nodes, h)
+    print(f"{name} triangles: max err {emax:.3e} ({emax / h:.2e} h) rms {erms:.3e}, "
+          f"eps-ties {ties}, mismatches {mism}", flush=True)
+    assert emax <= 1e-5 * h
+    assert mism == 0
+    assert np.array_equal(np.signbit(ifn), phi < 0.0)
+    # |d_tri| <= |d_pt| with the same interface
+    _, ifn_p, _ = rein.reset_phi(phi, mode="points")
+    assert np.all(np.abs(ifn) <= np.abs(ifn_p) * (1 + 1e-6))
+
+
+@pytest.mark.parametrize("name", ["cfg2s", "cfg3s"])
+def test_triangle_mode_sampled(rein, name):
+    m, phi, h, out, ifn, cnt = _run(rein, name, mode="triangles", want_nearest=True)
+    rng = np.random.default_rng(12)
+    nodes = rng.choice(m.n_nodes, 3000, replace=False)
+    emax, erms, ties, mism = triangle_errors(name, np.abs(ifn[nodes]), rein.nearest()[nodes],
+                                             nodes, h)
+    print(f"{name} triangles: max err {emax:.3e} ({emax / h:.2e} h) rms {erms:.3e}, "
+          f"eps-ties {ties}, mismatches {mism}", flush=True)
+    assert emax <= 1e-5 * h
+    assert mism == 0
+
+
+def test_triangle_mode_band(rein):
+    """Narrow band 6h: exact inside, clamped to +-band outside."""
+    name = "cfg2s"
+    m, phi, h, _, _, _ = _run(rein, name)
+    band = 6 * h
+    out, ifn, cnt = rein.reset_phi(phi, mode="triangles", band=band, want_nearest=True)
+    rng = np.random.default_rng(13)
+    nodes = rng.choice(m.n_nodes, 4000, replace=False)
+    itf = oracle_interface(name)
+    d64, f64 = oracle.triangle_distance(m.xyz, itf["ifp"], itf["tris"], nodes)
+    near = rein.nearest()[nodes]
+    got = np.abs(ifn[nodes])
+    inb = d64 < band * (1 - 1e-6)
+    outb = d64 > band * (1 + 1e-6)
+    assert np.all(near[outb] == -1) and np.all(got[outb] == np.float64(np.float32(band)))
+    assert np.max(np.abs(got[inb] - d64[inb])) <= 1e-5 * h
+    assert cnt.n_clamped == int((rein.nearest() == -1).sum())
+    assert cnt.n_clamped > 0
+
+
+def test_euclidean_distance_abi(rein):
+    """EuclideanDistance::init/resetPhi drop-in (host pointers)."""
+    m, phi, h = workload("cfg1")
+    ref = oracle_interface("cfg1")
+    ed = fa.EuclideanDistance(0)
+    ed.init(m.xyz, phi, m.n_nodes, 8)
+    phi_n = phi.copy()
+    ed.reset_phi(ref["ifp"], len(ref["ifp"]), phi_n)
+    ora, _ = oracle.reset_phi_points(m.xyz, ref["ifp"], phi)
+    assert np.array_equal(phi_n.view(np.uint64), ora.view(np.uint64))
+    # no interface at all: +-FLT_MAX like the reference's loop over zero points
+    phi_n = phi.copy()
+    ed.reset_phi(ref["ifp"][:0], 0, phi_n)
+    assert np.all(np.abs(phi_n) == FLT_MAX) and np.array_equal(np.signbit(phi_n), phi < 0)
+    ed.close()
+
+
+def test_no_crossing_gives_flt_max(rein):
+    m, phi, h = workload("cfg1")
+    rein.set_mesh(m)
+    rein._mesh_name = "cfg1"
+    pos = np.abs(phi) + 1.0
+    out, ifn, cnt = rein.reset_phi(pos, mode="points")
+    assert cnt.n_ifp == 0 and cnt.n_tri == 0
+    assert np.all(ifn == FLT_MAX)
+    out, ifn, cnt = rein.reset_phi(-pos, mode="triangles")
+    assert np.all(ifn == -FLT_MAX)
+
+
+def test_malformed_t2e_error(rein):
+    m = fa.kuhn_mesh(8)
+    phi = m.xyz[:, 0].astype(np.float64) - 0.51
+    sb = np.signbit(phi)[m.tets].sum(1)
+    bad = int(np.nonzero((sb > 0) & (sb < 4))[0][5])
+    t2e = m.t2e.copy()
+    t2e[bad] = t2e[bad][0]
+    r = fa.Reinit(0)
+    r.set_mesh(fa.Mesh(m.xyz, m.tets, m.edges, t2e))
+    with pytest.raises(_lib.FtError) as ei:
+        r.reset_phi(phi)
+    assert ei.value.code == _lib.FT_ERR_T2E
+    assert r.counts.bad_tet == bad
+    r.close()
+
+
+def test_degenerate_tet_error(rein):
+    m = fa.kuhn_mesh(5)
+    phi = np.ones(m.n_nodes)
+    t = m.tets[17]
+    phi[t] = [0.0, -0.0, 0.0, 0.0]
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    with pytest.raises(_lib.FtError) as ei:
+        r.reset_phi(phi)
+    assert ei.value.code == _lib.FT_ERR_DEGENERATE
+    r.close()
+
+
+def test_set_mesh_rejects_out_of_range(rein):
+    m = fa.kuhn_mesh(4)
+    tets = m.tets.copy()
+    tets[3, 2] = m.n_nodes
+    r = fa.Reinit(0)
+    with pytest.raises(_lib.FtError) as ei:
+        r.set_mesh(fa.Mesh(m.xyz, tets, m.edges, m.t2e))
+    assert ei.value.code == _lib.FT_ERR_ARG
+    r.close()
+
+
+def test_device_resident_matches_host(rein):
+    import torch
+    m, phi, h, out, ifn, cnt = _run(rein, "cfg2s", mode="triangles")
+    d_phi = torch.from_numpy(phi.copy()).cuda()
+    d_ifn = torch.empty_like(d_phi)
+    torch.cuda.synchronize()
+    for _ in range(2):     # repeated calls reuse the buffers
+        d_phi.copy_(torch.from_numpy(phi))
+        torch.cuda.synchronize()
+        rein.reset_phi_device(d_phi.data_ptr(), d_ifn.data_ptr(), mode="triangles")
+        assert np.array_equal(d_phi.cpu().numpy().view(np.uint64), out.view(np.uint64))
+        assert np.array_equal(d_ifn.cpu().numpy().view(np.uint64), ifn.view(np.uint64))

@@ -1,0 +1,289 @@
+"""Pins the CPU oracle against closed forms, exhaustive tiny cases and
+invariants (the reference ships no tests or golden data).  CPU only."""
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+
+FLT_MAX = float(np.finfo(np.float32).max)
+FLT_MIN = float(np.finfo(np.float32).tiny)
+
+
+# ---------------------------------------------------------------------------
+# FP64 point-triangle distance
+# ---------------------------------------------------------------------------
+TRI = (np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.0, 1, 0]))
+
+
+@pytest.mark.parametrize("x,d2", [
+    ((0.2, 0.3, 0.7), 0.49),        # face interior
+    ((-1, -1, 0), 2.0),             # vertex a
+    ((2, 0, 1), 2.0),               # vertex b
+    ((0, 3, 0), 4.0),               # vertex c
+    ((0.5, -1, 0), 1.0),            # edge ab
+    ((1, 1, 0), 0.5),               # edge bc
+    ((-1, 0.5, 2), 5.0),            # edge ca
+    ((0.25, 0.25, 0), 0.0),         # on the face
+])
+def test_tri_dist_closed_form(x, d2):
+    got = oracle.point_triangle_dist2(np.array(x, float), *TRI)
+    assert got == pytest.approx(d2, rel=1e-15, abs=1e-15)
+
+
+def _sampled_dist(x, a, b, c, n=300):
+    u, v = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))
+    m = (u + v) <= 1.0
+    u, v = u[m], v[m]
+    pts = a[None] + u[:, None] * (b - a)[None] + v[:, None] * (c - a)[None]
+    return np.sqrt(((pts - x[None]) ** 2).sum(1)).min()
+
+
+def test_tri_dist_vs_dense_sampling():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a, b, c = rng.normal(size=(3, 3))
+        x = rng.normal(size=3) * 2
+        d = math.sqrt(oracle.point_triangle_dist2(x, a, b, c))
+        ds = _sampled_dist(x, a, b, c)
+        diam = max(np.linalg.norm(a - b), np.linalg.norm(b - c), np.linalg.norm(c - a))
+        assert d <= ds + 1e-12
+        assert ds - d <= 2.0 * diam / 300
+        # permutations of the vertices give the same distance
+        for p in itertools.permutations((a, b, c)):
+            assert math.sqrt(oracle.point_triangle_dist2(x, *p)) == pytest.approx(d, rel=1e-12, abs=1e-14)
+
+
+def test_tri_dist_degenerate():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        a, d = rng.normal(size=(2, 3))
+        b = a + 0.3 * (d - a)                   # collinear a, b, c (up to rounding)
+        c = d
+        x = rng.normal(size=3)
+        got = math.sqrt(oracle.point_triangle_dist2(x, a, b, c))
+        t = np.linspace(0, 1, 20001)[:, None]
+        ref = np.sqrt((((a + t * (c - a)) - x) ** 2).sum(1)).min()
+        assert abs(got - ref) < 1e-7
+        # all three coincide: point distance
+        assert math.sqrt(oracle.point_triangle_dist2(x, a, a, a)) == pytest.approx(np.linalg.norm(x - a))
+
+
+# ---------------------------------------------------------------------------
+# findInterfaceNodes_ (LevelSet.cpp:261-300)
+# ---------------------------------------------------------------------------
+def _grid(n):
+    import fluidtissue_amd as fa
+    return fa.kuhn_mesh(n)
+
+
+def test_planar_crossings_closed_form():
+    n = 17
+    m = _grid(n)
+    h = 1.0 / (n - 1)
+    a = 7.3 * h                                     # between layers 7 and 8
+    phi = m.xyz[:, 0].astype(np.float64) - a
+    ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
+    # every edge with an x step between the layers crosses: x, xy, xz, xyz edges
+    assert len(ifp) == n * n + 2 * n * (n - 1) + (n - 1) ** 2
+    assert np.all(np.diff(ifp2e) > 0)               # edge-id order (:278)
+    ulp = np.spacing(np.float32(a))
+    assert np.all(np.abs(ifp[:, 0].astype(np.float64) - a) <= 2 * ulp)
+
+
+def test_crossing_point_rounding_exact_rational():
+    rng = np.random.default_rng(3)
+    xyz = rng.random((400, 3)).astype(np.float32)
+    phi = rng.normal(size=400)
+    edges = np.stack([np.arange(0, 400, 2), np.arange(1, 400, 2)], 1).astype(np.int32)
+    ifp, ifp2e = oracle.find_interface_nodes(edges, xyz, phi)
+    assert len(ifp) > 50
+    for k, e in enumerate(ifp2e):
+        i, j = edges[e]
+        t = Fraction(phi[i]) / (Fraction(phi[i]) - Fraction(phi[j]))
+        for c in range(3):
+            exact = (1 - t) * Fraction(float(xyz[i, c])) + t * Fraction(float(xyz[j, c]))
+            got = Fraction(float(ifp[k, c]))
+            assert abs(got - exact) <= Fraction(float(np.spacing(np.float32(abs(float(exact))))))
+
+
+def test_crossing_midpoint_and_signed_zero():
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 2, 0], [0, 4, 0], [0, 0, 8], [0, 0, 9]], np.float32)
+    edges = np.array([[0, 1], [2, 3], [4, 5]], np.int32)
+    # |phi0 - phi1| <= FLT_MIN -> midpoint (:287-290); -0.0 counts as negative (:281)
+    phi = np.array([1e-39, -1e-39, 0.0, -0.0, 0.0, 0.0])
+    ifp, ifp2e = oracle.find_interface_nodes(edges, xyz, phi)
+    assert list(ifp2e) == [0, 1]                    # the +0/+0 edge does not cross
+    assert np.array_equal(ifp, np.array([[0.5, 0, 0], [0, 3, 0]], np.float32))
+
+
+# ---------------------------------------------------------------------------
+# reconstructInterface_ (LevelSet.cpp:326-425)
+# ---------------------------------------------------------------------------
+TET_XYZ = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 1, 1]], np.float32)
+LOCAL = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+
+
+@pytest.mark.parametrize("pattern", range(16))
+def test_single_tet_all_sign_patterns(pattern):
+    neg = [(pattern >> k) & 1 for k in range(4)]
+    phi = np.array([-(0.3 + 0.2 * k) if neg[k] else (0.4 + 0.1 * k) for k in range(4)])
+    tets = np.array([[0, 1, 2, 3]], np.int32)
+    edges = np.array(LOCAL, np.int32)
+    t2e = np.arange(6, dtype=np.int32)[None]
+    ifp, ifp2e = oracle.find_interface_nodes(edges, TET_XYZ, phi)
+    tris, elems, area = oracle.reconstruct_interface(tets, t2e, edges, TET_XYZ, phi, ifp, ifp2e)
+    s = sum(neg)
+    assert len(ifp) == {0: 0, 1: 3, 2: 4, 3: 3, 4: 0}[s]
+    assert len(tris) == {0: 0, 1: 1, 2: 2, 3: 1, 4: 0}[s]
+    assert list(elems) == ([0] if 0 < s < 4 else [])
+    if s == 2:
+        # 4 crossings e0<e1<e2<e3 (t2e order): triangles (e0,e2,e3), (e0,e1,e3) (:397-398)
+        assert sorted(tris[0][:3]) == [0, 2, 3] and sorted(tris[1][:3]) == [0, 1, 3]
+    for t in tris:
+        assert t[3] == -1 and t[0] == min(t[0], t[0])
+        q = ifp[t[:3]].astype(np.float64)
+        n = np.cross(q[1] - q[0], q[2] - q[0])
+        cen = q.mean(0)
+        for k in range(4):                              # normal points toward phi > 0
+            side = np.dot(n, TET_XYZ[k] - cen)
+            assert (side > 0) == (phi[k] > 0)
+    if len(tris):
+        tri_area = [0.5 * np.linalg.norm(np.cross(*(ifp[t[1:3]] - ifp[t[0]]).astype(np.float64)))
+                    for t in tris]
+        assert area.sum() == pytest.approx(3 * sum(tri_area), rel=1e-6)
+
+
+def _sphere_case(n=40, r=0.3):
+    m = _grid(n)
+    c = np.array([0.5, 0.5, 0.5])
+    phi = np.linalg.norm(m.xyz.astype(np.float64) - c, axis=1) - r
+    ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
+    tris, elems, area = oracle.reconstruct_interface(m.tets, m.t2e, m.edges, m.xyz, phi, ifp, ifp2e)
+    return m, phi, ifp, ifp2e, tris, elems, area, r
+
+
+def test_closed_surface_invariants():
+    m, phi, ifp, ifp2e, tris, elems, area, r = _sphere_case()
+    T = tris[:, :3].astype(np.int64)
+    directed = np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]])
+    und = np.sort(directed, axis=1)
+    key = und[:, 0] * len(ifp) + und[:, 1]
+    _, cnt = np.unique(key, return_counts=True)
+    assert np.all(cnt == 2)                                 # watertight
+    dkey = directed[:, 0] * len(ifp) + directed[:, 1]
+    assert len(np.unique(dkey)) == len(dkey)                # consistent orientation
+    V, E, F = len(ifp), len(cnt), len(tris)
+    assert V - E + F == 2                                   # sphere topology
+    q = ifp[T].astype(np.float64)
+    vol = np.einsum("ij,ij->i", q[:, 0], np.cross(q[:, 1], q[:, 2])).sum() / 6.0
+    assert vol == pytest.approx(4.0 / 3.0 * math.pi * r ** 3, rel=0.02)   # outward normals
+    tri_area = 0.5 * np.linalg.norm(np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0]), axis=1)
+    assert tri_area.sum() == pytest.approx(4 * math.pi * r * r, rel=0.03)
+    assert area.sum() == pytest.approx(3 * tri_area.sum(), rel=1e-5)
+    # crossing tets: exactly those with mixed node signs, in increasing order
+    sb = np.signbit(phi)[m.tets].sum(1)
+    assert np.array_equal(elems, np.nonzero((sb > 0) & (sb < 4))[0])
+
+
+def test_area_sum_order_matches_p2t():
+    m, phi, ifp, ifp2e, tris, elems, area, r = _sphere_case(24, 0.31)
+    off, p2t = oracle.p2t_lists(tris, len(ifp))
+    q = ifp[tris[:, :3]].astype(np.float32)
+    e1, e2 = q[:, 1] - q[:, 0], q[:, 2] - q[:, 0]
+    # vertex order of each stored triangle may be swapped; the norm is invariant
+    nrm = np.sqrt((np.cross(e1.astype(np.float64), e2.astype(np.float64)) ** 2).sum(1))
+    ref = np.array([0.5 * nrm[p2t[off[v]:off[v + 1]]].sum() for v in range(len(ifp))])
+    assert np.allclose(area, ref, rtol=1e-5)
+
+
+def test_malformed_t2e_fails_like_reference():
+    m = _grid(6)
+    phi = m.xyz[:, 0].astype(np.float64) - 0.51
+    ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
+    t2e = m.t2e.copy()
+    sb = np.signbit(phi)[m.tets].sum(1)
+    bad = int(np.nonzero((sb > 0) & (sb < 4))[0][3])
+    t2e[bad] = t2e[bad][0]                      # all six slots -> one edge
+    with pytest.raises(oracle.ReconstructError) as ei:
+        oracle.reconstruct_interface(m.tets, t2e, m.edges, m.xyz, phi, ifp, ifp2e)
+    assert ei.value.code == -1 and ei.value.tet == bad
+
+
+def test_all_zero_crossing_tet_is_an_error():
+    tets = np.array([[0, 1, 2, 3]], np.int32)
+    edges = np.array(LOCAL, np.int32)
+    t2e = np.arange(6, dtype=np.int32)[None]
+    phi = np.array([0.0, -0.0, 0.0, 0.0])
+    ifp, ifp2e = oracle.find_interface_nodes(edges, TET_XYZ, phi)
+    with pytest.raises(oracle.ReconstructError) as ei:
+        oracle.reconstruct_interface(tets, t2e, edges, TET_XYZ, phi, ifp, ifp2e)
+    assert ei.value.code == -2
+
+
+# ---------------------------------------------------------------------------
+# EuclideanDistance::resetPhi CPU branch (EuclideanDistance.cu:14-35, 108-117)
+# ---------------------------------------------------------------------------
+def test_point_reset_vs_fp64_and_sign_rule():
+    rng = np.random.default_rng(4)
+    xyz = rng.random((500, 3)).astype(np.float32)
+    ifp = rng.random((80, 3)).astype(np.float32)
+    phi = rng.normal(size=500)
+    phi[:5] = [-0.0, 0.0, -1e-300, 1e-300, -5.0]
+    out, near = oracle.reset_phi_points(xyz, ifp, phi, want_nearest=True)
+    d64 = np.linalg.norm(xyz[:, None, :].astype(np.float64) - ifp[None].astype(np.float64), axis=2)
+    assert np.allclose(np.abs(out), d64.min(1), rtol=3e-7)
+    assert np.array_equal(near, d64.argmin(1))
+    assert np.array_equal(np.signbit(out), phi < 0.0)        # -0.0 -> positive (:113)
+
+
+def test_point_reset_ties_lowest_index_and_empty():
+    xyz = np.array([[0, 0, 0], [1, 1, 1]], np.float32)
+    ifp = np.array([[1, 0, 0], [0, 1, 0], [1, 0, 0], [1, 1, 1]], np.float32)
+    out, near = oracle.reset_phi_points(xyz, ifp, np.array([-1.0, 1.0]), want_nearest=True)
+    assert near[0] == 0 and out[0] == -1.0                  # strict '<' keeps the first
+    assert near[1] == 3 and out[1] == 0.0
+    out, near = oracle.reset_phi_points(xyz, ifp[:0], np.array([-1.0, 1.0]), want_nearest=True)
+    assert list(out) == [-FLT_MAX, FLT_MAX] and list(near) == [-1, -1]
+
+
+def test_finish_reset_blend_closed_form():
+    phi_old = np.array([1.0, -2.0, 3.0, -4.0, 5.0])
+    ifn = np.array([0.5, -0.25, 0.125, -1.0, 2.0])
+    edges = np.array([[0, 1], [3, 4]], np.int32)
+    ret, out = oracle.finish_reset(phi_old, ifn, np.array([0], np.int32), edges, 1)
+    assert list(ret) == [1.0, -2.0, 0.125, -1.0, 2.0]
+    assert list(out) == [0.75 * 1.0 + 0.25 * 0.5, 0.75 * -2.0 + 0.25 * -0.25,
+                         0.125, -1.0, 2.0]
+    ret, out = oracle.finish_reset(phi_old, ifn, np.array([0], np.int32), edges, 0)
+    assert list(out) == list(ifn)
+
+
+# ---------------------------------------------------------------------------
+# Triangle mode invariants (FP64 oracle)
+# ---------------------------------------------------------------------------
+def test_triangle_mode_invariants_sphere():
+    m, phi, ifp, ifp2e, tris, elems, area, r = _sphere_case(28, 0.27)
+    rng = np.random.default_rng(5)
+    nodes = rng.choice(m.n_nodes, 1500, replace=False)
+    dt, ft = oracle.triangle_distance(m.xyz, ifp, tris, nodes)
+    dp, _ = oracle.point_distance(m.xyz, ifp, nodes)
+    assert np.all(dt <= dp.astype(np.float64) * (1 + 1e-6) + 1e-12)     # |d_tri| <= |d_pt|
+    h = 1.0 / 27
+    exact = np.abs(np.linalg.norm(m.xyz[nodes].astype(np.float64) - 0.5, axis=1) - r)
+    near = exact < 3 * h
+    assert np.max(np.abs(dt[near] - exact[near])) < 0.5 * h * h / r + 1e-3 * h
+    # the chosen triangle realises the distance
+    for k in range(0, 1500, 97):
+        assert oracle.node_triangle_distance(nodes[k], m.xyz, ifp, tris, ft[k]) == dt[k]
+
+
+def test_triangle_mode_lipschitz_along_edges():
+    m, phi, ifp, ifp2e, tris, elems, area, r = _sphere_case(20, 0.3)
+    d, _ = oracle.triangle_distance(m.xyz, ifp, tris)
+    e = m.edges
+    le = np.linalg.norm(m.xyz[e[:, 0]].astype(np.float64) - m.xyz[e[:, 1]], axis=1)
+    assert np.all(np.abs(d[e[:, 0]] - d[e[:, 1]]) <= le * (1 + 1e-12))
