@@ -97,9 +97,8 @@ def main():
                                   bin_scale=args.bin_scale, stream=stream)
 
     for _ in range(args.warmup):
-        cnt = step()
+        step()
     torch.cuda.synchronize()
-    log(f"interface: P={cnt.n_ifp} F={cnt.n_tri} Tx={cnt.n_if_elems}")
     r.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -108,6 +107,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     phase_ms, calls = r.timing()
+    log(f"interface: P={cnt.n_ifp} F={cnt.n_tri} Tx={cnt.n_if_elems}")
     ms = (t1 - t0) * 1e3 / args.steps
     N, T = m.n_nodes, len(m.tets)
     P, F, X = cnt.n_ifp, cnt.n_tri, cnt.n_if_elems

@@ -59,6 +59,16 @@ class FtError(RuntimeError):
         self.code = code
 
 
+# PyTorch ships its own HIP runtime under the same SONAME (libamdhip64.so.7)
+# as /opt/rocm's.  Loading torch first makes this library bind to that one
+# runtime, so device pointers from torch tensors and from this library live
+# in a single HIP/HSA context; loading it second would put two runtimes in
+# the process and torch then finds no GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
 

@@ -23,6 +23,21 @@ __device__ __forceinline__ int get_bit(const uint64_t* bits, int64_t i)
     return (int)((bits[i >> 6] >> (i & 63)) & 1ull);
 }
 
+// Load of element i of a read-only array through the constant address space:
+// at a wave-uniform address this compiles to scalar (SMEM) loads into SGPRs.
+template <class T>
+__device__ __forceinline__ T cload(const T* p, int64_t i)
+{
+    static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
+    T v;
+    unsigned* d = reinterpret_cast<unsigned*>(&v);
+    const __attribute__((address_space(4))) unsigned* src =
+        (const __attribute__((address_space(4))) unsigned*)(p + i);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) d[k] = src[k];
+    return v;
+}
+
 // ---- scans ----------------------------------------------------------------
 __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d, int width = kWave)
 {
@@ -122,6 +137,39 @@ __device__ __forceinline__ float box_dist2(float x, float y, float z, const floa
     return __fmaf_rn(dz, dz, __fmaf_rn(dy, dy, dx * dx));
 }
 
+// ---- bounding volumes --------------------------------------------------------
+// An axis-aligned box plus a cylinder (centre c, unit axis n, radius rho,
+// half-thickness tau) that both contain every primitive below a tree node.
+// A patch of a smooth surface lies in a thin cylinder around its normal, so
+// for a node far away the cylinder bound is close to the true distance where
+// the box bound is loose by about the patch size.  tau < 0: no cylinder.
+struct Bound {
+    float lo[3], hi[3];
+    float c[3], n[3];
+    float rho, tau;
+    float pad[2];
+};
+
+__device__ __forceinline__ float cyl_dist2(float x, float y, float z, const Bound& b)
+{
+    if (b.tau < 0.0f) return 0.0f;
+    const float dx = x - b.c[0], dy = y - b.c[1], dz = z - b.c[2];
+    const float hn = __fmaf_rn(dz, b.n[2], __fmaf_rn(dy, b.n[1], dx * b.n[0]));
+    const float cx = __fmaf_rn(b.n[1], dz, -b.n[2] * dy);
+    const float cy = __fmaf_rn(b.n[2], dx, -b.n[0] * dz);
+    const float cz = __fmaf_rn(b.n[0], dy, -b.n[1] * dx);
+    const float lat = sqrtf(__fmaf_rn(cz, cz, __fmaf_rn(cy, cy, cx * cx)));
+    const float slack = 0x1.0p-20f * (fabsf(hn) + lat);     // rounding of hn and lat
+    const float a = fmaxf(fabsf(hn) - b.tau - slack, 0.0f);
+    const float r = fmaxf(lat - b.rho - slack, 0.0f);
+    return __fmaf_rn(r, r, a * a);
+}
+
+__device__ __forceinline__ float bound_dist2(float x, float y, float z, const Bound& b)
+{
+    return fmaxf(box_dist2(x, y, z, b.lo, b.hi), cyl_dist2(x, y, z, b));
+}
+
 // ---- FP64 point-triangle distance ------------------------------------------
 // The closest point of a triangle is the orthogonal projection onto its plane
 // when that falls inside (edge functions n.(e_i x (x - v_i)) >= 0), else the
@@ -198,8 +246,8 @@ __device__ __forceinline__ float tri_dist2_f32(float x, float y, float z, const 
     const float Z = __fmaf_rn(wz, r.F3.z, __fmaf_rn(wy, r.F3.y, wx * r.F3.x));
     const float B = r.F0.w, Cx = r.F1.w, Cy = r.F2.w, dCx = r.F4.z;
     const float xb = X - B;
-    const bool inside = (Y >= 0.0f) & (__fmaf_rn(Cy, X, -Cx * Y) >= 0.0f) &
-                        (__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f);
+    const bool inside = (int)(Y >= 0.0f) & (int)(__fmaf_rn(Cy, X, -Cx * Y) >= 0.0f) &
+                        (int)(__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f);
     // edge ab
     float t = fminf(fmaxf(X * r.F3.w, 0.0f), 1.0f);
     float ex = __fmaf_rn(-t, B, X);

@@ -34,6 +34,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -44,12 +46,13 @@
 
 using ftd::Grid;
 using ftd::TriRec;
+using ftd::Bound;
 
 namespace {
 
 constexpr int kBS = 256;                 // threads per block of the streaming kernels
 constexpr int kScanTile = 2048;          // elements per scan block (256 x 8)
-constexpr int kStack = 32;               // traversal stack depth (tree depth <= 24)
+constexpr int kStack = 24;               // traversal stack depth (tree depth <= 24)
 constexpr int kQueryBS = 256;
 constexpr int kMaxBinsLog2PerAxis = 8;   // <= 256 bins per axis, 2^24 Morton cells
 
@@ -455,9 +458,8 @@ __global__ __launch_bounds__(kBS) void k_point_area(const uint32_t* off, int64_t
 // ============================================================================
 // Spatial index: uniform bins by counting sort, binary radix tree on bins
 // ============================================================================
-struct __align__(16) BNode {  // 64 B: child boxes, children, parent
-    float lo_l[3], hi_l[3];
-    float lo_r[3], hi_r[3];
+struct __align__(16) BNode {  // child bounds, children, parent
+    Bound l, r;
     int32_t left, right, parent, pad;
 };
 
@@ -561,14 +563,16 @@ __global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, int64_t n, const
     trec[slot] = r;
 }
 
-__global__ __launch_bounds__(kBS) void k_leaf_boxes(int mode, int64_t L, const uint32_t* leaf_start,
-                                                    const uint32_t* leaf_cnt, const int32_t* pidx,
-                                                    const float* ifp, const int4* tris,
-                                                    float4* leaf_lo, float4* leaf_hi)
+// Leaf bounds: box of the vertices and, for triangles, a cylinder about the
+// area-weighted mean normal of the leaf's triangles.
+__global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, int64_t L, const uint32_t* leaf_start,
+                                                     const uint32_t* leaf_cnt, const int32_t* pidx,
+                                                     const float* ifp, const int4* tris, Bound* leaf_b)
 {
     const int64_t l = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (l >= L) return;
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    double ns[3] = {0.0, 0.0, 0.0};
     const uint32_t b = leaf_start[l], e = b + leaf_cnt[l];
     for (uint32_t s = b; s < e; s++) {
         const int32_t j = pidx[s];
@@ -578,16 +582,44 @@ __global__ __launch_bounds__(kBS) void k_leaf_boxes(int mode, int64_t L, const u
                 lo[k] = fminf(lo[k], v); hi[k] = fmaxf(hi[k], v);
             }
         } else {
-            float a[3], bb[3], c[3];
-            tri_vertices(tris, ifp, j, a, bb, c);
+            float va[3], vb[3], vc[3];
+            tri_vertices(tris, ifp, j, va, vb, vc);
             for (int k = 0; k < 3; k++) {
-                lo[k] = fminf(lo[k], fminf(a[k], fminf(bb[k], c[k])));
-                hi[k] = fmaxf(hi[k], fmaxf(a[k], fmaxf(bb[k], c[k])));
+                lo[k] = fminf(lo[k], fminf(va[k], fminf(vb[k], vc[k])));
+                hi[k] = fmaxf(hi[k], fmaxf(va[k], fmaxf(vb[k], vc[k])));
             }
+            double e0[3], e1[3], nn[3];
+            for (int k = 0; k < 3; k++) { e0[k] = (double)vb[k] - va[k]; e1[k] = (double)vc[k] - va[k]; }
+            ftd::cross3d(e0, e1, nn);
+            for (int k = 0; k < 3; k++) ns[k] += nn[k];
         }
     }
-    leaf_lo[l] = make_float4(lo[0], lo[1], lo[2], 0.0f);
-    leaf_hi[l] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    Bound B;
+    for (int k = 0; k < 3; k++) { B.lo[k] = lo[k]; B.hi[k] = hi[k]; B.c[k] = 0.5f * (lo[k] + hi[k]); }
+    B.pad[0] = B.pad[1] = 0.0f;
+    const double nl = sqrt(ftd::dot3d(ns, ns));
+    if (mode == 0 || !(nl > 0.0)) {
+        B.n[0] = 0.0f; B.n[1] = 0.0f; B.n[2] = 1.0f;
+        B.rho = 0.0f; B.tau = -1.0f;                 // box only
+    } else {
+        for (int k = 0; k < 3; k++) B.n[k] = (float)(ns[k] / nl);
+        double tau = 0.0, rho = 0.0;
+        const double n[3] = {B.n[0], B.n[1], B.n[2]};
+        for (uint32_t s = b; s < e; s++) {
+            float v[3][3];
+            tri_vertices(tris, ifp, pidx[s], v[0], v[1], v[2]);
+            for (int q = 0; q < 3; q++) {
+                double d[3], cr[3];
+                for (int k = 0; k < 3; k++) d[k] = (double)v[q][k] - B.c[k];
+                ftd::cross3d(n, d, cr);
+                tau = fmax(tau, fabs(ftd::dot3d(n, d)));
+                rho = fmax(rho, sqrt(ftd::dot3d(cr, cr)));
+            }
+        }
+        B.tau = (float)(tau * (1.0 + 1e-6));
+        B.rho = (float)(rho * (1.0 + 1e-6));
+    }
+    leaf_b[l] = B;
 }
 
 // Binary radix tree over L >= 2 sorted, distinct keys (T. Karras, "Maximizing
@@ -629,42 +661,86 @@ __global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, int L, BNo
     if (i == 0) nodes[0].parent = -1;
 }
 
-__device__ __forceinline__ float ld_acq(const float* p)
+__device__ __forceinline__ void st_bound(Bound* dst, const Bound& v)
 {
-    return __uint_as_float(__hip_atomic_load((const unsigned*)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
+    const unsigned* src = reinterpret_cast<const unsigned*>(&v);
+    unsigned* d = reinterpret_cast<unsigned*>(dst);
+    for (int k = 0; k < 16; k++)
+        __hip_atomic_store(&d[k], src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Bottom-up box fitting: the second child to arrive at a node merges both.
+__device__ __forceinline__ Bound ld_bound(const Bound* srcp)
+{
+    Bound v;
+    const unsigned* s = reinterpret_cast<const unsigned*>(srcp);
+    unsigned* d = reinterpret_cast<unsigned*>(&v);
+    for (int k = 0; k < 16; k++)
+        d[k] = __hip_atomic_load(&s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+
+// Parent bound of two children: box union and a cylinder that contains both
+// child cylinders.  For a child cylinder (c_i, n_i, rho_i, tau_i) seen from
+// the parent axis n at centre c, with cos = |n.n_i| and sin = sqrt(1-cos^2):
+//   thickness  |n.(c_i-c)| + tau_i cos + rho_i sin
+//   radius     |n x (c_i-c)| + rho_i + tau_i sin
+__device__ inline Bound merge_bounds(const Bound& A, const Bound& Bb)
+{
+    Bound P;
+    for (int k = 0; k < 3; k++) {
+        P.lo[k] = fminf(A.lo[k], Bb.lo[k]);
+        P.hi[k] = fmaxf(A.hi[k], Bb.hi[k]);
+        P.c[k] = 0.5f * (P.lo[k] + P.hi[k]);
+    }
+    P.pad[0] = P.pad[1] = 0.0f;
+    if (A.tau < 0.0f || Bb.tau < 0.0f) {
+        P.n[0] = 0.0f; P.n[1] = 0.0f; P.n[2] = 1.0f;
+        P.rho = 0.0f; P.tau = -1.0f;
+        return P;
+    }
+    const double wa = (double)A.rho * A.rho + 1e-30, wb = (double)Bb.rho * Bb.rho + 1e-30;
+    double n[3];
+    for (int k = 0; k < 3; k++) n[k] = wa * A.n[k] + wb * Bb.n[k];
+    double nl = sqrt(ftd::dot3d(n, n));
+    if (!(nl > 1e-12 * (wa + wb))) { for (int k = 0; k < 3; k++) n[k] = A.n[k]; nl = 1.0; }
+    for (int k = 0; k < 3; k++) n[k] /= nl;
+    double tau = 0.0, rho = 0.0;
+    const Bound* ch[2] = {&A, &Bb};
+    for (int i = 0; i < 2; i++) {
+        const Bound& C = *ch[i];
+        double dc[3], ni[3], cr[3];
+        for (int k = 0; k < 3; k++) { dc[k] = (double)C.c[k] - P.c[k]; ni[k] = C.n[k]; }
+        const double cs = fmin(1.0, fabs(ftd::dot3d(n, ni)));
+        const double sn = sqrt(fmax(0.0, 1.0 - cs * cs));
+        ftd::cross3d(n, dc, cr);
+        tau = fmax(tau, fabs(ftd::dot3d(n, dc)) + C.tau * cs + C.rho * sn);
+        rho = fmax(rho, sqrt(ftd::dot3d(cr, cr)) + C.rho + C.tau * sn);
+    }
+    for (int k = 0; k < 3; k++) P.n[k] = (float)n[k];
+    P.tau = (float)(tau * (1.0 + 1e-6));
+    P.rho = (float)(rho * (1.0 + 1e-6));
+    return P;
+}
+
+// Bottom-up fitting: the second child to arrive at a node merges both.
 __global__ __launch_bounds__(kBS) void k_refit(int L, const int32_t* leaf_parent,
-                                               const float4* leaf_lo, const float4* leaf_hi,
-                                               BNode* nodes, unsigned* flags)
+                                               const Bound* leaf_b, BNode* nodes, unsigned* flags)
 {
     const int l = blockIdx.x * kBS + threadIdx.x;
     if (l >= L) return;
-    float lo[3] = {leaf_lo[l].x, leaf_lo[l].y, leaf_lo[l].z};
-    float hi[3] = {leaf_hi[l].x, leaf_hi[l].y, leaf_hi[l].z};
+    Bound cur_b = leaf_b[l];
     int cur = ~l;
     int p = leaf_parent[l];
     for (int depth = 0; p >= 0 && depth < 64; depth++) {   // depth <= 24 for 24-bit keys
         BNode* nd = &nodes[p];
         const bool is_left = (nd->left == cur);
-        float* dlo = is_left ? nd->lo_l : nd->lo_r;
-        float* dhi = is_left ? nd->hi_l : nd->hi_r;
-        for (int k = 0; k < 3; k++) {
-            __hip_atomic_store((unsigned*)&dlo[k], __float_as_uint(lo[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store((unsigned*)&dhi[k], __float_as_uint(hi[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        st_bound(is_left ? &nd->l : &nd->r, cur_b);
         __threadfence();
         const unsigned old = atomicAdd(&flags[p], 1u);
         if (old == 0) return;                       // sibling not done yet
         __threadfence();
-        const float* slo = is_left ? nd->lo_r : nd->lo_l;
-        const float* shi = is_left ? nd->hi_r : nd->hi_l;
-        for (int k = 0; k < 3; k++) {
-            lo[k] = fminf(lo[k], ld_acq(&slo[k]));
-            hi[k] = fmaxf(hi[k], ld_acq(&shi[k]));
-        }
+        const Bound sib = ld_bound(is_left ? &nd->r : &nd->l);
+        cur_b = is_left ? merge_bounds(cur_b, sib) : merge_bounds(sib, cur_b);
         cur = p;
         p = __hip_atomic_load(&nd->parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -684,8 +760,7 @@ struct QueryArgs {
     const int32_t* pidx;             // sorted slot -> primitive id
     const uint32_t* leaf_start;
     const uint32_t* leaf_cnt;
-    const float4* leaf_lo;
-    const float4* leaf_hi;
+    const Bound* leaf_b;
     const BNode* nodes;
     int L;                           // leaves
     float band;                      // > 0: narrow band
@@ -700,6 +775,11 @@ struct QueryArgs {
     double w_phi, w_ifn;
     unsigned long long* n_clamped;
     unsigned* overflow;
+    unsigned long long* stats;       // nullable: [tree nodes, leaves, primitives]
+    int rep;                         // 1: seeding pass, one query per 64-node tile
+    int32_t* seed_out;               // seeding pass: best slot of each tile's query
+    const int32_t* seed_in;          // nullable: per-tile candidate tested first
+    float h_est;                     // mean node spacing (statistics only)
 };
 
 // Margin of the box prune: a box is skipped only if its lower bound exceeds
@@ -729,21 +809,51 @@ struct Searcher {
     {
         if (d < best || (d == best && d < INFINITY)) {
             const int32_t j = pidx[s];
-            if (d < best || bj < 0 || j < bj) { best = d; bj = j; }
+            if (d < best || bj < 0 || j < bj) { best = d; bj = j; bslot = (int32_t)s; }
         }
     }
 
-    __device__ void test_leaf(int l)
+    uint32_t n_leaves = 0, n_prims = 0;
+    int32_t bslot = -1;
+
+    __device__ void test_prim(uint32_t s)
     {
-        const uint32_t b = leaf_start[l], e = b + leaf_cnt[l];
+        if (MODE == 0) {
+            const float4 p = prec[s];
+            const float dx = p.x - x, dy = p.y - y, dz = p.z - z;
+            const float d = sqrtf(dx * dx + dy * dy + dz * dz);   // EuclideanDistance.cu:20-28
+            consider(d, s);
+        } else {
+            const TriRec r = trec[s];
+            float d2;
+            if (r.F4.w >= 0.0f) {
+                d2 = ftd::tri_dist2_f32(x, y, z, r);
+            } else {
+                float va[3], vb[3], vc[3];
+                tri_vertices(tris, ifp, pidx[s], va, vb, vc);
+                const double X[3] = {x, y, z};
+                const double A[3] = {va[0], va[1], va[2]}, B[3] = {vb[0], vb[1], vb[2]},
+                             C[3] = {vc[0], vc[1], vc[2]};
+                d2 = (float)ftd::tri_dist2_f64(X, A, B, C);
+            }
+            consider(d2, s);
+        }
+    }
+
+    // Leaf test at a wave-uniform leaf: records read through scalar loads.
+    __device__ void test_leaf_uniform(int l)
+    {
+        const uint32_t b = ftd::cload(leaf_start, l), e = b + ftd::cload(leaf_cnt, l);
+        n_leaves++;
+        n_prims += e - b;
         for (uint32_t s = b; s < e; s++) {
             if (MODE == 0) {
-                const float4 p = prec[s];
+                const float4 p = ftd::cload(prec, s);
                 const float dx = p.x - x, dy = p.y - y, dz = p.z - z;
                 const float d = sqrtf(dx * dx + dy * dy + dz * dz);   // EuclideanDistance.cu:20-28
                 consider(d, s);
             } else {
-                const TriRec r = trec[s];
+                const TriRec r = ftd::cload(trec, s);
                 float d2;
                 if (r.F4.w >= 0.0f) {
                     d2 = ftd::tri_dist2_f32(x, y, z, r);
@@ -759,16 +869,45 @@ struct Searcher {
             }
         }
     }
+
+    __device__ void test_leaf(int l)
+    {
+        const uint32_t b = leaf_start[l], e = b + leaf_cnt[l];
+        n_leaves++;
+        n_prims += e - b;
+        for (uint32_t s = b; s < e; s++) test_prim(s);
+    }
 };
 
+// Epilogue of one node (EuclideanDistance.cu:54-57, LevelSet.cpp:438-448, 74,
+// 148): sign from phi_old < 0, retained value at crossing-edge nodes, blend.
 template <int MODE>
-__global__ __launch_bounds__(kQueryBS) void k_query(QueryArgs a)
+__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t i)
 {
-    __shared__ int32_t stack_lds[kStack][kQueryBS];
-    const int64_t i = (int64_t)blockIdx.x * kQueryBS + threadIdx.x;
-    if (i >= a.n) return;
-    const float px = a.sx[i], py = a.sy[i], pz = a.sz[i];
-    Searcher<MODE> S;
+    float d;
+    if (MODE == 0) d = S.best;
+    else d = S.bj >= 0 ? sqrtf(S.best) : (a.band > 0.0f ? a.band : FLT_MAX);
+    if (a.band > 0.0f && S.bj < 0) {
+        d = a.band;
+        atomicAdd(a.n_clamped, 1ull);
+    }
+    const int64_t node = a.perm[i];
+    const double po = a.phi_old[node];
+    const double dn = (po < 0.0) ? -(double)d : (double)d;
+    if (a.phi_ifn) a.phi_ifn[node] = dn;
+    if (a.nearest) a.nearest[node] = S.bj;
+    double out = dn;
+    if (a.epi == EPI_UPDATE) {
+        const double kept = (a.retain && ftd::get_bit(a.retain, node)) ? po : dn;
+        out = a.w_phi * kept + a.w_ifn * dn;
+    }
+    a.phi_out[node] = out;
+}
+
+template <int MODE>
+__device__ __forceinline__ void init_searcher(Searcher<MODE>& S, const QueryArgs& a, float px,
+                                              float py, float pz)
+{
     S.x = px; S.y = py; S.z = pz;
     S.bj = -1;
     if (MODE == 0) S.best = a.band > 0.0f ? a.band : FLT_MAX;
@@ -776,20 +915,119 @@ __global__ __launch_bounds__(kQueryBS) void k_query(QueryArgs a)
     S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
     S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
     S.ifp = a.ifp; S.tris = a.tris;
+}
+
+template <int MODE>
+__device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<MODE>& S,
+                                             int64_t i, uint32_t n_nodes)
+{
+    // buckets: distance < 2h, < 8h, < 32h, beyond; +4 when phi < 0
+    const float dd = MODE == 0 ? S.best : sqrtf(S.best);
+    const float hh = a.h_est;
+    int bkt = dd < 2 * hh ? 0 : dd < 8 * hh ? 1 : dd < 32 * hh ? 2 : 3;
+    if (a.phi_old[a.perm[i]] < 0.0) bkt += 4;
+    unsigned long long* st = a.stats + 4 * bkt;
+    atomicAdd(&st[0], 1ull);
+    atomicAdd(&st[1], (unsigned long long)n_nodes);
+    atomicAdd(&st[2], (unsigned long long)S.n_leaves);
+    atomicAdd(&st[3], (unsigned long long)S.n_prims);
+}
+
+// Wave-coherent search: the 64 lanes of a wave hold 64 consecutive nodes in
+// Morton order (a compact tile) and walk the tree together.  A child is
+// entered when any lane's bound admits it; each lane keeps its own best.  All
+// control flow is wave-uniform (ballots), node and primitive records are read
+// at wave-uniform addresses, the stack is one per wave in LDS.
+template <int MODE>
+__global__ __launch_bounds__(kQueryBS) void k_query_wave(QueryArgs a)
+{
+    __shared__ int32_t wstack[kQueryBS / 64][kStack];
+    const int w = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kQueryBS + (threadIdx.x & ~63);
+    if (base >= a.n) return;                          // whole wave out of range
+    const int64_t gid = base + (threadIdx.x & 63);
+    const int64_t i = gid < a.n ? gid : a.n - 1;      // tail lanes shadow the last node
+    const float px = a.sx[i], py = a.sy[i], pz = a.sz[i];
+    Searcher<MODE> S;
+    init_searcher<MODE>(S, a, px, py, pz);
+    if (a.seed_in) {
+        const int32_t s0 = a.seed_in[base >> 6];
+        if (s0 >= 0) S.test_prim((uint32_t)s0);
+    }
+    uint32_t n_nodes = 0;
+    if (a.L == 1) {
+        S.test_leaf_uniform(0);
+    } else if (a.L > 1) {
+        int sp = 0;
+        int cur = 0;
+        int budget = 2 * a.L + 8;
+        while (--budget > 0) {
+            const BNode nd = ftd::cload(a.nodes, cur);    // scalar loads: cur is uniform
+            const int left = nd.left, right = nd.right;
+            const float ll = ftd::bound_dist2(px, py, pz, nd.l);
+            const float lr = ftd::bound_dist2(px, py, pz, nd.r);
+            n_nodes++;
+            const float bnd = prune_bound(S.best2());
+            const uint64_t mL = __ballot(ll <= bnd), mR = __ballot(lr <= bnd);
+            const uint64_t any = mL | mR;
+            const bool left_first = 2 * __popcll(__ballot(ll <= lr) & any) >= __popcll(any);
+            const int c0 = left_first ? left : right, c1 = left_first ? right : left;
+            const uint64_t m0 = left_first ? mL : mR, m1 = left_first ? mR : mL;
+            int next = -1;
+            if (m0) {
+                if (c0 < 0) S.test_leaf_uniform(~c0);
+                else next = c0;
+            }
+            if (m1) {
+                if (c1 < 0) S.test_leaf_uniform(~c1);
+                else if (next < 0) next = c1;
+                else if (sp < kStack) wstack[w][sp++] = c1;
+                else atomicOr(a.overflow, 1u);
+            }
+            if (next < 0) {
+                if (sp == 0) break;
+                next = wstack[w][--sp];
+            }
+            cur = __builtin_amdgcn_readfirstlane(next);
+        }
+        if (budget <= 0) atomicOr(a.overflow, 2u);
+    }
+    if (a.stats && gid < a.n) record_stats<MODE>(a, S, i, n_nodes);
+    if (gid < a.n) finish_node<MODE>(a, S, i);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kQueryBS) void k_query(QueryArgs a)
+{
+    __shared__ int32_t stack_lds[kStack][kQueryBS];
+    const int64_t gid = (int64_t)blockIdx.x * kQueryBS + threadIdx.x;
+    // Seeding pass: one query per tile of 64 consecutive (Morton-ordered) nodes.
+    const int64_t ntiles = (a.n + 63) >> 6;
+    if (a.rep ? gid >= ntiles : gid >= a.n) return;
+    const int64_t i = a.rep ? min(gid * 64 + 32, a.n - 1) : gid;
+    const float px = a.sx[i], py = a.sy[i], pz = a.sz[i];
+    Searcher<MODE> S;
+    init_searcher<MODE>(S, a, px, py, pz);
+    // The tile's seed is one exact candidate: a tight first bound for pruning.
+    if (a.seed_in) {
+        const int32_t s0 = a.seed_in[i >> 6];
+        if (s0 >= 0) S.test_prim((uint32_t)s0);
+    }
 
     if (a.L == 1) {
-        const float4 lo = a.leaf_lo[0], hi = a.leaf_hi[0];
-        const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
-        if (ftd::box_dist2(px, py, pz, l3, h3) <= prune_bound(S.best2())) S.test_leaf(0);
+        const Bound lb = a.leaf_b[0];
+        if (ftd::bound_dist2(px, py, pz, lb) <= prune_bound(S.best2())) S.test_leaf(0);
     } else if (a.L > 1) {
         int sp = 0;
         int cur = 0;
         // A binary tree over L leaves has 2L-1 nodes; no traversal visits more.
         int budget = 2 * a.L + 8;
+        uint32_t n_nodes = 0;
         while (--budget > 0) {
             const BNode nd = a.nodes[cur];
-            const float ll = ftd::box_dist2(px, py, pz, nd.lo_l, nd.hi_l);
-            const float lr = ftd::box_dist2(px, py, pz, nd.lo_r, nd.hi_r);
+            n_nodes++;
+            const float ll = ftd::bound_dist2(px, py, pz, nd.l);
+            const float lr = ftd::bound_dist2(px, py, pz, nd.r);
             int c0 = nd.left, c1 = nd.right;
             float l0 = ll, l1 = lr;
             if (lr < ll) { c0 = nd.right; c1 = nd.left; l0 = lr; l1 = ll; }
@@ -812,27 +1050,14 @@ __global__ __launch_bounds__(kQueryBS) void k_query(QueryArgs a)
             cur = next;
         }
         if (budget <= 0) atomicOr(a.overflow, 2u);
+        if (a.stats && !a.rep) record_stats<MODE>(a, S, i, n_nodes);
     }
 
-    // Epilogue (EuclideanDistance.cu:54-57, LevelSet.cpp:438-448, 74, 148).
-    float d;
-    if (MODE == 0) d = S.best;
-    else d = S.bj >= 0 ? sqrtf(S.best) : (a.band > 0.0f ? a.band : FLT_MAX);
-    if (a.band > 0.0f && S.bj < 0) {
-        d = a.band;
-        atomicAdd(a.n_clamped, 1ull);
+    if (a.rep) {
+        a.seed_out[gid] = S.bslot;
+        return;
     }
-    const int64_t node = a.perm[i];
-    const double po = a.phi_old[node];
-    const double dn = (po < 0.0) ? -(double)d : (double)d;
-    if (a.phi_ifn) a.phi_ifn[node] = dn;
-    if (a.nearest) a.nearest[node] = S.bj;
-    double out = dn;
-    if (a.epi == EPI_UPDATE) {
-        const double kept = (a.retain && ftd::get_bit(a.retain, node)) ? po : dn;
-        out = a.w_phi * kept + a.w_ifn * dn;
-    }
-    a.phi_out[node] = out;
+    finish_node<MODE>(a, S, i);
 }
 
 // ============================================================================
@@ -862,7 +1087,7 @@ struct ft_ctx {
     DevBuf nearest;
     // index
     DevBuf bin_count, bin_prefix, pcode, prec, trec, pidx, leaf_key, leaf_start, leaf_cnt;
-    DevBuf leaf_lo, leaf_hi, leaf_parent, bnodes, bflags;
+    DevBuf leaf_b, leaf_parent, bnodes, bflags, seeds, stats;
     // scan scratch and small scalars
     DevBuf scan_bsum, scalars;
     uint64_t* h_scalars = nullptr;   // pinned mirror
@@ -968,7 +1193,7 @@ void mark(ft_ctx* c, int k, hipStream_t s)
 }
 
 // Scalar slots in ctx->scalars.
-enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_CLAMP = 5, S_OVF = 6, S_N = 8 };
+enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_CLAMP = 5, S_OVF = 6, S_STATS = 8, S_N = 12 };
 
 // Builds the bin index over `n` primitives (mode 0: points in ifp, mode 1:
 // triangles tris/ifp) and runs the query with the epilogue.
@@ -1000,8 +1225,7 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
         FT_TRY(ensure(c->leaf_key, sizeof(uint32_t) * L));
         FT_TRY(ensure(c->leaf_start, sizeof(uint32_t) * L));
         FT_TRY(ensure(c->leaf_cnt, sizeof(uint32_t) * L));
-        FT_TRY(ensure(c->leaf_lo, sizeof(float4) * L));
-        FT_TRY(ensure(c->leaf_hi, sizeof(float4) * L));
+        FT_TRY(ensure(c->leaf_b, sizeof(Bound) * L));
         FT_TRY(ensure(c->leaf_parent, sizeof(int32_t) * L));
         FT_TRY(ensure(c->bnodes, sizeof(BNode) * std::max<int64_t>(1, L - 1)));
         FT_TRY(ensure(c->bflags, sizeof(unsigned) * std::max<int64_t>(1, L - 1)));
@@ -1011,18 +1235,17 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
         hipLaunchKernelGGL(k_prim_scatter, dim3(grid1d(n)), dim3(kBS), 0, s, mode, n, ifp, tris,
                            c->pcode.as<uint32_t>(), cnt, pre, c->prec.as<float4>(),
                            c->trec.as<TriRec>(), c->pidx.as<int32_t>());
-        hipLaunchKernelGGL(k_leaf_boxes, dim3(grid1d(L)), dim3(kBS), 0, s, mode, L,
+        hipLaunchKernelGGL(k_leaf_bounds, dim3(grid1d(L)), dim3(kBS), 0, s, mode, L,
                            c->leaf_start.as<uint32_t>(), c->leaf_cnt.as<uint32_t>(),
-                           c->pidx.as<int32_t>(), ifp, tris, c->leaf_lo.as<float4>(),
-                           c->leaf_hi.as<float4>());
+                           c->pidx.as<int32_t>(), ifp, tris, c->leaf_b.as<Bound>());
         if (L > 1) {
             FT_TRY(fill<unsigned>(s, c->bflags.as<unsigned>(), L - 1, 0u));
             hipLaunchKernelGGL(k_karras, dim3(grid1d(L - 1)), dim3(kBS), 0, s,
                                c->leaf_key.as<uint32_t>(), (int)L, c->bnodes.as<BNode>(),
                                c->leaf_parent.as<int32_t>());
             hipLaunchKernelGGL(k_refit, dim3(grid1d(L)), dim3(kBS), 0, s, (int)L,
-                               c->leaf_parent.as<int32_t>(), c->leaf_lo.as<float4>(),
-                               c->leaf_hi.as<float4>(), c->bnodes.as<BNode>(),
+                               c->leaf_parent.as<int32_t>(), c->leaf_b.as<Bound>(),
+                               c->bnodes.as<BNode>(),
                                c->bflags.as<unsigned>());
         }
         FT_TRY(check_launch());
@@ -1037,8 +1260,7 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
     q.pidx = c->pidx.as<int32_t>();
     q.leaf_start = c->leaf_start.as<uint32_t>();
     q.leaf_cnt = c->leaf_cnt.as<uint32_t>();
-    q.leaf_lo = c->leaf_lo.as<float4>();
-    q.leaf_hi = c->leaf_hi.as<float4>();
+    q.leaf_b = c->leaf_b.as<Bound>();
     q.nodes = c->bnodes.as<BNode>();
     q.L = (int)L;
     q.band = band;
@@ -1056,17 +1278,65 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
     q.w_phi = w_phi;
     q.w_ifn = w_ifn;
     q.n_clamped = (unsigned long long*)(sc + S_CLAMP);
+    static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
+    q.h_est = c->h_est;
+    if (want_stats) {
+        FT_TRY(ensure(c->stats, 8 * 32));
+        FT_TRY(fill<uint64_t>(s, c->stats.as<uint64_t>(), 32, 0ull));
+        q.stats = c->stats.as<unsigned long long>();
+    }
     q.overflow = (unsigned*)(sc + S_OVF);
+    const int64_t ntiles = (c->N + 63) / 64;
+    const bool seeded = L > 1 && c->N >= 4096;
+    static const int kind = getenv("FT_QUERY_KERNEL") && !strcmp(getenv("FT_QUERY_KERNEL"), "lane") ? 0 : 1;
+    static const bool use_seed = !(getenv("FT_SEED") && !strcmp(getenv("FT_SEED"), "0"));
     mark(c, 2, s);
-    if (c->N > 0) {
+    if (seeded && use_seed) {
+        FT_TRY(ensure(c->seeds, sizeof(int32_t) * ntiles));
+        QueryArgs qr = q;
+        qr.rep = 1;
+        qr.seed_out = c->seeds.as<int32_t>();
+        qr.stats = nullptr;
         if (mode == 0)
-            hipLaunchKernelGGL(k_query<0>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
+            hipLaunchKernelGGL(k_query<0>, dim3(grid1d(ntiles, kQueryBS)), dim3(kQueryBS), 0, s, qr);
         else
-            hipLaunchKernelGGL(k_query<1>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
+            hipLaunchKernelGGL(k_query<1>, dim3(grid1d(ntiles, kQueryBS)), dim3(kQueryBS), 0, s, qr);
+        q.seed_in = c->seeds.as<int32_t>();
+    }
+    if (c->N > 0) {
+        if (kind == 1) {
+            if (mode == 0)
+                hipLaunchKernelGGL(k_query_wave<0>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
+            else
+                hipLaunchKernelGGL(k_query_wave<1>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
+        } else {
+            if (mode == 0)
+                hipLaunchKernelGGL(k_query<0>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
+            else
+                hipLaunchKernelGGL(k_query<1>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
+        }
         FT_TRY(check_launch());
     }
     mark(c, 3, s);
     FT_TRY(read_scalars(c, s, S_N));
+    if (want_stats && c->N > 0) {
+        uint64_t st[32];
+        FT_HIP(hipMemcpy(st, c->stats.p, sizeof st, hipMemcpyDeviceToHost));
+        static const char* names[8] = {"out <2h", "out 2-8h", "out 8-32h", "out >32h",
+                                       "in <2h", "in 2-8h", "in 8-32h", "in >32h"};
+        double tn = 0, tl = 0, tp = 0;
+        for (int b = 0; b < 8; b++) { tn += st[4 * b + 1]; tl += st[4 * b + 2]; tp += st[4 * b + 3]; }
+        fprintf(stderr, "[ftreinit] query mode %d: N=%lld L=%lld per node: tree nodes %.1f, "
+                "leaves %.1f, primitives %.1f\n", mode, (long long)c->N, (long long)L,
+                tn / c->N, tl / c->N, tp / c->N);
+        for (int b = 0; b < 8; b++) {
+            const double n = (double)st[4 * b];
+            if (n > 0)
+                fprintf(stderr, "[ftreinit]   %-9s %5.1f%% of nodes: tree %.1f leaves %.1f prims %.1f\n",
+                        names[b], 100.0 * n / c->N, st[4 * b + 1] / n, st[4 * b + 2] / n,
+                        st[4 * b + 3] / n);
+        }
+    }
     if (c->h_scalars[S_OVF])
         return ft::set_error(FT_ERR_HIP, "search: traversal exceeded its bounds (%llu)",
                              (unsigned long long)c->h_scalars[S_OVF]);
@@ -1353,8 +1623,8 @@ int ft_destroy(ft_handle h)
                       &h->ifp2e, &h->tris, &h->tri_norm, &h->if_elems, &h->p2t_off,
                       &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
                       &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
-                      &h->leaf_start, &h->leaf_cnt, &h->leaf_lo, &h->leaf_hi, &h->leaf_parent,
-                      &h->bnodes, &h->bflags, &h->scan_bsum, &h->scalars};
+                      &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
+                      &h->bnodes, &h->bflags, &h->seeds, &h->stats, &h->scan_bsum, &h->scalars};
     for (DevBuf* b : bufs) release(*b);
     if (h->h_scalars) (void)hipHostFree(h->h_scalars);
     for (int k = 0; k < 4; k++)

@@ -174,8 +174,8 @@ class Reinit(_Handle):
         ifn = np.empty_like(out) if want_ifn else None
         o = make_opts(mode, blend, band, bin_scale, want_nearest)
         cnt = _lib.FtCounts()
-        check(lib.ft_reinit(self._h, _ptr(out), _ptr(ifn), C.byref(o), C.byref(cnt)))
         self.counts = cnt
+        check(lib.ft_reinit(self._h, _ptr(out), _ptr(ifn), C.byref(o), C.byref(cnt)))
         return out, ifn, cnt
 
     def reset_phi_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
@@ -183,10 +183,10 @@ class Reinit(_Handle):
         """Device-resident phi (raw device addresses, e.g. tensor.data_ptr())."""
         o = make_opts(mode, blend, band, bin_scale, want_nearest)
         cnt = _lib.FtCounts()
+        self.counts = cnt
         check(lib.ft_reinit_device(self._h, C.c_void_p(phi_ptr),
                                    C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o),
                                    C.byref(cnt), C.c_void_p(stream) if stream else None))
-        self.counts = cnt
         return cnt
 
     def set_timing(self, on=True):
