@@ -170,12 +170,7 @@ __device__ __forceinline__ float bound_dist2(float x, float y, float z, const Bo
     return fmaxf(box_dist2(x, y, z, b.lo, b.hi), cyl_dist2(x, y, z, b));
 }
 
-// ---- FP64 point-triangle distance ------------------------------------------
-// The closest point of a triangle is the orthogonal projection onto its plane
-// when that falls inside (edge functions n.(e_i x (x - v_i)) >= 0), else the
-// closest point of one of its three edge segments.  Triangles thinner than
-// 1e-7 of their longest edge are measured by their segments alone.  Used for
-// the degenerate and thin triangles only.
+// ---- FP64 vector helpers (index build) --------------------------------------
 __device__ inline double dot3d(const double* u, const double* v)
 {
     return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
@@ -186,68 +181,54 @@ __device__ inline void cross3d(const double* u, const double* v, double* w)
     w[1] = u[2] * v[0] - u[0] * v[2];
     w[2] = u[0] * v[1] - u[1] * v[0];
 }
-__device__ inline double seg_dist2_f64(const double* x, const double* a, const double* b)
-{
-    double ab[3], ax[3];
-    for (int c = 0; c < 3; c++) { ab[c] = b[c] - a[c]; ax[c] = x[c] - a[c]; }
-    const double l2 = dot3d(ab, ab);
-    double t = 0.0;
-    if (l2 > 0.0) {
-        t = dot3d(ax, ab) / l2;
-        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
-    }
-    double r[3];
-    for (int c = 0; c < 3; c++) r[c] = ax[c] - t * ab[c];
-    return dot3d(r, r);
-}
-
-__device__ inline double tri_dist2_f64(const double* x, const double* a, const double* b,
-                                       const double* c)
-{
-    double ab[3], bc[3], ca[3], n[3];
-    for (int k = 0; k < 3; k++) { ab[k] = b[k] - a[k]; bc[k] = c[k] - b[k]; ca[k] = a[k] - c[k]; }
-    cross3d(ab, bc, n);
-    const double nn = dot3d(n, n);
-    const double lmax = fmax(dot3d(ab, ab), fmax(dot3d(bc, bc), dot3d(ca, ca)));
-    if (nn > 1e-14 * lmax * lmax) {
-        double xa[3], xb[3], xc[3], t[3];
-        for (int k = 0; k < 3; k++) { xa[k] = x[k] - a[k]; xb[k] = x[k] - b[k]; xc[k] = x[k] - c[k]; }
-        cross3d(ab, xa, t);
-        const double s0 = dot3d(n, t);
-        cross3d(bc, xb, t);
-        const double s1 = dot3d(n, t);
-        cross3d(ca, xc, t);
-        const double s2 = dot3d(n, t);
-        if (s0 >= 0.0 && s1 >= 0.0 && s2 >= 0.0) {
-            const double h = dot3d(n, xa);
-            return h * h / nn;
-        }
-    }
-    return fmin(seg_dist2_f64(x, a, b), fmin(seg_dist2_f64(x, b, c), seg_dist2_f64(x, c, a)));
-}
 
 // ---- FP32 point-triangle distance -------------------------------------------
-// Triangle record (80 B), an orthonormal frame at vertex a with u along ab and
-// n the unit normal; in it the triangle is (0,0), (B,0), (Cx,Cy), Cy > 0:
+// Triangle record (80 B), an orthonormal frame at vertex a, built in FP64 when
+// the index is built, with u along ab (the longest edge) and n the unit
+// normal; in it the triangle is (0,0), (B,0), (Cx,Cy), Cy >= 0:
 //   F0 = (a, B)  F1 = (u, Cx)  F2 = (v, Cy)  F3 = (n, 1/B)
-//   F4 = (1/|ac|^2, 1/|bc|^2, Cx - B, flag)      flag < 0: FP64 path
-// The point is projected once (X, Y, Z); the in-plane distance is 0 inside
-// the triangle, else the nearest clamped edge projection; d^2 = d2D^2 + Z^2.
-// The frame keeps the 2D problem well conditioned for any non-thin triangle.
+//   F4 = (1/|ac|^2, 1/|bc|^2, Cx - B, interior)
+// (inverse lengths are 0 for zero-length edges; interior = 0 for collinear
+// vertices, whose distance is that of the edge segments).  The point is
+// projected once (X, Y, Z); the in-plane distance is 0 inside the triangle,
+// else the nearest clamped edge projection; d^2 = d2D^2 + Z^2.  With ab the
+// longest edge the 2D problem stays well conditioned however thin the
+// triangle is, so no case needs FP64 at query time.
 struct TriRec {
     float4 F0, F1, F2, F3, F4;
 };
 
-__device__ __forceinline__ float tri_dist2_f32(float x, float y, float z, const TriRec& r)
+// Projection of the point into the triangle's frame.
+struct TriProj {
+    float X, Y, Z;
+};
+__device__ __forceinline__ TriProj tri_project(float x, float y, float z, const TriRec& r)
 {
     const float wx = x - r.F0.x, wy = y - r.F0.y, wz = z - r.F0.z;
-    const float X = __fmaf_rn(wz, r.F1.z, __fmaf_rn(wy, r.F1.y, wx * r.F1.x));
-    const float Y = __fmaf_rn(wz, r.F2.z, __fmaf_rn(wy, r.F2.y, wx * r.F2.x));
-    const float Z = __fmaf_rn(wz, r.F3.z, __fmaf_rn(wy, r.F3.y, wx * r.F3.x));
+    TriProj p;
+    p.X = __fmaf_rn(wz, r.F1.z, __fmaf_rn(wy, r.F1.y, wx * r.F1.x));
+    p.Y = __fmaf_rn(wz, r.F2.z, __fmaf_rn(wy, r.F2.y, wx * r.F2.x));
+    p.Z = __fmaf_rn(wz, r.F3.z, __fmaf_rn(wy, r.F3.y, wx * r.F3.x));
+    return p;
+}
+
+// Lower bound from the disk about vertex a that holds the triangle (radius
+// |ab|, the longest edge); costs little once the point is projected.
+__device__ __forceinline__ float tri_disk_lb2(const TriProj& p, const TriRec& r)
+{
+    const float rho = r.F0.w * (1.0f + 0x1.0p-18f);
+    const float lat = sqrtf(__fmaf_rn(p.Y, p.Y, p.X * p.X));
+    const float g = fmaxf(lat - rho, 0.0f) * (1.0f - 0x1.0p-18f);
+    return __fmaf_rn(g, g, p.Z * p.Z) * (1.0f - 0x1.0p-18f);
+}
+
+__device__ __forceinline__ float tri_dist2_from(const TriProj& p, const TriRec& r)
+{
+    const float X = p.X, Y = p.Y, Z = p.Z;
     const float B = r.F0.w, Cx = r.F1.w, Cy = r.F2.w, dCx = r.F4.z;
     const float xb = X - B;
     const bool inside = (int)(Y >= 0.0f) & (int)(__fmaf_rn(Cy, X, -Cx * Y) >= 0.0f) &
-                        (int)(__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f);
+                        (int)(__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f) & (int)(r.F4.w > 0.0f);
     // edge ab
     float t = fminf(fmaxf(X * r.F3.w, 0.0f), 1.0f);
     float ex = __fmaf_rn(-t, B, X);
@@ -263,6 +244,11 @@ __device__ __forceinline__ float tri_dist2_f32(float x, float y, float z, const 
     ey = __fmaf_rn(-t, Cy, Y);
     d = fminf(d, __fmaf_rn(ey, ey, ex * ex));
     return __fmaf_rn(Z, Z, inside ? 0.0f : d);
+}
+
+__device__ __forceinline__ float tri_dist2_f32(float x, float y, float z, const TriRec& r)
+{
+    return tri_dist2_from(tri_project(x, y, z, r), r);
 }
 
 }  // namespace ftd

@@ -526,40 +526,57 @@ __global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, int64_t n, const
         prec[slot] = make_float4(ifp[3 * j], ifp[3 * j + 1], ifp[3 * j + 2], 0.0f);
         return;
     }
-    float a[3], b[3], d[3];
-    tri_vertices(tris, ifp, (int32_t)j, a, b, d);
-    // Frame in FP64: u along ab, n the unit normal, v = n x u.
-    double e0[3], e1[3], e2[3];
-    for (int k = 0; k < 3; k++) {
-        e0[k] = (double)b[k] - a[k];
-        e1[k] = (double)d[k] - a[k];
-        e2[k] = (double)d[k] - b[k];
+    float v3[3][3];
+    tri_vertices(tris, ifp, (int32_t)j, v3[0], v3[1], v3[2]);
+    // Rotate (orientation kept) so that ab is the longest edge, then build the
+    // frame in FP64: u along ab, n the unit normal, v = n x u.  Collinear or
+    // coincident vertices get a frame along ab and no interior: the distance
+    // is then that of the edge segments.
+    double len[3];
+    for (int q = 0; q < 3; q++) {
+        const float* p0 = v3[q];
+        const float* p1 = v3[(q + 1) % 3];
+        double l = 0.0;
+        for (int k = 0; k < 3; k++) { const double dd = (double)p1[k] - p0[k]; l += dd * dd; }
+        len[q] = l;
     }
+    const int r0 = (len[0] >= len[1] && len[0] >= len[2]) ? 0 : (len[1] >= len[2] ? 1 : 2);
+    const float* a = v3[r0];
+    const float* b = v3[(r0 + 1) % 3];
+    const float* d = v3[(r0 + 2) % 3];
+    double e0[3], e1[3];
+    for (int k = 0; k < 3; k++) { e0[k] = (double)b[k] - a[k]; e1[k] = (double)d[k] - a[k]; }
     double nv[3];
     ftd::cross3d(e0, e1, nv);
     const double nn = ftd::dot3d(nv, nv);
-    const double l0 = ftd::dot3d(e0, e0), l1 = ftd::dot3d(e1, e1), l2 = ftd::dot3d(e2, e2);
-    const double lmax = fmax(l0, fmax(l1, l2));
-    TriRec r;
-    // thin (height < 1e-3 of the longest edge) or degenerate: FP64 path
-    if (!(nn > 1e-6 * lmax * lmax)) {
-        r.F0 = make_float4(a[0], a[1], a[2], 0.0f);
-        r.F1 = r.F2 = r.F3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        r.F4 = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    } else {
-        const double L0 = sqrt(l0), inn = 1.0 / sqrt(nn);
-        double u[3], nu[3], v[3];
-        for (int k = 0; k < 3; k++) { u[k] = e0[k] / L0; nu[k] = nv[k] * inn; }
+    const double L0 = sqrt(ftd::dot3d(e0, e0));
+    double u[3], v[3], nu[3];
+    bool interior = nn > 0.0 && L0 > 0.0;
+    if (L0 > 0.0) for (int k = 0; k < 3; k++) u[k] = e0[k] / L0;
+    else { u[0] = 1.0; u[1] = 0.0; u[2] = 0.0; }
+    if (interior) {
+        const double inn = 1.0 / sqrt(nn);
+        for (int k = 0; k < 3; k++) nu[k] = nv[k] * inn;
         ftd::cross3d(nu, u, v);
-        const double Cx = ftd::dot3d(e1, u), Cy = ftd::dot3d(e1, v);
-        const double dCx = Cx - L0;
-        r.F0 = make_float4(a[0], a[1], a[2], (float)L0);
-        r.F1 = make_float4((float)u[0], (float)u[1], (float)u[2], (float)Cx);
-        r.F2 = make_float4((float)v[0], (float)v[1], (float)v[2], (float)Cy);
-        r.F3 = make_float4((float)nu[0], (float)nu[1], (float)nu[2], (float)(1.0 / L0));
-        r.F4 = make_float4((float)(1.0 / (Cx * Cx + Cy * Cy)), (float)(1.0 / (dCx * dCx + Cy * Cy)),
-                           (float)dCx, 1.0f);
+    } else {
+        // any unit vector orthogonal to u
+        const double t0[3] = {fabs(u[0]) < 0.9 ? 1.0 : 0.0, fabs(u[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        double w[3];
+        ftd::cross3d(u, t0, w);
+        const double wl = sqrt(ftd::dot3d(w, w));
+        for (int k = 0; k < 3; k++) v[k] = w[k] / wl;
+        ftd::cross3d(u, v, nu);
     }
+    const double Cx = ftd::dot3d(e1, u), Cy = interior ? ftd::dot3d(e1, v) : 0.0;
+    const double dCx = Cx - L0;
+    const double lac = Cx * Cx + Cy * Cy, lbc = dCx * dCx + Cy * Cy;
+    TriRec r;
+    r.F0 = make_float4(a[0], a[1], a[2], (float)L0);
+    r.F1 = make_float4((float)u[0], (float)u[1], (float)u[2], (float)Cx);
+    r.F2 = make_float4((float)v[0], (float)v[1], (float)v[2], (float)Cy);
+    r.F3 = make_float4((float)nu[0], (float)nu[1], (float)nu[2], L0 > 0.0 ? (float)(1.0 / L0) : 0.0f);
+    r.F4 = make_float4(lac > 0.0 ? (float)(1.0 / lac) : 0.0f, lbc > 0.0 ? (float)(1.0 / lbc) : 0.0f,
+                       (float)dCx, interior ? 1.0f : 0.0f);
     trec[slot] = r;
 }
 
@@ -780,6 +797,7 @@ struct QueryArgs {
     int32_t* seed_out;               // seeding pass: best slot of each tile's query
     const int32_t* seed_in;          // nullable: per-tile candidate tested first
     float h_est;                     // mean node spacing (statistics only)
+    int probe;                       // wave kernel: greedy first candidate
 };
 
 // Margin of the box prune: a box is skipped only if its lower bound exceeds
@@ -825,17 +843,7 @@ struct Searcher {
             consider(d, s);
         } else {
             const TriRec r = trec[s];
-            float d2;
-            if (r.F4.w >= 0.0f) {
-                d2 = ftd::tri_dist2_f32(x, y, z, r);
-            } else {
-                float va[3], vb[3], vc[3];
-                tri_vertices(tris, ifp, pidx[s], va, vb, vc);
-                const double X[3] = {x, y, z};
-                const double A[3] = {va[0], va[1], va[2]}, B[3] = {vb[0], vb[1], vb[2]},
-                             C[3] = {vc[0], vc[1], vc[2]};
-                d2 = (float)ftd::tri_dist2_f64(X, A, B, C);
-            }
+            const float d2 = ftd::tri_dist2_f32(x, y, z, r);
             consider(d2, s);
         }
     }
@@ -854,17 +862,10 @@ struct Searcher {
                 consider(d, s);
             } else {
                 const TriRec r = ftd::cload(trec, s);
-                float d2;
-                if (r.F4.w >= 0.0f) {
-                    d2 = ftd::tri_dist2_f32(x, y, z, r);
-                } else {
-                    float va[3], vb[3], vc[3];
-                    tri_vertices(tris, ifp, pidx[s], va, vb, vc);
-                    const double X[3] = {x, y, z};
-                    const double A[3] = {va[0], va[1], va[2]}, B[3] = {vb[0], vb[1], vb[2]},
-                                 C[3] = {vc[0], vc[1], vc[2]};
-                    d2 = (float)ftd::tri_dist2_f64(X, A, B, C);
-                }
+                const ftd::TriProj pj = ftd::tri_project(x, y, z, r);
+                // wave-uniform early out when the disk bound rules it out for all lanes
+                if (__ballot(ftd::tri_disk_lb2(pj, r) <= prune_bound(best2())) == 0ull) continue;
+                const float d2 = ftd::tri_dist2_from(pj, r);
                 consider(d2, s);
             }
         }
@@ -958,16 +959,32 @@ __global__ __launch_bounds__(kQueryBS) void k_query_wave(QueryArgs a)
     if (a.L == 1) {
         S.test_leaf_uniform(0);
     } else if (a.L > 1) {
+        if (a.probe) {
+            // Greedy descent for the tile's middle node to the leaf nearest to it
+            // by bound: a first exact candidate for every lane.
+            const float cx = __shfl(px, 32), cy = __shfl(py, 32), cz = __shfl(pz, 32);
+            int cur = 0;
+            for (int depth = 0; depth < kStack + 1 && cur >= 0; depth++) {
+                const BNode nd = ftd::cload(a.nodes, cur);
+                const float ll = ftd::bound_dist2(cx, cy, cz, nd.l);
+                const float lr = ftd::bound_dist2(cx, cy, cz, nd.r);
+                cur = __builtin_amdgcn_readfirstlane(ll <= lr ? nd.left : nd.right);
+            }
+            if (cur < 0) S.test_leaf_uniform(~cur);
+        }
         int sp = 0;
         int cur = 0;
         int budget = 2 * a.L + 8;
         while (--budget > 0) {
             const BNode nd = ftd::cload(a.nodes, cur);    // scalar loads: cur is uniform
             const int left = nd.left, right = nd.right;
-            const float ll = ftd::bound_dist2(px, py, pz, nd.l);
-            const float lr = ftd::bound_dist2(px, py, pz, nd.r);
-            n_nodes++;
             const float bnd = prune_bound(S.best2());
+            // box bound first; the cylinder only where some lane's box admits it
+            float ll = ftd::box_dist2(px, py, pz, nd.l.lo, nd.l.hi);
+            float lr = ftd::box_dist2(px, py, pz, nd.r.lo, nd.r.hi);
+            if (__ballot(ll <= bnd)) ll = fmaxf(ll, ftd::cyl_dist2(px, py, pz, nd.l));
+            if (__ballot(lr <= bnd)) lr = fmaxf(lr, ftd::cyl_dist2(px, py, pz, nd.r));
+            n_nodes++;
             const uint64_t mL = __ballot(ll <= bnd), mR = __ballot(lr <= bnd);
             const uint64_t any = mL | mR;
             const bool left_first = 2 * __popcll(__ballot(ll <= lr) & any) >= __popcll(any);
@@ -1289,7 +1306,8 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
     const int64_t ntiles = (c->N + 63) / 64;
     const bool seeded = L > 1 && c->N >= 4096;
     static const int kind = getenv("FT_QUERY_KERNEL") && !strcmp(getenv("FT_QUERY_KERNEL"), "lane") ? 0 : 1;
-    static const bool use_seed = !(getenv("FT_SEED") && !strcmp(getenv("FT_SEED"), "0"));
+    static const bool use_seed = getenv("FT_SEED") && !strcmp(getenv("FT_SEED"), "1");
+    q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     mark(c, 2, s);
     if (seeded && use_seed) {
         FT_TRY(ensure(c->seeds, sizeof(int32_t) * ntiles));

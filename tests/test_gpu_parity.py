@@ -205,3 +205,27 @@ def test_device_resident_matches_host(rein):
         rein.reset_phi_device(d_phi.data_ptr(), d_ifn.data_ptr(), mode="triangles")
         assert np.array_equal(d_phi.cpu().numpy().view(np.uint64), out.view(np.uint64))
         assert np.array_equal(d_ifn.cpu().numpy().view(np.uint64), ifn.view(np.uint64))
+
+
+def test_triangle_mode_degenerate_triangles(rein):
+    """phi = 0 exactly on a grid layer: crossing points coincide with nodes and
+    half of the triangles have zero area.  The surface is the plane x = x_k, so
+    every node's distance is |x - x_k| in closed form."""
+    n = 24
+    m = fa.kuhn_mesh(n)
+    xk = m.xyz[11 * n * n, 0]                      # node layer i = 11, exact float
+    phi = m.xyz[:, 0].astype(np.float64) - float(xk)
+    assert (phi == 0.0).sum() == n * n
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, cnt = r.reset_phi(phi, mode="triangles", want_nearest=True)
+    itf = r.interface()
+    q = itf.ifp[itf.tris[:, :3]].astype(np.float64)
+    area2 = np.linalg.norm(np.cross(q[:, 1] - q[:, 0], q[:, 2] - q[:, 0]), axis=1)
+    assert (area2 == 0).sum() > 100                 # many degenerate triangles
+    h = 1.0 / (n - 1)
+    exact = np.abs(m.xyz[:, 0].astype(np.float64) - float(xk))
+    assert np.max(np.abs(np.abs(ifn) - exact)) <= 1e-5 * h
+    d64, f64 = oracle.triangle_distance(m.xyz, itf.ifp, itf.tris)
+    assert np.max(np.abs(np.abs(ifn) - d64)) <= 1e-5 * h
+    r.close()
