@@ -90,6 +90,12 @@ _SIGS = {
     "ft_get_interface": (C.c_int, [_H, _P, _P, _P, _P, _P, _P, _P]),
     "ft_get_nearest": (C.c_int, [_H, _P]),
     "ft_interface_device": (C.c_int, [_H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "ft_set_mesh_batch": (C.c_int, [_H, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "ft_set_kuhn_batch": (C.c_int, [_H, C.c_int32, _P, _P, _P, C.c_double, C.c_uint64]),
+    "ft_batch_info": (C.c_int, [_H, C.POINTER(C.c_int32), _P, _P, _P]),
+    "ft_get_mesh": (C.c_int, [_H, _P, _P, _P, _P]),
+    "ft_get_field_counts": (C.c_int, [_H, _P]),
+    "ft_reinit_batch": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
     "ft_set_timing": (C.c_int, [_H, C.c_int32]),
     "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "ft_kuhn_counts": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64),

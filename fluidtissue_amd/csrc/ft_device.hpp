@@ -127,6 +127,18 @@ __device__ __forceinline__ uint32_t grid_code(const Grid& g, float x, float y, f
     return morton3(b[0], b[1], b[2]);
 }
 
+// Field of element x given the exclusive offsets off[0..nf] (off[0] = 0):
+// the largest k < nf with off[k] <= x.
+__device__ __forceinline__ int field_of(const int64_t* off, int nf, int64_t x)
+{
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Squared distance from x to an axis-aligned box (0 inside).
 __device__ __forceinline__ float box_dist2(float x, float y, float z, const float* lo,
                                            const float* hi)

@@ -151,6 +151,17 @@ __global__ __launch_bounds__(kBS) void k_scan_down(const TI* in, TO* out, int64_
 // ============================================================================
 // Mesh set-up: Morton order of the nodes (processing order of k_query)
 // ============================================================================
+// Fields: a handle holds one or more independent fields (a plain reinit has
+// one).  Field k owns nodes [node_off[k], node_off[k+1]) and likewise tets and
+// edges; its index bins and query tiles never mix with another field's.
+struct FieldGeom {
+    const int64_t* node_off;         // nf + 1
+    const int64_t* cnt_off;          // unused by the slot padding
+    const Grid* grids;               // unused by the slot padding
+    int nf;
+};
+
+// Node sort of one field (nodes node_base.. of xyz, local index i).
 __global__ __launch_bounds__(kBS) void k_node_bin(const float* xyz, int64_t n, Grid g,
                                                   uint32_t* code, uint32_t* count)
 {
@@ -161,7 +172,8 @@ __global__ __launch_bounds__(kBS) void k_node_bin(const float* xyz, int64_t n, G
     atomicAdd(&count[c], 1u);
 }
 
-__global__ __launch_bounds__(kBS) void k_node_scatter(const float* xyz, int64_t n,
+// Sorted position p of local node i goes to query slot slot_base + p.
+__global__ __launch_bounds__(kBS) void k_node_scatter(const float* xyz, int64_t n, int64_t node_base,
                                                       const uint32_t* code, uint32_t* count,
                                                       const uint32_t* offset, int32_t* perm,
                                                       float* sx, float* sy, float* sz)
@@ -169,11 +181,31 @@ __global__ __launch_bounds__(kBS) void k_node_scatter(const float* xyz, int64_t 
     const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = code[i];
-    const uint32_t slot = offset[c] + atomicSub(&count[c], 1u) - 1u;
-    perm[slot] = (int32_t)i;
+    const int64_t slot = (int64_t)offset[c] + atomicSub(&count[c], 1u) - 1u;
+    perm[slot] = (int32_t)(node_base + i);
     sx[slot] = xyz[3 * i];
     sy[slot] = xyz[3 * i + 1];
     sz[slot] = xyz[3 * i + 2];
+}
+
+// Pad slots of each field's last wave: perm -1, coordinates of the field's
+// first slot (lanes that only shadow a real node); tile -> field table.
+__global__ __launch_bounds__(kBS) void k_pad_slots(int64_t n_slots, FieldGeom fg,
+                                                   const int64_t* slot_off, int32_t* perm,
+                                                   float* sx, float* sy, float* sz,
+                                                   int32_t* tile_field)
+{
+    const int64_t s = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (s >= n_slots) return;
+    const int k = ftd::field_of(slot_off, fg.nf, s);
+    if ((s & 63) == 0) tile_field[s >> 6] = k;
+    const int64_t nk = fg.node_off[k + 1] - fg.node_off[k];
+    if (s - slot_off[k] < nk) return;
+    const int64_t f0 = slot_off[k];
+    perm[s] = -1;
+    sx[s] = sx[f0];
+    sy[s] = sy[f0];
+    sz[s] = sz[f0];
 }
 
 // ============================================================================
@@ -475,28 +507,43 @@ __device__ __forceinline__ void tri_vertices(const int4* tris, const float* ifp,
     }
 }
 
+// Index geometry of one reinit: per-field grids and counter offsets, the
+// field of a crossing point through its edge (edge_off), and the key layout
+// (field << code_bits) | Morton code.
+struct IndexGeom {
+    const int64_t* edge_off;         // nf + 1
+    const int64_t* cnt_off;          // nf + 1
+    const Grid* grids;               // nf
+    int nf;
+    int code_bits;
+};
+
 // mode 0: points (ifp), mode 1: triangles (centroid).
 __global__ __launch_bounds__(kBS) void k_prim_bin(int mode, int64_t n, const float* ifp,
-                                                  const int4* tris, Grid g, uint32_t* code,
-                                                  uint32_t* count)
+                                                  const int4* tris, const int32_t* ifp2e,
+                                                  IndexGeom ig, uint32_t* code, uint32_t* count)
 {
     const int64_t j = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (j >= n) return;
     float c[3];
+    int32_t v0;
     if (mode == 0) {
         for (int k = 0; k < 3; k++) c[k] = ifp[3 * j + k];
+        v0 = (int32_t)j;
     } else {
         float a[3], b[3], d[3];
         tri_vertices(tris, ifp, (int32_t)j, a, b, d);
         for (int k = 0; k < 3; k++) c[k] = (a[k] + b[k] + d[k]) * (1.0f / 3.0f);
+        v0 = tris[j].x;
     }
-    const uint32_t cc = ftd::grid_code(g, c[0], c[1], c[2]);
+    const int f = ifp2e ? ftd::field_of(ig.edge_off, ig.nf, ifp2e[v0]) : 0;
+    const uint32_t cc = (uint32_t)ig.cnt_off[f] + ftd::grid_code(ig.grids[f], c[0], c[1], c[2]);
     code[j] = cc;
     atomicAdd(&count[cc], 1u);
 }
 
 __global__ __launch_bounds__(kBS) void k_leaves(const uint32_t* count, const uint64_t* prefix,
-                                                int64_t G, uint32_t* leaf_key,
+                                                int64_t G, IndexGeom ig, uint32_t* leaf_key,
                                                 uint32_t* leaf_start, uint32_t* leaf_cnt)
 {
     const int64_t c = (int64_t)blockIdx.x * kBS + threadIdx.x;
@@ -505,9 +552,29 @@ __global__ __launch_bounds__(kBS) void k_leaves(const uint32_t* count, const uin
     if (!n) return;
     const uint64_t p = prefix[c];
     const uint32_t l = (uint32_t)(p >> 32);
-    leaf_key[l] = (uint32_t)c;
+    const int f = ftd::field_of(ig.cnt_off, ig.nf, c);
+    leaf_key[l] = ((uint32_t)f << ig.code_bits) | (uint32_t)(c - ig.cnt_off[f]);
     leaf_start[l] = (uint32_t)p;
     leaf_cnt[l] = n;
+}
+
+// Leaf range [leaf_off[f], leaf_off[f+1]) of each field and its tree root:
+// none (INT_MIN) without leaves, the leaf itself when alone; fields with two
+// or more leaves get the internal node covering exactly their range from
+// k_karras.
+__global__ void k_field_roots(int nf, const int64_t* cnt_off, const uint64_t* prefix,
+                              const uint64_t* total, int64_t G, int32_t* leaf_off, int32_t* roots)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f > nf) return;
+    const uint32_t L = (uint32_t)(*total >> 32);
+    const int64_t c = cnt_off[f];
+    const uint32_t lf = c < G ? (uint32_t)(prefix[c] >> 32) : L;
+    leaf_off[f] = (int32_t)lf;
+    if (f == nf) return;
+    const int64_t c1 = cnt_off[f + 1];
+    const uint32_t ll = c1 < G ? (uint32_t)(prefix[c1] >> 32) : L;
+    roots[f] = (ll == lf) ? INT32_MIN : (ll == lf + 1 ? ~(int32_t)lf : -1);
 }
 
 // Scatter primitives into bin order and write their search records.
@@ -649,7 +716,8 @@ __device__ __forceinline__ int kdelta(const uint32_t* keys, int L, int i, int j)
 }
 
 __global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, int L, BNode* nodes,
-                                                int32_t* leaf_parent)
+                                                int32_t* leaf_parent, int code_bits,
+                                                const int32_t* leaf_off, int32_t* roots)
 {
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= L - 1) return;
@@ -676,6 +744,10 @@ __global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, int L, BNo
     if (left >= 0) nodes[left].parent = i; else leaf_parent[~left] = i;
     if (right >= 0) nodes[right].parent = i; else leaf_parent[~right] = i;
     if (i == 0) nodes[0].parent = -1;
+    // the node whose range is exactly one field's leaves is that field's root
+    const int lo = min(i, j), hi = max(i, j);
+    const int f = (int)(keys[lo] >> code_bits);
+    if (leaf_off[f] == lo && leaf_off[f + 1] == hi + 1) roots[f] = i;
 }
 
 __device__ __forceinline__ void st_bound(Bound* dst, const Bound& v)
@@ -769,9 +841,11 @@ __global__ __launch_bounds__(kBS) void k_refit(int L, const int32_t* leaf_parent
 enum Epi : int { EPI_RAW = 0, EPI_INIT = 1, EPI_UPDATE = 2 };
 
 struct QueryArgs {
-    int64_t n;                       // nodes
-    const float *sx, *sy, *sz;       // Morton-ordered coordinates
-    const int32_t* perm;             // sorted slot -> node
+    int64_t n_slots;                 // query slots (per-field Morton order, 64-padded)
+    const float *sx, *sy, *sz;       // slot coordinates
+    const int32_t* perm;             // slot -> node (-1: padding)
+    const int32_t* tile_field;       // 64-slot tile -> field
+    const int32_t* roots;            // field -> tree root (>= 0 node, < 0 ~leaf, INT_MIN none)
     const float4* prec;              // point records (mode 0)
     const TriRec* trec;              // triangle records (mode 1)
     const int32_t* pidx;             // sorted slot -> primitive id
@@ -781,7 +855,7 @@ struct QueryArgs {
     const BNode* nodes;
     int L;                           // leaves
     float band;                      // > 0: narrow band
-    const float* ifp;                // FP64 fallback of triangles
+    const float* ifp;
     const int4* tris;
     const double* phi_old;           // sign + retained values
     double* phi_out;
@@ -790,14 +864,11 @@ struct QueryArgs {
     int32_t* nearest;                // nullable
     int epi;
     double w_phi, w_ifn;
-    unsigned long long* n_clamped;
+    unsigned long long* n_clamped;   // per field
     unsigned* overflow;
-    unsigned long long* stats;       // nullable: [tree nodes, leaves, primitives]
-    int rep;                         // 1: seeding pass, one query per 64-node tile
-    int32_t* seed_out;               // seeding pass: best slot of each tile's query
-    const int32_t* seed_in;          // nullable: per-tile candidate tested first
-    float h_est;                     // mean node spacing (statistics only)
-    int probe;                       // wave kernel: greedy first candidate
+    unsigned long long* stats;       // nullable: per distance bucket
+    float h_est;                     // node spacing (statistics only)
+    int probe;                       // greedy first candidate
 };
 
 // Margin of the box prune: a box is skipped only if its lower bound exceeds
@@ -834,20 +905,6 @@ struct Searcher {
     uint32_t n_leaves = 0, n_prims = 0;
     int32_t bslot = -1;
 
-    __device__ void test_prim(uint32_t s)
-    {
-        if (MODE == 0) {
-            const float4 p = prec[s];
-            const float dx = p.x - x, dy = p.y - y, dz = p.z - z;
-            const float d = sqrtf(dx * dx + dy * dy + dz * dz);   // EuclideanDistance.cu:20-28
-            consider(d, s);
-        } else {
-            const TriRec r = trec[s];
-            const float d2 = ftd::tri_dist2_f32(x, y, z, r);
-            consider(d2, s);
-        }
-    }
-
     // Leaf test at a wave-uniform leaf: records read through scalar loads.
     __device__ void test_leaf_uniform(int l)
     {
@@ -871,28 +928,17 @@ struct Searcher {
         }
     }
 
-    __device__ void test_leaf(int l)
-    {
-        const uint32_t b = leaf_start[l], e = b + leaf_cnt[l];
-        n_leaves++;
-        n_prims += e - b;
-        for (uint32_t s = b; s < e; s++) test_prim(s);
-    }
 };
 
 // Epilogue of one node (EuclideanDistance.cu:54-57, LevelSet.cpp:438-448, 74,
 // 148): sign from phi_old < 0, retained value at crossing-edge nodes, blend.
 template <int MODE>
-__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t i)
+__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t node)
 {
     float d;
     if (MODE == 0) d = S.best;
     else d = S.bj >= 0 ? sqrtf(S.best) : (a.band > 0.0f ? a.band : FLT_MAX);
-    if (a.band > 0.0f && S.bj < 0) {
-        d = a.band;
-        atomicAdd(a.n_clamped, 1ull);
-    }
-    const int64_t node = a.perm[i];
+    if (a.band > 0.0f && S.bj < 0) d = a.band;
     const double po = a.phi_old[node];
     const double dn = (po < 0.0) ? -(double)d : (double)d;
     if (a.phi_ifn) a.phi_ifn[node] = dn;
@@ -906,27 +952,14 @@ __device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<M
 }
 
 template <int MODE>
-__device__ __forceinline__ void init_searcher(Searcher<MODE>& S, const QueryArgs& a, float px,
-                                              float py, float pz)
-{
-    S.x = px; S.y = py; S.z = pz;
-    S.bj = -1;
-    if (MODE == 0) S.best = a.band > 0.0f ? a.band : FLT_MAX;
-    else S.best = a.band > 0.0f ? a.band * a.band : INFINITY;
-    S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
-    S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
-    S.ifp = a.ifp; S.tris = a.tris;
-}
-
-template <int MODE>
 __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<MODE>& S,
-                                             int64_t i, uint32_t n_nodes)
+                                             int64_t node, uint32_t n_nodes)
 {
     // buckets: distance < 2h, < 8h, < 32h, beyond; +4 when phi < 0
     const float dd = MODE == 0 ? S.best : sqrtf(S.best);
     const float hh = a.h_est;
     int bkt = dd < 2 * hh ? 0 : dd < 8 * hh ? 1 : dd < 32 * hh ? 2 : 3;
-    if (a.phi_old[a.perm[i]] < 0.0) bkt += 4;
+    if (a.phi_old[node] < 0.0) bkt += 4;
     unsigned long long* st = a.stats + 4 * bkt;
     atomicAdd(&st[0], 1ull);
     atomicAdd(&st[1], (unsigned long long)n_nodes);
@@ -934,36 +967,41 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
     atomicAdd(&st[3], (unsigned long long)S.n_prims);
 }
 
-// Wave-coherent search: the 64 lanes of a wave hold 64 consecutive nodes in
-// Morton order (a compact tile) and walk the tree together.  A child is
-// entered when any lane's bound admits it; each lane keeps its own best.  All
-// control flow is wave-uniform (ballots), node and primitive records are read
-// at wave-uniform addresses, the stack is one per wave in LDS.
+// Wave-coherent search: the 64 lanes of a wave hold 64 consecutive slots of
+// one field in Morton order (a compact tile) and walk that field's tree
+// together.  A child is entered when any lane's bound admits it; each lane
+// keeps its own best.  All control flow is wave-uniform (ballots), node and
+// primitive records are read at wave-uniform addresses through scalar loads,
+// the stack is one per wave in LDS.
 template <int MODE>
 __global__ __launch_bounds__(kQueryBS) void k_query_wave(QueryArgs a)
 {
     __shared__ int32_t wstack[kQueryBS / 64][kStack];
     const int w = threadIdx.x >> 6;
     const int64_t base = (int64_t)blockIdx.x * kQueryBS + (threadIdx.x & ~63);
-    if (base >= a.n) return;                          // whole wave out of range
-    const int64_t gid = base + (threadIdx.x & 63);
-    const int64_t i = gid < a.n ? gid : a.n - 1;      // tail lanes shadow the last node
-    const float px = a.sx[i], py = a.sy[i], pz = a.sz[i];
+    if (base >= a.n_slots) return;                    // whole wave out of range
+    const int64_t slot = base + (threadIdx.x & 63);
+    const int32_t node = a.perm[slot];                 // -1: padding lane
+    const float px = a.sx[slot], py = a.sy[slot], pz = a.sz[slot];
+    const int field = __builtin_amdgcn_readfirstlane(a.tile_field[base >> 6]);
+    const int32_t root = __builtin_amdgcn_readfirstlane(ftd::cload(a.roots, field));
     Searcher<MODE> S;
-    init_searcher<MODE>(S, a, px, py, pz);
-    if (a.seed_in) {
-        const int32_t s0 = a.seed_in[base >> 6];
-        if (s0 >= 0) S.test_prim((uint32_t)s0);
-    }
+    S.x = px; S.y = py; S.z = pz;
+    S.bj = -1;
+    if (MODE == 0) S.best = a.band > 0.0f ? a.band : FLT_MAX;
+    else S.best = a.band > 0.0f ? a.band * a.band : INFINITY;
+    S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
+    S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
+    S.ifp = a.ifp; S.tris = a.tris;
     uint32_t n_nodes = 0;
-    if (a.L == 1) {
-        S.test_leaf_uniform(0);
-    } else if (a.L > 1) {
+    if (root != INT32_MIN && root < 0) {
+        S.test_leaf_uniform(~root);
+    } else if (root >= 0) {
         if (a.probe) {
-            // Greedy descent for the tile's middle node to the leaf nearest to it
-            // by bound: a first exact candidate for every lane.
+            // Greedy descent for the tile's middle slot to the leaf nearest to
+            // it by bound: a first exact candidate for every lane.
             const float cx = __shfl(px, 32), cy = __shfl(py, 32), cz = __shfl(pz, 32);
-            int cur = 0;
+            int cur = root;
             for (int depth = 0; depth < kStack + 1 && cur >= 0; depth++) {
                 const BNode nd = ftd::cload(a.nodes, cur);
                 const float ll = ftd::bound_dist2(cx, cy, cz, nd.l);
@@ -973,7 +1011,7 @@ __global__ __launch_bounds__(kQueryBS) void k_query_wave(QueryArgs a)
             if (cur < 0) S.test_leaf_uniform(~cur);
         }
         int sp = 0;
-        int cur = 0;
+        int cur = root;
         int budget = 2 * a.L + 8;
         while (--budget > 0) {
             const BNode nd = ftd::cload(a.nodes, cur);    // scalar loads: cur is uniform
@@ -1009,72 +1047,174 @@ __global__ __launch_bounds__(kQueryBS) void k_query_wave(QueryArgs a)
         }
         if (budget <= 0) atomicOr(a.overflow, 2u);
     }
-    if (a.stats && gid < a.n) record_stats<MODE>(a, S, i, n_nodes);
-    if (gid < a.n) finish_node<MODE>(a, S, i);
+    const bool clamped = node >= 0 && a.band > 0.0f && S.bj < 0;
+    const uint64_t cm = __ballot(clamped);
+    if ((threadIdx.x & 63) == 0 && cm) atomicAdd(&a.n_clamped[field], (unsigned long long)__popcll(cm));
+    if (node < 0) return;
+    if (a.stats) record_stats<MODE>(a, S, node, n_nodes);
+    finish_node<MODE>(a, S, node);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kQueryBS) void k_query(QueryArgs a)
+// ============================================================================
+// Per-field interface counts (batches): ranks at the field boundaries
+// ============================================================================
+__device__ __forceinline__ uint64_t low_mask(int b) { return b ? ((1ull << b) - 1ull) : 0ull; }
+
+// crossing tets before tet t (t <= T): block prefix + words of the block
+__device__ inline uint32_t tet_rank(int64_t t, int64_t T, const uint64_t* tbits,
+                                    const uint64_t* blk_prefix, uint32_t X)
 {
-    __shared__ int32_t stack_lds[kStack][kQueryBS];
-    const int64_t gid = (int64_t)blockIdx.x * kQueryBS + threadIdx.x;
-    // Seeding pass: one query per tile of 64 consecutive (Morton-ordered) nodes.
-    const int64_t ntiles = (a.n + 63) >> 6;
-    if (a.rep ? gid >= ntiles : gid >= a.n) return;
-    const int64_t i = a.rep ? min(gid * 64 + 32, a.n - 1) : gid;
-    const float px = a.sx[i], py = a.sy[i], pz = a.sz[i];
-    Searcher<MODE> S;
-    init_searcher<MODE>(S, a, px, py, pz);
-    // The tile's seed is one exact candidate: a tight first bound for pruning.
-    if (a.seed_in) {
-        const int32_t s0 = a.seed_in[i >> 6];
-        if (s0 >= 0) S.test_prim((uint32_t)s0);
-    }
+    if (t >= T) return X;
+    uint32_t r = (uint32_t)(blk_prefix[t >> 8] >> 32);
+    for (int64_t w = (t >> 8) << 2; w < (t >> 6); w++) r += __popcll(tbits[w]);
+    return r + __popcll(tbits[t >> 6] & low_mask((int)(t & 63)));
+}
 
-    if (a.L == 1) {
-        const Bound lb = a.leaf_b[0];
-        if (ftd::bound_dist2(px, py, pz, lb) <= prune_bound(S.best2())) S.test_leaf(0);
-    } else if (a.L > 1) {
-        int sp = 0;
-        int cur = 0;
-        // A binary tree over L leaves has 2L-1 nodes; no traversal visits more.
-        int budget = 2 * a.L + 8;
-        uint32_t n_nodes = 0;
-        while (--budget > 0) {
-            const BNode nd = a.nodes[cur];
-            n_nodes++;
-            const float ll = ftd::bound_dist2(px, py, pz, nd.l);
-            const float lr = ftd::bound_dist2(px, py, pz, nd.r);
-            int c0 = nd.left, c1 = nd.right;
-            float l0 = ll, l1 = lr;
-            if (lr < ll) { c0 = nd.right; c1 = nd.left; l0 = lr; l1 = ll; }
-            int next = -1;
-            // nearer child first; leaves are tested at once
-            if (l0 <= prune_bound(S.best2())) {
-                if (c0 < 0) S.test_leaf(~c0);
-                else next = c0;
-            }
-            if (l1 <= prune_bound(S.best2())) {
-                if (c1 < 0) S.test_leaf(~c1);
-                else if (next < 0) next = c1;
-                else if (sp < kStack) stack_lds[sp++][threadIdx.x] = c1;
-                else atomicOr(a.overflow, 1u);
-            }
-            if (next < 0) {
-                if (sp == 0) break;
-                next = stack_lds[--sp][threadIdx.x];
-            }
-            cur = next;
+__device__ inline uint32_t edge_rank_at(int64_t e, int64_t E, const uint64_t* ebits,
+                                        const uint32_t* eprefix, uint32_t P)
+{
+    if (e >= E) return P;
+    return eprefix[e >> 6] + (uint32_t)__popcll(ebits[e >> 6] & low_mask((int)(e & 63)));
+}
+
+// out[3f..3f+2] = (crossing points, triangles, crossing tets) of field f
+__global__ void k_field_counts(int nf, const int64_t* tet_off, const int64_t* edge_off, int64_t T,
+                               int64_t E, const uint64_t* tbits, const uint64_t* blk_prefix,
+                               const int32_t* xtri_off, const uint64_t* ebits,
+                               const uint32_t* eprefix, uint32_t X, uint32_t F, uint32_t P,
+                               int64_t* out)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const uint32_t x0 = tet_rank(tet_off[f], T, tbits, blk_prefix, X);
+    const uint32_t x1 = tet_rank(tet_off[f + 1], T, tbits, blk_prefix, X);
+    const uint32_t f0 = x0 < X ? (uint32_t)xtri_off[x0] : F;
+    const uint32_t f1 = x1 < X ? (uint32_t)xtri_off[x1] : F;
+    const uint32_t p0 = edge_rank_at(edge_off[f], E, ebits, eprefix, P);
+    const uint32_t p1 = edge_rank_at(edge_off[f + 1], E, ebits, eprefix, P);
+    out[3 * f + 0] = p1 - p0;
+    out[3 * f + 1] = f1 - f0;
+    out[3 * f + 2] = x1 - x0;
+}
+
+// ============================================================================
+// Kuhn grids generated on the device, numbered exactly as ft_kuhn_mesh
+// ============================================================================
+struct KuhnField {
+    int nx, ny, nz, pad;
+    double origin[3], spacing[3];
+    double jitter;
+    uint64_t seed;
+    int64_t node_off, tet_off, edge_off;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__constant__ int c_kdir[7][3] = {{1,0,0},{0,1,0},{0,0,1},{1,1,0},{1,0,1},{0,1,1},{1,1,1}};
+__constant__ int c_kperm[6][3] = {{0,1,2},{0,2,1},{1,0,2},{1,2,0},{2,0,1},{2,1,0}};
+
+// Edges numbered by low node (x-major), then direction x,y,z,xy,xz,yz,xyz; a
+// node has (1+I)(1+J)(1+K)-1 of them (I = not on the upper x face, ...), so the
+// count before node (i,j,k) factorises over the axes.
+__device__ inline int64_t kuhn_edges_before(int64_t i, int64_t j, int64_t k, int64_t X, int64_t Y,
+                                            int64_t Z)
+{
+    auto S = [](int64_t q, int64_t Q) { return q + (q < Q - 1 ? q : Q - 1); };
+    const int64_t I = i < X - 1, J = j < Y - 1;
+    return (S(i, X) * S(Y, Y) * S(Z, Z) - i * Y * Z) + ((1 + I) * S(j, Y) * S(Z, Z) - j * Z) +
+           ((1 + I) * (1 + J) * S(k, Z) - k);
+}
+
+__device__ inline bool kuhn_valid(int64_t i, int64_t j, int64_t k, int d, int64_t X, int64_t Y,
+                                  int64_t Z)
+{
+    return (c_kdir[d][0] == 0 || i < X - 1) && (c_kdir[d][1] == 0 || j < Y - 1) &&
+           (c_kdir[d][2] == 0 || k < Z - 1);
+}
+
+__device__ inline int32_t kuhn_edge_id(int64_t i, int64_t j, int64_t k, int d, int64_t X, int64_t Y,
+                                       int64_t Z)
+{
+    int r = 0;
+    for (int q = 0; q < d; q++) r += kuhn_valid(i, j, k, q, X, Y, Z);
+    return (int32_t)(kuhn_edges_before(i, j, k, X, Y, Z) + r);
+}
+
+__global__ __launch_bounds__(kBS) void k_kuhn_nodes(KuhnField f, float* xyz)
+{
+    const int64_t X = f.nx, Y = f.ny, Z = f.nz;
+    const int64_t a = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (a >= X * Y * Z) return;
+    const int64_t i = a / (Y * Z), j = (a / Z) % Y, k = a % Z;
+    const bool interior = i > 0 && i < X - 1 && j > 0 && j < Y - 1 && k > 0 && k < Z - 1;
+    const int64_t ijk[3] = {i, j, k};
+    float* out = xyz + 3 * (f.node_off + a);
+    for (int c = 0; c < 3; c++) {
+        double g = (double)ijk[c];
+        if (f.jitter > 0.0 && interior) {
+            const uint64_t z = mix64(f.seed + (3 * (uint64_t)a + c + 1) * 0x9E3779B97F4A7C15ULL);
+            const double u = (double)(z >> 11) * 0x1.0p-53;
+            g += (2.0 * u - 1.0) * f.jitter;
         }
-        if (budget <= 0) atomicOr(a.overflow, 2u);
-        if (a.stats && !a.rep) record_stats<MODE>(a, S, i, n_nodes);
+        out[c] = (float)(f.origin[c] + g * f.spacing[c]);
     }
+}
 
-    if (a.rep) {
-        a.seed_out[gid] = S.bslot;
-        return;
+__global__ __launch_bounds__(kBS) void k_kuhn_edges(KuhnField f, int2* edges)
+{
+    const int64_t X = f.nx, Y = f.ny, Z = f.nz;
+    const int64_t a = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (a >= X * Y * Z) return;
+    const int64_t i = a / (Y * Z), j = (a / Z) % Y, k = a % Z;
+    int64_t e = f.edge_off + kuhn_edges_before(i, j, k, X, Y, Z);
+    for (int d = 0; d < 7; d++) {
+        if (!kuhn_valid(i, j, k, d, X, Y, Z)) continue;
+        const int64_t b = ((i + c_kdir[d][0]) * Y + (j + c_kdir[d][1])) * Z + (k + c_kdir[d][2]);
+        edges[e++] = make_int2((int32_t)(f.node_off + a), (int32_t)(f.node_off + b));
     }
-    finish_node<MODE>(a, S, i);
+}
+
+__global__ __launch_bounds__(kBS) void k_kuhn_tets(KuhnField f, int4* tets, int32_t* t2e)
+{
+    const int64_t X = f.nx, Y = f.ny, Z = f.nz;
+    const int64_t CX = X - 1, CY = Y - 1, CZ = Z - 1;
+    const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (t >= 6 * CX * CY * CZ) return;
+    const int64_t cell = t / 6;
+    const int p = (int)(t % 6);
+    int64_t v[4][3];
+    v[0][0] = cell / (CY * CZ); v[0][1] = (cell / CZ) % CY; v[0][2] = cell % CZ;
+    for (int q = 1; q <= 3; q++) {
+        for (int c = 0; c < 3; c++) v[q][c] = v[q - 1][c];
+        v[q][c_kperm[p][q - 1]] += 1;
+    }
+    int32_t nd[4];
+    for (int q = 0; q < 4; q++) nd[q] = (int32_t)(f.node_off + (v[q][0] * Y + v[q][1]) * Z + v[q][2]);
+    tets[f.tet_off + t] = make_int4(nd[0], nd[1], nd[2], nd[3]);
+    const int loc[6][2] = {{0,1},{0,2},{0,3},{1,2},{1,3},{2,3}};
+    for (int q = 0; q < 6; q++) {
+        const int64_t* lo = v[loc[q][0]];
+        const int64_t* hi = v[loc[q][1]];
+        const int dx = (int)(hi[0] - lo[0]), dy = (int)(hi[1] - lo[1]), dz = (int)(hi[2] - lo[2]);
+        int d = 0;
+        for (int r = 0; r < 7; r++)
+            if (c_kdir[r][0] == dx && c_kdir[r][1] == dy && c_kdir[r][2] == dz) d = r;
+        t2e[6 * (f.tet_off + t) + q] = (int32_t)(f.edge_off + kuhn_edge_id(lo[0], lo[1], lo[2], d, X, Y, Z));
+    }
+}
+
+// Localised copies of index arrays (local -> global): mesh uploads and batch
+// assembly from resident meshes.
+__global__ __launch_bounds__(kBS) void k_add_offset(const int32_t* src, int64_t n, int64_t add,
+                                                    int32_t* dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i < n) dst[i] = (int32_t)(src[i] + add);
 }
 
 // ============================================================================
@@ -1088,27 +1228,37 @@ struct DevBuf {
 
 }  // namespace
 
+struct FieldHost {
+    float lo[3], hi[3];
+    float h;                         // mean node spacing estimate
+};
+
 struct ft_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // mesh
-    int64_t N = 0, T = 0, E = 0;
+    // fields and mesh (field k: nodes [node_off[k], node_off[k+1]), ...)
+    int nf = 0;
+    std::vector<int64_t> node_off, tet_off, edge_off, slot_off;
+    std::vector<FieldHost> fh;
+    int64_t N = 0, T = 0, E = 0, n_slots = 0;
     bool has_conn = false;
-    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     float h_est = 0.0f;
-    DevBuf xyz, tets, edges, t2e, perm, sx, sy, sz;
+    DevBuf xyz, tets, edges, t2e, perm, sx, sy, sz, tile_field;
+    DevBuf d_node_off, d_tet_off, d_edge_off, d_slot_off;
     // per reinit
     DevBuf phi_tmp, phi_ifn_tmp;
     DevBuf sbits, retain, tbits, blk, xtet, xtri_off, xedges, ebits, eprefix;
     DevBuf ifp, ifp2e, tris, tri_norm, if_elems, p2t_off, p2t_cursor, p2t, ifp_area;
-    DevBuf nearest;
+    DevBuf nearest, fcounts, fclamp;
     // index
     DevBuf bin_count, bin_prefix, pcode, prec, trec, pidx, leaf_key, leaf_start, leaf_cnt;
-    DevBuf leaf_b, leaf_parent, bnodes, bflags, seeds, stats;
+    DevBuf leaf_b, leaf_parent, bnodes, bflags, stats;
+    DevBuf d_grids, d_cnt_off, leaf_off, roots;
     // scan scratch and small scalars
     DevBuf scan_bsum, scalars;
     uint64_t* h_scalars = nullptr;   // pinned mirror
     ft_counts last{};
+    std::vector<ft_counts> field_counts;
     bool have_interface = false;
     bool have_nearest = false;
     // phase timing (ft_set_timing): events at the phase boundaries of a reinit
@@ -1173,25 +1323,33 @@ int scan(ft_ctx* c, hipStream_t s, const TI* in, TO* out, int64_t n, TO* d_total
     return FT_OK;
 }
 
-// Uniform cubic grid over the node box with edge >= `edge`, <= 256 per axis.
-Grid make_grid(const ft_ctx* c, float edge, int64_t* G)
+// Uniform cubic grid over a field's box with edge >= `edge` and at most
+// 2^max_lg bins per axis; *lg receives log2 of the Morton side.
+Grid make_grid(const FieldHost& f, float edge, int max_lg, int* lg_out)
 {
     Grid g{};
     float ext_max = 0.0f;
-    for (int k = 0; k < 3; k++) ext_max = std::max(ext_max, c->hi[k] - c->lo[k]);
-    float b = std::max(edge, ext_max / (float)(1 << kMaxBinsLog2PerAxis) * 1.0001f);
+    for (int k = 0; k < 3; k++) ext_max = std::max(ext_max, f.hi[k] - f.lo[k]);
+    float b = std::max(edge, ext_max / (float)(1 << max_lg) * 1.0001f);
     if (!(b > 0.0f)) b = 1.0f;
     int lg = 0;
     for (int k = 0; k < 3; k++) {
-        int d = (int)std::ceil((c->hi[k] - c->lo[k]) / b);
-        d = std::max(1, std::min(d, 1 << kMaxBinsLog2PerAxis));
+        int d = (int)std::ceil((f.hi[k] - f.lo[k]) / b);
+        d = std::max(1, std::min(d, 1 << max_lg));
         g.dim[k] = d;
-        g.lo[k] = c->lo[k];
+        g.lo[k] = f.lo[k];
         while ((1 << lg) < d) lg++;
     }
     g.inv_b = 1.0f / b;
-    *G = (int64_t)1 << (3 * lg);
+    *lg_out = lg;
     return g;
+}
+
+int field_bits(int nf)
+{
+    int b = 0;
+    while ((1 << b) < nf) b++;
+    return b;
 }
 
 int check_launch() { FT_HIP(hipGetLastError()); return FT_OK; }
@@ -1210,21 +1368,53 @@ void mark(ft_ctx* c, int k, hipStream_t s)
 }
 
 // Scalar slots in ctx->scalars.
-enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_CLAMP = 5, S_OVF = 6, S_STATS = 8, S_N = 12 };
+enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_OVF = 6, S_N = 8 };
+
+int check_handle(ft_handle h)
+{
+    if (!h) return ft::set_error(FT_ERR_ARG, "null handle");
+    FT_HIP(hipSetDevice(h->device));
+    return FT_OK;
+}
 
 // Builds the bin index over `n` primitives (mode 0: points in ifp, mode 1:
-// triangles tris/ifp) and runs the query with the epilogue.
+// triangles tris/ifp), one uniform grid per field, and runs the query with the
+// epilogue over every node.
 int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, const int4* tris,
-           float bin_scale, float band, const double* phi_old, double* phi_out, double* phi_ifn,
-           const uint64_t* retain, int epi, double w_phi, double w_ifn, bool want_nearest,
-           int64_t* n_clamped)
+           const int32_t* ifp2e, float bin_scale, float band, const double* phi_old,
+           double* phi_out, double* phi_ifn, const uint64_t* retain, int epi, double w_phi,
+           double w_ifn, bool want_nearest)
 {
     uint64_t* sc = c->scalars.as<uint64_t>();
+    const int nf = c->nf;
     int64_t L = 0;
+    FT_TRY(ensure(c->roots, sizeof(int32_t) * nf));
+    FT_TRY(ensure(c->leaf_off, sizeof(int32_t) * (nf + 1)));
+    FT_TRY(ensure(c->fclamp, sizeof(uint64_t) * nf));
     if (n > 0) {
-        const float edge = (bin_scale > 0.0f ? bin_scale : 1.0f) * c->h_est;
-        int64_t G = 0;
-        const Grid g = make_grid(c, edge, &G);
+        // per-field grids; keys (field << code_bits) | code stay below 2^30
+        const int fb = field_bits(nf);
+        const int max_lg = std::min(kMaxBinsLog2PerAxis, (30 - fb) / 3);
+        std::vector<Grid> grids(nf);
+        std::vector<int64_t> cnt_off(nf + 1, 0);
+        int lg_max = 0;
+        for (int k = 0; k < nf; k++) {
+            int lg = 0;
+            const float edge = (bin_scale > 0.0f ? bin_scale : 1.0f) * c->fh[k].h;
+            grids[k] = make_grid(c->fh[k], edge, max_lg, &lg);
+            lg_max = std::max(lg_max, lg);
+            cnt_off[k + 1] = cnt_off[k] + ((int64_t)1 << (3 * lg));
+        }
+        const int64_t G = cnt_off[nf];
+        if (G > ((int64_t)1 << 32))
+            return ft::set_error(FT_ERR_ARG, "index: %lld bins exceed 2^32; raise bin_scale", (long long)G);
+        FT_TRY(ensure(c->d_grids, sizeof(Grid) * nf));
+        FT_TRY(ensure(c->d_cnt_off, sizeof(int64_t) * (nf + 1)));
+        FT_HIP(hipMemcpyAsync(c->d_grids.p, grids.data(), sizeof(Grid) * nf, hipMemcpyHostToDevice, s));
+        FT_HIP(hipMemcpyAsync(c->d_cnt_off.p, cnt_off.data(), sizeof(int64_t) * (nf + 1),
+                              hipMemcpyHostToDevice, s));
+        IndexGeom ig{c->d_edge_off.as<int64_t>(), c->d_cnt_off.as<int64_t>(), c->d_grids.as<Grid>(), nf,
+                     3 * lg_max};
         FT_TRY(ensure(c->bin_count, sizeof(uint32_t) * G));
         FT_TRY(ensure(c->bin_prefix, sizeof(uint64_t) * G));
         FT_TRY(ensure(c->pcode, sizeof(uint32_t) * n));
@@ -1234,9 +1424,12 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
         uint32_t* cnt = c->bin_count.as<uint32_t>();
         uint64_t* pre = c->bin_prefix.as<uint64_t>();
         FT_TRY(fill<uint32_t>(s, cnt, G, 0u));
-        hipLaunchKernelGGL(k_prim_bin, dim3(grid1d(n)), dim3(kBS), 0, s, mode, n, ifp, tris, g,
-                           c->pcode.as<uint32_t>(), cnt);
+        hipLaunchKernelGGL(k_prim_bin, dim3(grid1d(n)), dim3(kBS), 0, s, mode, n, ifp, tris,
+                           nf > 1 ? ifp2e : nullptr, ig, c->pcode.as<uint32_t>(), cnt);
         FT_TRY(scan<uint32_t, uint64_t>(c, s, cnt, pre, G, sc + S_BINS, LoadNonEmpty{}));
+        hipLaunchKernelGGL(k_field_roots, dim3((nf + 1 + 63) / 64), dim3(64), 0, s, nf,
+                           c->d_cnt_off.as<int64_t>(), pre, sc + S_BINS, G,
+                           c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
         FT_TRY(read_scalars(c, s, S_N));
         L = (int64_t)(c->h_scalars[S_BINS] >> 32);
         FT_TRY(ensure(c->leaf_key, sizeof(uint32_t) * L));
@@ -1246,7 +1439,7 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
         FT_TRY(ensure(c->leaf_parent, sizeof(int32_t) * L));
         FT_TRY(ensure(c->bnodes, sizeof(BNode) * std::max<int64_t>(1, L - 1)));
         FT_TRY(ensure(c->bflags, sizeof(unsigned) * std::max<int64_t>(1, L - 1)));
-        hipLaunchKernelGGL(k_leaves, dim3(grid1d(G)), dim3(kBS), 0, s, cnt, pre, G,
+        hipLaunchKernelGGL(k_leaves, dim3(grid1d(G)), dim3(kBS), 0, s, cnt, pre, G, ig,
                            c->leaf_key.as<uint32_t>(), c->leaf_start.as<uint32_t>(),
                            c->leaf_cnt.as<uint32_t>());
         hipLaunchKernelGGL(k_prim_scatter, dim3(grid1d(n)), dim3(kBS), 0, s, mode, n, ifp, tris,
@@ -1259,19 +1452,24 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
             FT_TRY(fill<unsigned>(s, c->bflags.as<unsigned>(), L - 1, 0u));
             hipLaunchKernelGGL(k_karras, dim3(grid1d(L - 1)), dim3(kBS), 0, s,
                                c->leaf_key.as<uint32_t>(), (int)L, c->bnodes.as<BNode>(),
-                               c->leaf_parent.as<int32_t>());
+                               c->leaf_parent.as<int32_t>(), 3 * lg_max,
+                               c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
             hipLaunchKernelGGL(k_refit, dim3(grid1d(L)), dim3(kBS), 0, s, (int)L,
                                c->leaf_parent.as<int32_t>(), c->leaf_b.as<Bound>(),
-                               c->bnodes.as<BNode>(),
-                               c->bflags.as<unsigned>());
+                               c->bnodes.as<BNode>(), c->bflags.as<unsigned>());
         }
         FT_TRY(check_launch());
+    } else {
+        FT_TRY(fill<int32_t>(s, c->roots.as<int32_t>(), nf, INT32_MIN));
     }
-    FT_TRY(fill<uint64_t>(s, sc + S_CLAMP, 2, 0ull));
+    FT_TRY(fill<uint64_t>(s, c->fclamp.as<uint64_t>(), nf, 0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
     QueryArgs q{};
-    q.n = c->N;
+    q.n_slots = c->n_slots;
     q.sx = c->sx.as<float>(); q.sy = c->sy.as<float>(); q.sz = c->sz.as<float>();
     q.perm = c->perm.as<int32_t>();
+    q.tile_field = c->tile_field.as<int32_t>();
+    q.roots = c->roots.as<int32_t>();
     q.prec = c->prec.as<float4>();
     q.trec = c->trec.as<TriRec>();
     q.pidx = c->pidx.as<int32_t>();
@@ -1294,7 +1492,7 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
     q.epi = epi;
     q.w_phi = w_phi;
     q.w_ifn = w_ifn;
-    q.n_clamped = (unsigned long long*)(sc + S_CLAMP);
+    q.n_clamped = c->fclamp.as<unsigned long long>();
     static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
     q.h_est = c->h_est;
     if (want_stats) {
@@ -1303,36 +1501,13 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
         q.stats = c->stats.as<unsigned long long>();
     }
     q.overflow = (unsigned*)(sc + S_OVF);
-    const int64_t ntiles = (c->N + 63) / 64;
-    const bool seeded = L > 1 && c->N >= 4096;
-    static const int kind = getenv("FT_QUERY_KERNEL") && !strcmp(getenv("FT_QUERY_KERNEL"), "lane") ? 0 : 1;
-    static const bool use_seed = getenv("FT_SEED") && !strcmp(getenv("FT_SEED"), "1");
     q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     mark(c, 2, s);
-    if (seeded && use_seed) {
-        FT_TRY(ensure(c->seeds, sizeof(int32_t) * ntiles));
-        QueryArgs qr = q;
-        qr.rep = 1;
-        qr.seed_out = c->seeds.as<int32_t>();
-        qr.stats = nullptr;
+    if (c->n_slots > 0) {
         if (mode == 0)
-            hipLaunchKernelGGL(k_query<0>, dim3(grid1d(ntiles, kQueryBS)), dim3(kQueryBS), 0, s, qr);
+            hipLaunchKernelGGL(k_query_wave<0>, dim3(grid1d(c->n_slots, kQueryBS)), dim3(kQueryBS), 0, s, q);
         else
-            hipLaunchKernelGGL(k_query<1>, dim3(grid1d(ntiles, kQueryBS)), dim3(kQueryBS), 0, s, qr);
-        q.seed_in = c->seeds.as<int32_t>();
-    }
-    if (c->N > 0) {
-        if (kind == 1) {
-            if (mode == 0)
-                hipLaunchKernelGGL(k_query_wave<0>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
-            else
-                hipLaunchKernelGGL(k_query_wave<1>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
-        } else {
-            if (mode == 0)
-                hipLaunchKernelGGL(k_query<0>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
-            else
-                hipLaunchKernelGGL(k_query<1>, dim3(grid1d(c->N, kQueryBS)), dim3(kQueryBS), 0, s, q);
-        }
+            hipLaunchKernelGGL(k_query_wave<1>, dim3(grid1d(c->n_slots, kQueryBS)), dim3(kQueryBS), 0, s, q);
         FT_TRY(check_launch());
     }
     mark(c, 3, s);
@@ -1348,71 +1523,121 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
                 "leaves %.1f, primitives %.1f\n", mode, (long long)c->N, (long long)L,
                 tn / c->N, tl / c->N, tp / c->N);
         for (int b = 0; b < 8; b++) {
-            const double n = (double)st[4 * b];
-            if (n > 0)
+            const double nn = (double)st[4 * b];
+            if (nn > 0)
                 fprintf(stderr, "[ftreinit]   %-9s %5.1f%% of nodes: tree %.1f leaves %.1f prims %.1f\n",
-                        names[b], 100.0 * n / c->N, st[4 * b + 1] / n, st[4 * b + 2] / n,
-                        st[4 * b + 3] / n);
+                        names[b], 100.0 * nn / c->N, st[4 * b + 1] / nn, st[4 * b + 2] / nn,
+                        st[4 * b + 3] / nn);
         }
     }
     if (c->h_scalars[S_OVF])
         return ft::set_error(FT_ERR_HIP, "search: traversal exceeded its bounds (%llu)",
                              (unsigned long long)c->h_scalars[S_OVF]);
-    *n_clamped = (int64_t)c->h_scalars[S_CLAMP];
+    std::vector<uint64_t> clamp(nf);
+    FT_HIP(hipMemcpy(clamp.data(), c->fclamp.p, 8 * nf, hipMemcpyDeviceToHost));
+    c->field_counts.resize(nf);
+    for (int k = 0; k < nf; k++) c->field_counts[k].n_clamped = (int64_t)clamp[k];
     c->have_nearest = want_nearest;
     return FT_OK;
 }
 
-int check_handle(ft_handle h)
+// Field layout: offsets on host and device, slot layout (each field padded to
+// whole waves).
+int set_fields(ft_ctx* c, int nf, const int64_t* node_off, const int64_t* tet_off,
+               const int64_t* edge_off)
 {
-    if (!h) return ft::set_error(FT_ERR_ARG, "null handle");
-    FT_HIP(hipSetDevice(h->device));
+    c->nf = nf;
+    c->node_off.assign(node_off, node_off + nf + 1);
+    c->tet_off.assign(tet_off, tet_off + nf + 1);
+    c->edge_off.assign(edge_off, edge_off + nf + 1);
+    c->slot_off.assign(nf + 1, 0);
+    for (int k = 0; k < nf; k++)
+        c->slot_off[k + 1] = c->slot_off[k] + ((node_off[k + 1] - node_off[k] + 63) / 64) * 64;
+    c->N = node_off[nf];
+    c->T = tet_off[nf];
+    c->E = edge_off[nf];
+    c->n_slots = c->slot_off[nf];
+    c->fh.assign(nf, FieldHost{});
+    c->field_counts.assign(nf, ft_counts{});
+    const size_t ob = sizeof(int64_t) * (nf + 1);
+    FT_TRY(ensure(c->d_node_off, ob));
+    FT_TRY(ensure(c->d_tet_off, ob));
+    FT_TRY(ensure(c->d_edge_off, ob));
+    FT_TRY(ensure(c->d_slot_off, ob));
+    FT_HIP(hipMemcpyAsync(c->d_node_off.p, c->node_off.data(), ob, hipMemcpyHostToDevice, c->stream));
+    FT_HIP(hipMemcpyAsync(c->d_tet_off.p, c->tet_off.data(), ob, hipMemcpyHostToDevice, c->stream));
+    FT_HIP(hipMemcpyAsync(c->d_edge_off.p, c->edge_off.data(), ob, hipMemcpyHostToDevice, c->stream));
+    FT_HIP(hipMemcpyAsync(c->d_slot_off.p, c->slot_off.data(), ob, hipMemcpyHostToDevice, c->stream));
+    FT_HIP(hipStreamSynchronize(c->stream));
     return FT_OK;
 }
 
-int upload_nodes(ft_ctx* c, const float* xyz, int64_t n)
+void finish_field_box(FieldHost& f, int64_t n)
 {
-    c->N = n;
-    FT_TRY(ensure(c->xyz, sizeof(float) * 3 * n));
-    FT_HIP(hipMemcpyAsync(c->xyz.p, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    // node box and spacing estimate
-    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (int64_t i = 0; i < n; i++)
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], xyz[3 * i + k]);
-            hi[k] = std::max(hi[k], xyz[3 * i + k]);
-        }
-    if (n == 0) for (int k = 0; k < 3; k++) { lo[k] = 0; hi[k] = 1; }
+    if (n == 0) for (int k = 0; k < 3; k++) { f.lo[k] = 0; f.hi[k] = 1; }
     double vol = 1.0, ext_max = 0.0;
-    for (int k = 0; k < 3; k++) ext_max = std::max(ext_max, (double)hi[k] - lo[k]);
+    for (int k = 0; k < 3; k++) ext_max = std::max(ext_max, (double)f.hi[k] - f.lo[k]);
     if (!(ext_max > 0.0)) ext_max = 1.0;
-    for (int k = 0; k < 3; k++) vol *= std::max((double)hi[k] - lo[k], ext_max * 1e-3);
-    for (int k = 0; k < 3; k++) { c->lo[k] = lo[k]; c->hi[k] = hi[k]; }
-    c->h_est = (float)std::cbrt(vol / (double)std::max<int64_t>(1, n));
-    // Morton order of the nodes
-    FT_TRY(ensure(c->perm, sizeof(int32_t) * n));
-    FT_TRY(ensure(c->sx, sizeof(float) * n));
-    FT_TRY(ensure(c->sy, sizeof(float) * n));
-    FT_TRY(ensure(c->sz, sizeof(float) * n));
-    if (n == 0) return FT_OK;
-    int64_t G = 0;
-    const Grid g = make_grid(c, c->h_est, &G);
-    FT_TRY(ensure(c->bin_count, sizeof(uint32_t) * G));
-    FT_TRY(ensure(c->bin_prefix, sizeof(uint64_t) * G));
-    FT_TRY(ensure(c->pcode, sizeof(uint32_t) * n));
-    uint32_t* cnt = c->bin_count.as<uint32_t>();
-    uint32_t* off = reinterpret_cast<uint32_t*>(c->bin_prefix.p);
-    FT_TRY(fill<uint32_t>(c->stream, cnt, G, 0u));
-    hipLaunchKernelGGL(k_node_bin, dim3(grid1d(n)), dim3(kBS), 0, c->stream, c->xyz.as<float>(), n, g,
-                       c->pcode.as<uint32_t>(), cnt);
-    FT_TRY(scan<uint32_t, uint32_t>(c, c->stream, cnt, off, G,
-                                    reinterpret_cast<uint32_t*>(c->scalars.as<uint64_t>() + S_BINS),
-                                    LoadSame{}));
-    hipLaunchKernelGGL(k_node_scatter, dim3(grid1d(n)), dim3(kBS), 0, c->stream, c->xyz.as<float>(), n,
-                       c->pcode.as<uint32_t>(), cnt, off, c->perm.as<int32_t>(), c->sx.as<float>(),
-                       c->sy.as<float>(), c->sz.as<float>());
+    for (int k = 0; k < 3; k++) vol *= std::max((double)f.hi[k] - f.lo[k], ext_max * 1e-3);
+    f.h = (float)std::cbrt(vol / (double)std::max<int64_t>(1, n));
+}
+
+// Field boxes from host coordinates.
+void boxes_from_host(ft_ctx* c, const float* xyz)
+{
+    for (int k = 0; k < c->nf; k++) {
+        FieldHost& f = c->fh[k];
+        for (int q = 0; q < 3; q++) { f.lo[q] = FLT_MAX; f.hi[q] = -FLT_MAX; }
+        for (int64_t i = c->node_off[k]; i < c->node_off[k + 1]; i++)
+            for (int q = 0; q < 3; q++) {
+                f.lo[q] = std::min(f.lo[q], xyz[3 * i + q]);
+                f.hi[q] = std::max(f.hi[q], xyz[3 * i + q]);
+            }
+        finish_field_box(f, c->node_off[k + 1] - c->node_off[k]);
+    }
+    c->h_est = c->nf ? c->fh[0].h : 0.0f;
+}
+
+// Morton order of each field's nodes into the query slots (device xyz):
+// one counting sort per field on a grid of about one node per bin.
+int sort_nodes(ft_ctx* c)
+{
+    const int nf = c->nf;
+    FT_TRY(ensure(c->perm, sizeof(int32_t) * std::max<int64_t>(1, c->n_slots)));
+    FT_TRY(ensure(c->sx, sizeof(float) * std::max<int64_t>(1, c->n_slots)));
+    FT_TRY(ensure(c->sy, sizeof(float) * std::max<int64_t>(1, c->n_slots)));
+    FT_TRY(ensure(c->sz, sizeof(float) * std::max<int64_t>(1, c->n_slots)));
+    FT_TRY(ensure(c->tile_field, sizeof(int32_t) * std::max<int64_t>(1, c->n_slots / 64)));
+    if (c->N == 0) return FT_OK;
+    hipStream_t s = c->stream;
+    for (int k = 0; k < nf; k++) {
+        const int64_t n = c->node_off[k + 1] - c->node_off[k];
+        if (n == 0) continue;
+        int lg = 0;
+        const Grid g = make_grid(c->fh[k], c->fh[k].h, kMaxBinsLog2PerAxis, &lg);
+        const int64_t G = (int64_t)1 << (3 * lg);
+        FT_TRY(ensure(c->bin_count, sizeof(uint32_t) * G));
+        FT_TRY(ensure(c->bin_prefix, sizeof(uint64_t) * G));
+        FT_TRY(ensure(c->pcode, sizeof(uint32_t) * n));
+        uint32_t* cnt = c->bin_count.as<uint32_t>();
+        uint32_t* off = reinterpret_cast<uint32_t*>(c->bin_prefix.p);
+        const float* xk = c->xyz.as<float>() + 3 * c->node_off[k];
+        const int64_t sb = c->slot_off[k];
+        FT_TRY(fill<uint32_t>(s, cnt, G, 0u));
+        hipLaunchKernelGGL(k_node_bin, dim3(grid1d(n)), dim3(kBS), 0, s, xk, n, g, c->pcode.as<uint32_t>(), cnt);
+        FT_TRY(scan<uint32_t, uint32_t>(c, s, cnt, off, G,
+                                        reinterpret_cast<uint32_t*>(c->scalars.as<uint64_t>() + S_BINS),
+                                        LoadSame{}));
+        hipLaunchKernelGGL(k_node_scatter, dim3(grid1d(n)), dim3(kBS), 0, s, xk, n, c->node_off[k],
+                           c->pcode.as<uint32_t>(), cnt, off, c->perm.as<int32_t>() + sb,
+                           c->sx.as<float>() + sb, c->sy.as<float>() + sb, c->sz.as<float>() + sb);
+    }
+    FieldGeom fg{c->d_node_off.as<int64_t>(), nullptr, nullptr, nf};
+    hipLaunchKernelGGL(k_pad_slots, dim3(grid1d(c->n_slots)), dim3(kBS), 0, s, c->n_slots, fg,
+                       c->d_slot_off.as<int64_t>(), c->perm.as<int32_t>(), c->sx.as<float>(),
+                       c->sy.as<float>(), c->sz.as<float>(), c->tile_field.as<int32_t>());
     FT_TRY(check_launch());
-    FT_HIP(hipStreamSynchronize(c->stream));
+    FT_HIP(hipStreamSynchronize(s));
     return FT_OK;
 }
 
@@ -1516,6 +1741,28 @@ int marching_tets(ft_ctx* c, hipStream_t s, const double* phi, ft_counts* cnt)
                              "tet %lld crosses the interface but all its nodes have |phi| <= "
                              "FLT_MIN (LevelSet.cpp:338-341)", (long long)cnt->bad_tet);
     }
+    // per-field counts (one field: the totals)
+    if (c->nf > 1) {
+        FT_TRY(ensure(c->fcounts, sizeof(int64_t) * 3 * c->nf));
+        hipLaunchKernelGGL(k_field_counts, dim3((c->nf + 63) / 64), dim3(64), 0, s, c->nf,
+                           c->d_tet_off.as<int64_t>(), c->d_edge_off.as<int64_t>(), T, E,
+                           c->tbits.as<uint64_t>(), c->blk.as<uint64_t>(), c->xtri_off.as<int32_t>(),
+                           c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(), (uint32_t)X,
+                           (uint32_t)F, (uint32_t)P, c->fcounts.as<int64_t>());
+        FT_TRY(check_launch());
+        std::vector<int64_t> fc(3 * c->nf);
+        FT_HIP(hipMemcpyAsync(fc.data(), c->fcounts.p, sizeof(int64_t) * 3 * c->nf, hipMemcpyDeviceToHost, s));
+        FT_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < c->nf; k++) {
+            c->field_counts[k] = ft_counts{};
+            c->field_counts[k].n_ifp = fc[3 * k];
+            c->field_counts[k].n_tri = fc[3 * k + 1];
+            c->field_counts[k].n_if_elems = fc[3 * k + 2];
+            c->field_counts[k].bad_tet = -1;
+        }
+    } else {
+        c->field_counts.assign(1, *cnt);
+    }
     c->have_interface = true;
     return FT_OK;
 }
@@ -1538,7 +1785,7 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
                   ft_counts* counts, hipStream_t s)
 {
     FT_TRY(validate_opts(o));
-    if (!c->has_conn) return ft::set_error(FT_ERR_STATE, "reinit before ft_set_mesh");
+    if (!c->has_conn) return ft::set_error(FT_ERR_STATE, "reinit before a mesh was set");
     if (!d_phi) return ft::set_error(FT_ERR_ARG, "null phi");
     c->have_interface = false;
     ft_counts cnt{};
@@ -1548,16 +1795,15 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     if (rc) return rc;
     // the query reads phi_old and writes phi_out in place per node
     const int epi = (o->blend == FT_BLEND_INIT) ? EPI_INIT : EPI_UPDATE;
-    int64_t clamped = 0;
     if (o->mode == FT_DIST_POINTS)
-        rc = search(c, s, 0, cnt.n_ifp, c->ifp.as<float>(), nullptr, o->bin_scale, (float)o->band,
-                    d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(), epi, o->w_phi, o->w_ifn,
-                    o->want_nearest != 0, &clamped);
+        rc = search(c, s, 0, cnt.n_ifp, c->ifp.as<float>(), nullptr, c->ifp2e.as<int32_t>(),
+                    o->bin_scale, (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(),
+                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0);
     else
-        rc = search(c, s, 1, cnt.n_tri, c->ifp.as<float>(), c->tris.as<int4>(), o->bin_scale,
-                    (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(), epi, o->w_phi,
-                    o->w_ifn, o->want_nearest != 0, &clamped);
-    cnt.n_clamped = clamped;
+        rc = search(c, s, 1, cnt.n_tri, c->ifp.as<float>(), c->tris.as<int4>(), c->ifp2e.as<int32_t>(),
+                    o->bin_scale, (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(),
+                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0);
+    for (const ft_counts& f : c->field_counts) cnt.n_clamped += f.n_clamped;
     if (counts) *counts = cnt;
     c->last = cnt;
     if (rc == FT_OK && c->timing) {
@@ -1636,13 +1882,15 @@ int ft_destroy(ft_handle h)
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     DevBuf* bufs[] = {&h->xyz, &h->tets, &h->edges, &h->t2e, &h->perm, &h->sx, &h->sy, &h->sz,
+                      &h->tile_field, &h->d_node_off, &h->d_tet_off, &h->d_edge_off, &h->d_slot_off,
+                      &h->fcounts, &h->fclamp, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots,
                       &h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk,
                       &h->xtet, &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp,
                       &h->ifp2e, &h->tris, &h->tri_norm, &h->if_elems, &h->p2t_off,
                       &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
                       &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
                       &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
-                      &h->bnodes, &h->bflags, &h->seeds, &h->stats, &h->scan_bsum, &h->scalars};
+                      &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars};
     for (DevBuf* b : bufs) release(*b);
     if (h->h_scalars) (void)hipHostFree(h->h_scalars);
     for (int k = 0; k < 4; k++)
@@ -1659,47 +1907,193 @@ int ft_set_nodes(ft_handle h, const float* xyz, int64_t n_nodes)
         return ft::set_error(FT_ERR_ARG, "ft_set_nodes: bad arguments");
     h->has_conn = false;
     h->have_interface = false;
-    h->T = h->E = 0;
-    return upload_nodes(h, xyz, n_nodes);
+    const int64_t no[2] = {0, n_nodes}, zo[2] = {0, 0};
+    FT_TRY(set_fields(h, 1, no, zo, zo));
+    FT_TRY(ensure(h->xyz, sizeof(float) * 3 * std::max<int64_t>(1, n_nodes)));
+    if (n_nodes > 0)
+        FT_HIP(hipMemcpyAsync(h->xyz.p, xyz, sizeof(float) * 3 * n_nodes, hipMemcpyHostToDevice, h->stream));
+    boxes_from_host(h, xyz);
+    return sort_nodes(h);
+}
+
+int ft_set_mesh_batch(ft_handle h, int32_t n_fields, const int64_t* node_off, const float* xyz,
+                      const int64_t* tet_off, const int32_t* tets, const int64_t* edge_off,
+                      const int32_t* edges, const int32_t* t2e)
+{
+    FT_TRY(check_handle(h));
+    if (n_fields < 1 || !node_off || !tet_off || !edge_off || !xyz)
+        return ft::set_error(FT_ERR_ARG, "ft_set_mesh_batch: bad arguments");
+    if (node_off[0] != 0 || tet_off[0] != 0 || edge_off[0] != 0)
+        return ft::set_error(FT_ERR_ARG, "ft_set_mesh_batch: offsets must start at 0");
+    for (int k = 0; k < n_fields; k++)
+        if (node_off[k + 1] <= node_off[k] || tet_off[k + 1] < tet_off[k] || edge_off[k + 1] < edge_off[k])
+            return ft::set_error(FT_ERR_ARG, "ft_set_mesh_batch: field %d has bad offsets", k);
+    const int64_t N = node_off[n_fields], T = tet_off[n_fields], E = edge_off[n_fields];
+    if (N > INT32_MAX || T > INT32_MAX || E > INT32_MAX)
+        return ft::set_error(FT_ERR_ARG, "ft_set_mesh_batch: sizes exceed int32 indices");
+    if ((T > 0 && (!tets || !t2e)) || (E > 0 && !edges))
+        return ft::set_error(FT_ERR_ARG, "ft_set_mesh_batch: null connectivity");
+    // Field-local indices are validated here so that no kernel reads out of bounds.
+    for (int k = 0; k < n_fields; k++) {
+        const int64_t nk = node_off[k + 1] - node_off[k], ek = edge_off[k + 1] - edge_off[k];
+        for (int64_t i = 4 * tet_off[k]; i < 4 * tet_off[k + 1]; i++)
+            if (tets[i] < 0 || tets[i] >= nk)
+                return ft::set_error(FT_ERR_ARG, "tet %lld (field %d) has node %d out of range",
+                                     (long long)(i / 4), k, tets[i]);
+        for (int64_t i = 2 * edge_off[k]; i < 2 * edge_off[k + 1]; i++)
+            if (edges[i] < 0 || edges[i] >= nk)
+                return ft::set_error(FT_ERR_ARG, "edge %lld (field %d) has node %d out of range",
+                                     (long long)(i / 2), k, edges[i]);
+        for (int64_t i = 6 * tet_off[k]; i < 6 * tet_off[k + 1]; i++)
+            if (t2e[i] < 0 || t2e[i] >= ek)
+                return ft::set_error(FT_ERR_ARG, "t2e of tet %lld (field %d) has edge %d out of range",
+                                     (long long)(i / 6), k, t2e[i]);
+    }
+    h->has_conn = false;
+    h->have_interface = false;
+    FT_TRY(set_fields(h, n_fields, node_off, tet_off, edge_off));
+    hipStream_t s = h->stream;
+    FT_TRY(ensure(h->xyz, sizeof(float) * 3 * N));
+    FT_TRY(ensure(h->tets, 16 * std::max<int64_t>(1, T)));
+    FT_TRY(ensure(h->edges, 8 * std::max<int64_t>(1, E)));
+    FT_TRY(ensure(h->t2e, 24 * std::max<int64_t>(1, T)));
+    FT_HIP(hipMemcpyAsync(h->xyz.p, xyz, sizeof(float) * 3 * N, hipMemcpyHostToDevice, s));
+    if (T > 0) {
+        FT_HIP(hipMemcpyAsync(h->tets.p, tets, 16 * T, hipMemcpyHostToDevice, s));
+        FT_HIP(hipMemcpyAsync(h->t2e.p, t2e, 24 * T, hipMemcpyHostToDevice, s));
+    }
+    if (E > 0) FT_HIP(hipMemcpyAsync(h->edges.p, edges, 8 * E, hipMemcpyHostToDevice, s));
+    // field-local -> global indices, in place
+    for (int k = 0; k < n_fields; k++) {
+        const int64_t nt = tet_off[k + 1] - tet_off[k], ne = edge_off[k + 1] - edge_off[k];
+        if (node_off[k] != 0) {
+            int32_t* tp = h->tets.as<int32_t>() + 4 * tet_off[k];
+            int32_t* ep = h->edges.as<int32_t>() + 2 * edge_off[k];
+            if (nt) hipLaunchKernelGGL(k_add_offset, dim3(grid1d(4 * nt)), dim3(kBS), 0, s, tp, 4 * nt, node_off[k], tp);
+            if (ne) hipLaunchKernelGGL(k_add_offset, dim3(grid1d(2 * ne)), dim3(kBS), 0, s, ep, 2 * ne, node_off[k], ep);
+        }
+        if (edge_off[k] != 0 && nt) {
+            int32_t* qp = h->t2e.as<int32_t>() + 6 * tet_off[k];
+            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(6 * nt)), dim3(kBS), 0, s, qp, 6 * nt, edge_off[k], qp);
+        }
+    }
+    FT_TRY(check_launch());
+    boxes_from_host(h, xyz);
+    FT_TRY(sort_nodes(h));
+    h->has_conn = true;
+    return FT_OK;
 }
 
 int ft_set_mesh(ft_handle h, const float* xyz, int64_t n_nodes, const int32_t* tets,
                 int64_t n_tets, const int32_t* edges, int64_t n_edges, const int32_t* t2e)
 {
     FT_TRY(check_handle(h));
-    if (n_nodes <= 0 || n_nodes > INT32_MAX || n_tets < 0 || n_tets > INT32_MAX || n_edges < 0 ||
-        n_edges > INT32_MAX || !xyz || (n_tets > 0 && (!tets || !t2e)) || (n_edges > 0 && !edges))
+    if (n_nodes <= 0 || n_tets < 0 || n_edges < 0)
         return ft::set_error(FT_ERR_ARG, "ft_set_mesh: bad arguments");
-    // Index ranges are validated once here so that no kernel reads out of bounds.
-    for (int64_t i = 0; i < 4 * n_tets; i++)
-        if (tets[i] < 0 || tets[i] >= n_nodes)
-            return ft::set_error(FT_ERR_ARG, "ft_set_mesh: tet %lld has node %d out of range",
-                                 (long long)(i / 4), tets[i]);
-    for (int64_t i = 0; i < 2 * n_edges; i++)
-        if (edges[i] < 0 || edges[i] >= n_nodes)
-            return ft::set_error(FT_ERR_ARG, "ft_set_mesh: edge %lld has node %d out of range",
-                                 (long long)(i / 2), edges[i]);
-    for (int64_t i = 0; i < 6 * n_tets; i++)
-        if (t2e[i] < 0 || t2e[i] >= n_edges)
-            return ft::set_error(FT_ERR_ARG, "ft_set_mesh: t2e of tet %lld has edge %d out of range",
-                                 (long long)(i / 6), t2e[i]);
+    const int64_t no[2] = {0, n_nodes}, to[2] = {0, n_tets}, eo[2] = {0, n_edges};
+    return ft_set_mesh_batch(h, 1, no, xyz, to, tets, eo, edges, t2e);
+}
+
+int ft_set_kuhn_batch(ft_handle h, int32_t n_fields, const int32_t* dims, const double* origins,
+                      const double* spacings, double jitter, uint64_t seed)
+{
+    FT_TRY(check_handle(h));
+    if (n_fields < 1 || !dims || !(jitter >= 0.0 && jitter < 0.5))
+        return ft::set_error(FT_ERR_ARG, "ft_set_kuhn_batch: bad arguments");
+    std::vector<int64_t> no(n_fields + 1, 0), to(n_fields + 1, 0), eo(n_fields + 1, 0);
+    for (int k = 0; k < n_fields; k++) {
+        int64_t nn, nt, ne;
+        FT_TRY(ft_kuhn_counts(dims[3 * k], dims[3 * k + 1], dims[3 * k + 2], &nn, &nt, &ne));
+        no[k + 1] = no[k] + nn;
+        to[k + 1] = to[k] + nt;
+        eo[k + 1] = eo[k] + ne;
+    }
+    if (no[n_fields] > INT32_MAX || to[n_fields] > INT32_MAX || eo[n_fields] > INT32_MAX)
+        return ft::set_error(FT_ERR_ARG, "ft_set_kuhn_batch: batch exceeds int32 indices");
     h->has_conn = false;
     h->have_interface = false;
-    FT_TRY(upload_nodes(h, xyz, n_nodes));
-    h->T = n_tets;
-    h->E = n_edges;
-    FT_TRY(ensure(h->tets, 16 * std::max<int64_t>(1, n_tets)));
-    FT_TRY(ensure(h->edges, 8 * std::max<int64_t>(1, n_edges)));
-    FT_TRY(ensure(h->t2e, 24 * std::max<int64_t>(1, n_tets)));
-    if (n_tets > 0) {
-        FT_HIP(hipMemcpyAsync(h->tets.p, tets, 16 * n_tets, hipMemcpyHostToDevice, h->stream));
-        FT_HIP(hipMemcpyAsync(h->t2e.p, t2e, 24 * n_tets, hipMemcpyHostToDevice, h->stream));
+    FT_TRY(set_fields(h, n_fields, no.data(), to.data(), eo.data()));
+    hipStream_t s = h->stream;
+    FT_TRY(ensure(h->xyz, sizeof(float) * 3 * no[n_fields]));
+    FT_TRY(ensure(h->tets, 16 * std::max<int64_t>(1, to[n_fields])));
+    FT_TRY(ensure(h->edges, 8 * std::max<int64_t>(1, eo[n_fields])));
+    FT_TRY(ensure(h->t2e, 24 * std::max<int64_t>(1, to[n_fields])));
+    for (int k = 0; k < n_fields; k++) {
+        KuhnField f{};
+        f.nx = dims[3 * k]; f.ny = dims[3 * k + 1]; f.nz = dims[3 * k + 2];
+        const double sp = 1.0 / (std::max(f.nx, std::max(f.ny, f.nz)) - 1);
+        for (int q = 0; q < 3; q++) {
+            f.origin[q] = origins ? origins[3 * k + q] : 0.0;
+            f.spacing[q] = spacings ? spacings[k] : sp;
+        }
+        f.jitter = jitter;
+        f.seed = seed;
+        f.node_off = no[k]; f.tet_off = to[k]; f.edge_off = eo[k];
+        const int64_t nn = no[k + 1] - no[k], nt = to[k + 1] - to[k];
+        hipLaunchKernelGGL(k_kuhn_nodes, dim3(grid1d(nn)), dim3(kBS), 0, s, f, h->xyz.as<float>());
+        hipLaunchKernelGGL(k_kuhn_edges, dim3(grid1d(nn)), dim3(kBS), 0, s, f, h->edges.as<int2>());
+        hipLaunchKernelGGL(k_kuhn_tets, dim3(grid1d(nt)), dim3(kBS), 0, s, f, h->tets.as<int4>(),
+                           h->t2e.as<int32_t>());
+        // box: the grid's extent (jitter moves interior nodes only)
+        FieldHost& fb = h->fh[k];
+        const int d3[3] = {f.nx, f.ny, f.nz};
+        for (int q = 0; q < 3; q++) {
+            fb.lo[q] = (float)f.origin[q];
+            fb.hi[q] = (float)(f.origin[q] + (d3[q] - 1) * f.spacing[q]);
+        }
+        finish_field_box(fb, nn);
     }
-    if (n_edges > 0)
-        FT_HIP(hipMemcpyAsync(h->edges.p, edges, 8 * n_edges, hipMemcpyHostToDevice, h->stream));
-    FT_HIP(hipStreamSynchronize(h->stream));
+    h->h_est = h->fh[0].h;
+    FT_TRY(check_launch());
+    FT_TRY(sort_nodes(h));
     h->has_conn = true;
     return FT_OK;
+}
+
+int ft_batch_info(ft_handle h, int32_t* n_fields, int64_t* node_off, int64_t* tet_off,
+                  int64_t* edge_off)
+{
+    FT_TRY(check_handle(h));
+    if (n_fields) *n_fields = h->nf;
+    for (int k = 0; k <= h->nf; k++) {
+        if (node_off) node_off[k] = h->node_off[k];
+        if (tet_off) tet_off[k] = h->tet_off[k];
+        if (edge_off) edge_off[k] = h->edge_off[k];
+    }
+    return FT_OK;
+}
+
+int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges, int32_t* t2e)
+{
+    FT_TRY(check_handle(h));
+    hipStream_t s = h->stream;
+    if (xyz && h->N) FT_HIP(hipMemcpyAsync(xyz, h->xyz.p, 12 * h->N, hipMemcpyDeviceToHost, s));
+    if (tets && h->T) FT_HIP(hipMemcpyAsync(tets, h->tets.p, 16 * h->T, hipMemcpyDeviceToHost, s));
+    if (edges && h->E) FT_HIP(hipMemcpyAsync(edges, h->edges.p, 8 * h->E, hipMemcpyDeviceToHost, s));
+    if (t2e && h->T) FT_HIP(hipMemcpyAsync(t2e, h->t2e.p, 24 * h->T, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
+    return FT_OK;
+}
+
+int ft_get_field_counts(ft_handle h, ft_counts* per_field)
+{
+    FT_TRY(check_handle(h));
+    if (!per_field) return ft::set_error(FT_ERR_ARG, "null output");
+    for (int k = 0; k < h->nf; k++)
+        per_field[k] = k < (int)h->field_counts.size() ? h->field_counts[k] : ft_counts{};
+    return FT_OK;
+}
+
+int ft_reinit_batch(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts* o,
+                    ft_counts* per_field, void* stream)
+{
+    FT_TRY(check_handle(h));
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    ft_counts tot{};
+    const int rc = reinit_device(h, d_phi, d_phi_ifn, o, &tot, s);
+    if (per_field)
+        for (int k = 0; k < h->nf && k < (int)h->field_counts.size(); k++) per_field[k] = h->field_counts[k];
+    return rc;
 }
 
 int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* phi_n)
@@ -1707,15 +2101,16 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     FT_TRY(check_handle(h));
     if (n_ifp < 0 || (n_ifp > 0 && !ifp) || (h->N > 0 && !phi_n))
         return ft::set_error(FT_ERR_ARG, "ft_reset_phi_points: bad arguments");
+    if (h->nf != 1) return ft::set_error(FT_ERR_STATE, "ft_reset_phi_points needs a single field");
     hipStream_t s = h->stream;
     FT_TRY(ensure(h->phi_tmp, 8 * std::max<int64_t>(1, h->N)));
     FT_TRY(ensure(h->ifp, 12 * std::max<int64_t>(1, n_ifp)));
     if (n_ifp > 0) FT_HIP(hipMemcpyAsync(h->ifp.p, ifp, 12 * n_ifp, hipMemcpyHostToDevice, s));
     if (h->N > 0) FT_HIP(hipMemcpyAsync(h->phi_tmp.p, phi_n, 8 * h->N, hipMemcpyHostToDevice, s));
-    int64_t clamped = 0;
     h->have_interface = false;
-    FT_TRY(search(h, s, 0, n_ifp, h->ifp.as<float>(), nullptr, 0.0f, 0.0f, h->phi_tmp.as<double>(),
-                  h->phi_tmp.as<double>(), nullptr, nullptr, EPI_RAW, 0.0, 0.0, true, &clamped));
+    FT_TRY(search(h, s, 0, n_ifp, h->ifp.as<float>(), nullptr, nullptr, 0.0f, 0.0f,
+                  h->phi_tmp.as<double>(), h->phi_tmp.as<double>(), nullptr, nullptr, EPI_RAW, 0.0,
+                  0.0, true));
     if (h->N > 0) FT_HIP(hipMemcpyAsync(phi_n, h->phi_tmp.p, 8 * h->N, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
     return FT_OK;

@@ -189,6 +189,69 @@ class Reinit(_Handle):
                                    C.byref(cnt), C.c_void_p(stream) if stream else None))
         return cnt
 
+    # ---- batches of independent fields ------------------------------------
+    def set_batch(self, meshes):
+        """Resident batch of independent fields from host meshes."""
+        nf = len(meshes)
+        no = np.zeros(nf + 1, np.int64)
+        to = np.zeros(nf + 1, np.int64)
+        eo = np.zeros(nf + 1, np.int64)
+        for k, m in enumerate(meshes):
+            no[k + 1] = no[k] + m.xyz.shape[0]
+            to[k + 1] = to[k] + m.tets.shape[0]
+            eo[k + 1] = eo[k] + m.edges.shape[0]
+        xyz = np.ascontiguousarray(np.concatenate([m.xyz for m in meshes]), np.float32)
+        tets = np.ascontiguousarray(np.concatenate([m.tets for m in meshes]), np.int32)
+        edges = np.ascontiguousarray(np.concatenate([m.edges for m in meshes]), np.int32)
+        t2e = np.ascontiguousarray(np.concatenate([m.t2e for m in meshes]), np.int32)
+        check(lib.ft_set_mesh_batch(self._h, nf, _ptr(no), _ptr(xyz), _ptr(to), _ptr(tets),
+                                    _ptr(eo), _ptr(edges), _ptr(t2e)))
+        self.n = int(no[-1])
+
+    def set_kuhn_batch(self, dims, jitter=0.0, seed=0, origins=None, spacings=None):
+        """Resident batch of Kuhn grids generated on the device; dims: list of
+        n or (nx, ny, nz)."""
+        d = np.array([(x, x, x) if np.isscalar(x) else tuple(x) for x in dims], np.int32)
+        o = None if origins is None else np.ascontiguousarray(origins, np.float64)
+        sp = None if spacings is None else np.ascontiguousarray(spacings, np.float64)
+        check(lib.ft_set_kuhn_batch(self._h, d.shape[0], _ptr(d), _ptr(o), _ptr(sp), float(jitter),
+                                    int(seed) & (2**64 - 1)))
+        self.n = int(self.batch_info()[0][-1])
+
+    def batch_info(self):
+        """(node_off, tet_off, edge_off) of the resident fields."""
+        nf = C.c_int32()
+        check(lib.ft_batch_info(self._h, C.byref(nf), None, None, None))
+        no = np.empty(nf.value + 1, np.int64)
+        to = np.empty(nf.value + 1, np.int64)
+        eo = np.empty(nf.value + 1, np.int64)
+        check(lib.ft_batch_info(self._h, C.byref(nf), _ptr(no), _ptr(to), _ptr(eo)))
+        return no, to, eo
+
+    def mesh(self):
+        """The resident mesh (global indices)."""
+        no, to, eo = self.batch_info()
+        xyz = np.empty((no[-1], 3), np.float32)
+        tets = np.empty((to[-1], 4), np.int32)
+        edges = np.empty((eo[-1], 2), np.int32)
+        t2e = np.empty((to[-1], 6), np.int32)
+        check(lib.ft_get_mesh(self._h, _ptr(xyz), _ptr(tets), _ptr(edges), _ptr(t2e)))
+        return Mesh(xyz, tets, edges, t2e)
+
+    def field_counts(self):
+        nf = len(self.batch_info()[0]) - 1
+        arr = (_lib.FtCounts * nf)()
+        check(lib.ft_get_field_counts(self._h, arr))
+        return list(arr)
+
+    def reset_phi_batch_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
+                               bin_scale=0.0, want_nearest=False, stream=None):
+        o = make_opts(mode, blend, band, bin_scale, want_nearest)
+        check(lib.ft_reinit_batch(self._h, C.c_void_p(phi_ptr),
+                                  C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
+                                  C.c_void_p(stream) if stream else None))
+        return self.field_counts()
+
     def set_timing(self, on=True):
         check(lib.ft_set_timing(self._h, 1 if on else 0))
 

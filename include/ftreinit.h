@@ -105,6 +105,45 @@ int ft_set_mesh(ft_handle h, const float* xyz, int64_t n_nodes,
                 const int32_t* tets, int64_t n_tets, const int32_t* edges,
                 int64_t n_edges, const int32_t* t2e);
 
+/* Batch of independent fields resident together (no counterpart in the
+ * reference, which holds one mesh per LevelSet).  Field k owns nodes
+ * [node_off[k], node_off[k+1]), tets [tet_off[k], ...) and edges
+ * [edge_off[k], ...) of the concatenated arrays; its tets, edges and t2e use
+ * field-local indices.  Offsets have n_fields+1 entries starting at 0.  A
+ * reinit of the batch treats every field exactly as a reinit of that field
+ * alone would (same results, bit for bit), in one pass of each kernel. */
+int ft_set_mesh_batch(ft_handle h, int32_t n_fields, const int64_t* node_off,
+                      const float* xyz, const int64_t* tet_off,
+                      const int32_t* tets, const int64_t* edge_off,
+                      const int32_t* edges, const int32_t* t2e);
+
+/* Batch of Kuhn-grid fields generated on the device, identical to
+ * ft_kuhn_mesh: dims 3 per field; origins 3 per field (NULL: 0); spacings one
+ * per field (NULL: 1/(max dim - 1)); interior nodes jittered as in
+ * ft_kuhn_mesh with the same seed for every field. */
+int ft_set_kuhn_batch(ft_handle h, int32_t n_fields, const int32_t* dims,
+                      const double* origins, const double* spacings,
+                      double jitter, uint64_t seed);
+
+/* Layout of the resident fields: n_fields and the global offsets (arrays of
+ * n_fields+1; any pointer may be NULL). */
+int ft_batch_info(ft_handle h, int32_t* n_fields, int64_t* node_off,
+                  int64_t* tet_off, int64_t* edge_off);
+
+/* Copies the resident mesh (global indices) to host memory; NULL skips. */
+int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges,
+                int32_t* t2e);
+
+/* Reinit of every resident field in one pass on device-resident phi (all
+ * fields concatenated, node_off layout).  per_field (nullable, n_fields
+ * entries) receives each field's counts; interface outputs are the
+ * concatenation of the fields' interfaces in global numbering. */
+int ft_reinit_batch(ft_handle h, double* d_phi, double* d_phi_ifn,
+                    const ft_opts* o, ft_counts* per_field, void* stream);
+
+/* Per-field counts of the last reinit (n_fields entries). */
+int ft_get_field_counts(ft_handle h, ft_counts* per_field);
+
 /* EuclideanDistance::resetPhi (EuclideanDistance.cu:90-118), host pointers:
  * phi_n[i] = sign(phi_n[i] < 0) * min_j |ifp_j - p_i| in float, bit-equal to
  * the reference's CPU branch.  n_ifp == 0 gives +-FLT_MAX. */
