@@ -39,7 +39,8 @@ class FtOpts(C.Structure):
         ("band", C.c_double),
         ("bin_scale", C.c_float),
         ("want_nearest", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("query_block", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -95,6 +96,14 @@ _SIGS = {
     "ft_batch_info": (C.c_int, [_H, C.POINTER(C.c_int32), _P, _P, _P]),
     "ft_get_mesh": (C.c_int, [_H, _P, _P, _P, _P]),
     "ft_get_field_counts": (C.c_int, [_H, _P]),
+    "ft_mesh_create_kuhn": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                      C.c_uint64, C.POINTER(C.c_int32)]),
+    "ft_mesh_create": (C.c_int, [C.c_int32, _P, C.c_int64, _P, C.c_int64, _P, C.c_int64, _P,
+                                 C.POINTER(C.c_int32)]),
+    "ft_mesh_info": (C.c_int, [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ft_mesh_release": (C.c_int, [C.c_int32]),
+    "ft_set_batch_meshes": (C.c_int, [_H, C.c_int32, _P]),
     "ft_reinit_batch": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
     "ft_set_timing": (C.c_int, [_H, C.c_int32]),
     "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

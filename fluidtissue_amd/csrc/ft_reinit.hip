@@ -973,12 +973,12 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
 // keeps its own best.  All control flow is wave-uniform (ballots), node and
 // primitive records are read at wave-uniform addresses through scalar loads,
 // the stack is one per wave in LDS.
-template <int MODE>
-__global__ __launch_bounds__(kQueryBS) void k_query_wave(QueryArgs a)
+template <int MODE, int BS>
+__global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
 {
-    __shared__ int32_t wstack[kQueryBS / 64][kStack];
+    __shared__ int32_t wstack[BS / 64][kStack];
     const int w = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * kQueryBS + (threadIdx.x & ~63);
+    const int64_t base = (int64_t)blockIdx.x * BS + (threadIdx.x & ~63);
     if (base >= a.n_slots) return;                    // whole wave out of range
     const int64_t slot = base + (threadIdx.x & 63);
     const int32_t node = a.perm[slot];                 // -1: padding lane
@@ -1209,7 +1209,14 @@ __global__ __launch_bounds__(kBS) void k_kuhn_tets(KuhnField f, int4* tets, int3
 }
 
 // Localised copies of index arrays (local -> global): mesh uploads and batch
-// assembly from resident meshes.
+// assembly from resident meshes.  Negative entries (slot padding) stay.
+__global__ __launch_bounds__(kBS) void k_add_offset_pos(const int32_t* src, int64_t n, int64_t add,
+                                                        int32_t* dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i < n) dst[i] = src[i] < 0 ? src[i] : (int32_t)(src[i] + add);
+}
+
 __global__ __launch_bounds__(kBS) void k_add_offset(const int32_t* src, int64_t n, int64_t add,
                                                     int32_t* dst)
 {
@@ -1259,6 +1266,7 @@ struct ft_ctx {
     uint64_t* h_scalars = nullptr;   // pinned mirror
     ft_counts last{};
     std::vector<ft_counts> field_counts;
+    std::vector<int32_t> batch_meshes;   // resident meshes of the current batch
     bool have_interface = false;
     bool have_nearest = false;
     // phase timing (ft_set_timing): events at the phase boundaries of a reinit
@@ -1383,7 +1391,7 @@ int check_handle(ft_handle h)
 int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, const int4* tris,
            const int32_t* ifp2e, float bin_scale, float band, const double* phi_old,
            double* phi_out, double* phi_ifn, const uint64_t* retain, int epi, double w_phi,
-           double w_ifn, bool want_nearest)
+           double w_ifn, bool want_nearest, int query_block = 0)
 {
     uint64_t* sc = c->scalars.as<uint64_t>();
     const int nf = c->nf;
@@ -1504,10 +1512,17 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
     q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     mark(c, 2, s);
     if (c->n_slots > 0) {
-        if (mode == 0)
-            hipLaunchKernelGGL(k_query_wave<0>, dim3(grid1d(c->n_slots, kQueryBS)), dim3(kQueryBS), 0, s, q);
-        else
-            hipLaunchKernelGGL(k_query_wave<1>, dim3(grid1d(c->n_slots, kQueryBS)), dim3(kQueryBS), 0, s, q);
+        const int bs = query_block == 64 || query_block == 128 ? query_block : 256;
+        const dim3 g(grid1d(c->n_slots, bs)), b(bs);
+        if (mode == 0) {
+            if (bs == 64) hipLaunchKernelGGL((k_query_wave<0, 64>), g, b, 0, s, q);
+            else if (bs == 128) hipLaunchKernelGGL((k_query_wave<0, 128>), g, b, 0, s, q);
+            else hipLaunchKernelGGL((k_query_wave<0, 256>), g, b, 0, s, q);
+        } else {
+            if (bs == 64) hipLaunchKernelGGL((k_query_wave<1, 64>), g, b, 0, s, q);
+            else if (bs == 128) hipLaunchKernelGGL((k_query_wave<1, 128>), g, b, 0, s, q);
+            else hipLaunchKernelGGL((k_query_wave<1, 256>), g, b, 0, s, q);
+        }
         FT_TRY(check_launch());
     }
     mark(c, 3, s);
@@ -1546,6 +1561,7 @@ int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, cons
 int set_fields(ft_ctx* c, int nf, const int64_t* node_off, const int64_t* tet_off,
                const int64_t* edge_off)
 {
+    c->batch_meshes.clear();
     c->nf = nf;
     c->node_off.assign(node_off, node_off + nf + 1);
     c->tet_off.assign(tet_off, tet_off + nf + 1);
@@ -1778,6 +1794,8 @@ int validate_opts(const ft_opts* o)
         return ft::set_error(FT_ERR_ARG, "band must be >= 0");
     if (!(o->bin_scale >= 0.0f) || o->bin_scale > 1e6f)
         return ft::set_error(FT_ERR_ARG, "bin_scale must be >= 0");
+    if (o->query_block != 0 && o->query_block != 64 && o->query_block != 128 && o->query_block != 256)
+        return ft::set_error(FT_ERR_ARG, "query_block must be 0, 64, 128 or 256");
     return FT_OK;
 }
 
@@ -1798,11 +1816,11 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     if (o->mode == FT_DIST_POINTS)
         rc = search(c, s, 0, cnt.n_ifp, c->ifp.as<float>(), nullptr, c->ifp2e.as<int32_t>(),
                     o->bin_scale, (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(),
-                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0);
+                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0, o->query_block);
     else
         rc = search(c, s, 1, cnt.n_tri, c->ifp.as<float>(), c->tris.as<int4>(), c->ifp2e.as<int32_t>(),
                     o->bin_scale, (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(),
-                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0);
+                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0, o->query_block);
     for (const ft_counts& f : c->field_counts) cnt.n_clamped += f.n_clamped;
     if (counts) *counts = cnt;
     c->last = cnt;
@@ -2072,6 +2090,157 @@ int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges, int32_t*
     if (edges && h->E) FT_HIP(hipMemcpyAsync(edges, h->edges.p, 8 * h->E, hipMemcpyDeviceToHost, s));
     if (t2e && h->T) FT_HIP(hipMemcpyAsync(t2e, h->t2e.p, 24 * h->T, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
+    return FT_OK;
+}
+
+// ---- resident meshes --------------------------------------------------------
+// A resident mesh is a one-field handle kept only for its mesh and its query
+// slots; batches are assembled from resident meshes by device copies.
+namespace {
+struct Resident {
+    ft_ctx* c = nullptr;
+    int64_t bytes = 0;
+};
+std::mutex g_res_mu;
+std::vector<Resident> g_res;
+
+int resident_finish(ft_ctx* c, int32_t* id)
+{
+    // drop the set-up workspaces, keep mesh + slots
+    DevBuf* tmp[] = {&c->bin_count, &c->bin_prefix, &c->pcode, &c->scan_bsum};
+    for (DevBuf* b : tmp) release(*b);
+    Resident r;
+    r.c = c;
+    r.bytes = (int64_t)(c->xyz.cap + c->tets.cap + c->edges.cap + c->t2e.cap + c->perm.cap + c->sx.cap +
+                        c->sy.cap + c->sz.cap + c->tile_field.cap);
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    for (size_t i = 0; i < g_res.size(); i++)
+        if (!g_res[i].c) { g_res[i] = r; *id = (int32_t)i; return FT_OK; }
+    g_res.push_back(r);
+    *id = (int32_t)g_res.size() - 1;
+    return FT_OK;
+}
+
+ft_ctx* resident_get(int32_t id)
+{
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    if (id < 0 || id >= (int32_t)g_res.size()) return nullptr;
+    return g_res[id].c;
+}
+}  // namespace
+
+int ft_mesh_create_kuhn(int32_t device, int32_t nx, int32_t ny, int32_t nz, double jitter,
+                        uint64_t seed, int32_t* id)
+{
+    if (!id) return ft::set_error(FT_ERR_ARG, "null output");
+    ft_handle h = nullptr;
+    FT_TRY(ft_create(device, &h));
+    const int32_t dims[3] = {nx, ny, nz};
+    int rc = ft_set_kuhn_batch(h, 1, dims, nullptr, nullptr, jitter, seed);
+    if (rc) { ft_destroy(h); return rc; }
+    return resident_finish(h, id);
+}
+
+int ft_mesh_create(int32_t device, const float* xyz, int64_t n_nodes, const int32_t* tets,
+                   int64_t n_tets, const int32_t* edges, int64_t n_edges, const int32_t* t2e,
+                   int32_t* id)
+{
+    if (!id) return ft::set_error(FT_ERR_ARG, "null output");
+    ft_handle h = nullptr;
+    FT_TRY(ft_create(device, &h));
+    int rc = ft_set_mesh(h, xyz, n_nodes, tets, n_tets, edges, n_edges, t2e);
+    if (rc) { ft_destroy(h); return rc; }
+    return resident_finish(h, id);
+}
+
+int ft_mesh_info(int32_t id, int64_t* n_nodes, int64_t* n_tets, int64_t* n_edges, int64_t* bytes)
+{
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    if (id < 0 || id >= (int32_t)g_res.size() || !g_res[id].c)
+        return ft::set_error(FT_ERR_ARG, "no resident mesh %d", id);
+    const ft_ctx* c = g_res[id].c;
+    if (n_nodes) *n_nodes = c->N;
+    if (n_tets) *n_tets = c->T;
+    if (n_edges) *n_edges = c->E;
+    if (bytes) *bytes = g_res[id].bytes;
+    return FT_OK;
+}
+
+int ft_mesh_release(int32_t id)
+{
+    ft_ctx* c = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_res_mu);
+        if (id < 0 || id >= (int32_t)g_res.size() || !g_res[id].c)
+            return ft::set_error(FT_ERR_ARG, "no resident mesh %d", id);
+        c = g_res[id].c;
+        g_res[id] = Resident{};
+    }
+    return ft_destroy(c);
+}
+
+int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
+{
+    FT_TRY(check_handle(h));
+    if (n_fields < 1 || !mesh_ids) return ft::set_error(FT_ERR_ARG, "ft_set_batch_meshes: bad arguments");
+    std::vector<ft_ctx*> ms(n_fields);
+    for (int k = 0; k < n_fields; k++) {
+        ms[k] = resident_get(mesh_ids[k]);
+        if (!ms[k]) return ft::set_error(FT_ERR_ARG, "no resident mesh %d", mesh_ids[k]);
+        if (ms[k]->device != h->device)
+            return ft::set_error(FT_ERR_ARG, "resident mesh %d lives on another device", mesh_ids[k]);
+    }
+    if (h->has_conn && h->batch_meshes.size() == (size_t)n_fields &&
+        std::equal(h->batch_meshes.begin(), h->batch_meshes.end(), mesh_ids))
+        return FT_OK;                                  // already assembled
+    std::vector<int64_t> no(n_fields + 1, 0), to(n_fields + 1, 0), eo(n_fields + 1, 0);
+    for (int k = 0; k < n_fields; k++) {
+        no[k + 1] = no[k] + ms[k]->N;
+        to[k + 1] = to[k] + ms[k]->T;
+        eo[k + 1] = eo[k] + ms[k]->E;
+    }
+    if (no[n_fields] > INT32_MAX || to[n_fields] > INT32_MAX || eo[n_fields] > INT32_MAX)
+        return ft::set_error(FT_ERR_ARG, "batch exceeds int32 indices");
+    h->has_conn = false;
+    h->have_interface = false;
+    h->batch_meshes.clear();
+    FT_TRY(set_fields(h, n_fields, no.data(), to.data(), eo.data()));
+    hipStream_t s = h->stream;
+    FT_TRY(ensure(h->xyz, 12 * no[n_fields]));
+    FT_TRY(ensure(h->tets, 16 * std::max<int64_t>(1, to[n_fields])));
+    FT_TRY(ensure(h->edges, 8 * std::max<int64_t>(1, eo[n_fields])));
+    FT_TRY(ensure(h->t2e, 24 * std::max<int64_t>(1, to[n_fields])));
+    FT_TRY(ensure(h->perm, 4 * std::max<int64_t>(1, h->n_slots)));
+    FT_TRY(ensure(h->sx, 4 * std::max<int64_t>(1, h->n_slots)));
+    FT_TRY(ensure(h->sy, 4 * std::max<int64_t>(1, h->n_slots)));
+    FT_TRY(ensure(h->sz, 4 * std::max<int64_t>(1, h->n_slots)));
+    FT_TRY(ensure(h->tile_field, 4 * std::max<int64_t>(1, h->n_slots / 64)));
+    for (int k = 0; k < n_fields; k++) {
+        const ft_ctx* m = ms[k];
+        const int64_t so = h->slot_off[k];
+        FT_HIP(hipMemcpyAsync(h->xyz.as<float>() + 3 * no[k], m->xyz.p, 12 * m->N, hipMemcpyDeviceToDevice, s));
+        if (m->T) {
+            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(4 * m->T)), dim3(kBS), 0, s, m->tets.as<int32_t>(),
+                               4 * m->T, no[k], h->tets.as<int32_t>() + 4 * to[k]);
+            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(6 * m->T)), dim3(kBS), 0, s, m->t2e.as<int32_t>(),
+                               6 * m->T, eo[k], h->t2e.as<int32_t>() + 6 * to[k]);
+        }
+        if (m->E)
+            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(2 * m->E)), dim3(kBS), 0, s, m->edges.as<int32_t>(),
+                               2 * m->E, no[k], h->edges.as<int32_t>() + 2 * eo[k]);
+        hipLaunchKernelGGL(k_add_offset_pos, dim3(grid1d(m->n_slots)), dim3(kBS), 0, s,
+                           m->perm.as<int32_t>(), m->n_slots, no[k], h->perm.as<int32_t>() + so);
+        FT_HIP(hipMemcpyAsync(h->sx.as<float>() + so, m->sx.p, 4 * m->n_slots, hipMemcpyDeviceToDevice, s));
+        FT_HIP(hipMemcpyAsync(h->sy.as<float>() + so, m->sy.p, 4 * m->n_slots, hipMemcpyDeviceToDevice, s));
+        FT_HIP(hipMemcpyAsync(h->sz.as<float>() + so, m->sz.p, 4 * m->n_slots, hipMemcpyDeviceToDevice, s));
+        FT_TRY(fill<int32_t>(s, h->tile_field.as<int32_t>() + so / 64, m->n_slots / 64, (int32_t)k));
+        h->fh[k] = m->fh[0];
+    }
+    h->h_est = h->fh[0].h;
+    FT_TRY(check_launch());
+    FT_HIP(hipStreamSynchronize(s));
+    h->batch_meshes.assign(mesh_ids, mesh_ids + n_fields);
+    h->has_conn = true;
     return FT_OK;
 }
 

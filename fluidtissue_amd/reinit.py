@@ -95,7 +95,7 @@ def phi_field(kind, xyz, seed=0):
 
 
 def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest=False,
-              w_phi=0.75, w_ifn=0.25):
+              w_phi=0.75, w_ifn=0.25, query_block=0):
     o = _lib.FtOpts()
     lib.ft_opts_default(C.byref(o))
     o.mode = MODES[mode]
@@ -105,6 +105,7 @@ def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest
     o.want_nearest = 1 if want_nearest else 0
     o.w_phi = float(w_phi)
     o.w_ifn = float(w_ifn)
+    o.query_block = int(query_block)
     return o
 
 
@@ -168,20 +169,20 @@ class Reinit(_Handle):
         self.n = xyz.shape[0]
 
     def reset_phi(self, phi, mode="points", blend="init", band=0.0, bin_scale=0.0,
-                  want_ifn=True, want_nearest=False):
+                  want_ifn=True, want_nearest=False, query_block=0):
         """Returns (phi_out, phi_ifn, counts); phi is not modified."""
         out = np.array(phi, dtype=np.float64, copy=True)
         ifn = np.empty_like(out) if want_ifn else None
-        o = make_opts(mode, blend, band, bin_scale, want_nearest)
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit(self._h, _ptr(out), _ptr(ifn), C.byref(o), C.byref(cnt)))
         return out, ifn, cnt
 
     def reset_phi_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
-                         bin_scale=0.0, want_nearest=False, stream=None):
+                         bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
         """Device-resident phi (raw device addresses, e.g. tensor.data_ptr())."""
-        o = make_opts(mode, blend, band, bin_scale, want_nearest)
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit_device(self._h, C.c_void_p(phi_ptr),
@@ -238,6 +239,11 @@ class Reinit(_Handle):
         check(lib.ft_get_mesh(self._h, _ptr(xyz), _ptr(tets), _ptr(edges), _ptr(t2e)))
         return Mesh(xyz, tets, edges, t2e)
 
+    def set_batch_meshes(self, mesh_ids):
+        ids = np.ascontiguousarray(mesh_ids, np.int32)
+        check(lib.ft_set_batch_meshes(self._h, ids.shape[0], _ptr(ids)))
+        self.n = int(self.batch_info()[0][-1])
+
     def field_counts(self):
         nf = len(self.batch_info()[0]) - 1
         arr = (_lib.FtCounts * nf)()
@@ -245,8 +251,8 @@ class Reinit(_Handle):
         return list(arr)
 
     def reset_phi_batch_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
-                               bin_scale=0.0, want_nearest=False, stream=None):
-        o = make_opts(mode, blend, band, bin_scale, want_nearest)
+                               bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
         check(lib.ft_reinit_batch(self._h, C.c_void_p(phi_ptr),
                                   C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
                                   C.c_void_p(stream) if stream else None))

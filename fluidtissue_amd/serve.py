@@ -1,0 +1,232 @@
+"""Served reinit: continuous batching of reinit jobs over resident meshes.
+
+Concurrent simulations submit reinit jobs (a level set on a grid of some
+resolution).  Meshes stay resident in HBM, keyed by grid dimensions, and
+are evicted least-recently-used only to stay within a byte budget.  Each
+worker lane owns a handle and a HIP stream; whenever a lane is free it admits
+the queued jobs that fit (by count, nodes and memory budget) as one batch and
+reinitialises all of them in one pass (ft_reinit_batch).  Jobs that arrive
+while batches are on the GPU wait for the next free lane, so two lanes keep
+two batches in flight and their work overlaps on the device.
+
+A batch gives every job exactly the result the job would get alone (the
+batched kernels keep each field's data apart); tests check it bit for bit.
+"""
+import collections
+import ctypes as C
+import threading
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .reinit import Reinit
+
+# Device bytes per node of one reinit beyond the resident mesh: phi, phi_ifn,
+# bitmaps, interface and index workspaces (upper estimate).
+WORK_BYTES_PER_NODE = 96
+
+
+class MeshCache:
+    """Resident Kuhn meshes keyed by dims, within a byte budget (LRU)."""
+
+    def __init__(self, device=0, budget_bytes=200e9):
+        self.device = device
+        self.budget = float(budget_bytes)
+        self.meshes = collections.OrderedDict()   # dims -> [id, bytes, pins]
+        self.bytes = 0
+        self.lock = threading.Lock()
+
+    @staticmethod
+    def estimate_bytes(dims):
+        nx, ny, nz = dims
+        n = nx * ny * nz
+        t = 6 * (nx - 1) * (ny - 1) * (nz - 1)
+        return 12 * n + 40 * t + 8 * 7 * n + 24 * n
+
+    def _evict_for(self, need):
+        for key in list(self.meshes):
+            if self.bytes + need <= self.budget:
+                return True
+            mid, b, pins = self.meshes[key]
+            if pins == 0:
+                check(lib.ft_mesh_release(mid))
+                self.bytes -= b
+                del self.meshes[key]
+        return self.bytes + need <= self.budget
+
+    def can_admit(self, dims, extra):
+        with self.lock:
+            need = extra + (0 if tuple(dims) in self.meshes else self.estimate_bytes(dims))
+            if self.bytes + need <= self.budget:
+                return True
+            free = sum(b for _, b, p in self.meshes.values() if p == 0)
+            return self.bytes - free + need <= self.budget
+
+    def acquire(self, dims):
+        key = tuple(dims)
+        with self.lock:
+            if key in self.meshes:
+                self.meshes.move_to_end(key)
+                self.meshes[key][2] += 1
+                return self.meshes[key][0]
+            if not self._evict_for(self.estimate_bytes(key)):
+                raise _lib.FtError(_lib.FT_ERR_BUDGET, f"mesh {key} does not fit the budget")
+            mid = C.c_int32()
+            check(lib.ft_mesh_create_kuhn(self.device, key[0], key[1], key[2], 0.0, 0, C.byref(mid)))
+            b = C.c_int64()
+            check(lib.ft_mesh_info(mid.value, None, None, None, C.byref(b)))
+            self.meshes[key] = [mid.value, b.value, 1]
+            self.bytes += b.value
+            return mid.value
+
+    def release(self, dims):
+        with self.lock:
+            self.meshes[tuple(dims)][2] -= 1
+
+    def close(self):
+        with self.lock:
+            for mid, _, _ in self.meshes.values():
+                lib.ft_mesh_release(mid)
+            self.meshes.clear()
+            self.bytes = 0
+
+
+class Job:
+    __slots__ = ("dims", "phi", "t_arrival", "t_start", "t_done", "out", "ifn", "counts",
+                 "error", "done", "n")
+
+    def __init__(self, dims, phi, t_arrival=None):
+        self.dims = tuple(int(d) for d in dims)
+        self.n = self.dims[0] * self.dims[1] * self.dims[2]
+        self.phi = np.ascontiguousarray(phi, np.float64)
+        if self.phi.shape != (self.n,):
+            raise ValueError("phi does not match dims")
+        self.t_arrival = time.perf_counter() if t_arrival is None else t_arrival
+        self.t_start = self.t_done = None
+        self.out = self.ifn = self.counts = self.error = None
+        self.done = threading.Event()
+
+    @property
+    def latency(self):
+        return self.t_done - self.t_arrival
+
+    def wait(self, timeout=None):
+        if not self.done.wait(timeout):
+            raise TimeoutError("reinit job not done")
+        if self.error:
+            raise self.error
+        return self.out
+
+
+class Server:
+    """Continuous-batching reinit server (see module docstring)."""
+
+    def __init__(self, device=0, lanes=2, budget_bytes=200e9, max_fields=64, max_nodes=64_000_000,
+                 mode="triangles", blend="init", band=0.0, bin_scale=0.0, query_block=0):
+        self.cache = MeshCache(device, budget_bytes)
+        self.budget = float(budget_bytes)
+        self.max_fields = int(max_fields)
+        self.max_nodes = int(max_nodes)
+        self.kw = dict(mode=mode, blend=blend, band=band, bin_scale=bin_scale,
+                       query_block=query_block)
+        self.queue = collections.deque()
+        self.cv = threading.Condition(threading.RLock())
+        self.stop = False
+        self.inflight_bytes = 0
+        self.batches = []              # (n_jobs, n_nodes, seconds) per batch
+        self.handles = [Reinit(device) for _ in range(lanes)]
+        self.threads = [threading.Thread(target=self._lane, args=(h,), daemon=True)
+                        for h in self.handles]
+        for t in self.threads:
+            t.start()
+
+    def submit(self, dims, phi, t_arrival=None):
+        job = Job(dims, phi, t_arrival)
+        with self.cv:                  # re-entrant: a caller may hold cv to submit a burst
+            self.queue.append(job)
+            self.cv.notify()
+        return job
+
+    def _admit(self):
+        """Takes the queued jobs that fit one batch (FIFO, no overtaking)."""
+        jobs, nodes, extra = [], 0, 0
+        while self.queue and len(jobs) < self.max_fields:
+            j = self.queue[0]
+            work = WORK_BYTES_PER_NODE * j.n
+            if jobs and nodes + j.n > self.max_nodes:
+                break
+            if not self.cache.can_admit(j.dims, self.inflight_bytes + extra + work):
+                break
+            self.queue.popleft()
+            jobs.append(j)
+            nodes += j.n
+            extra += work
+        self.inflight_bytes += extra
+        return jobs, extra
+
+    def _lane(self, h):
+        while True:
+            with self.cv:
+                while not self.queue and not self.stop:
+                    self.cv.wait()
+                if self.stop and not self.queue:
+                    return
+                jobs, extra = self._admit()
+                if not jobs:              # memory is held by the other lane
+                    self.cv.wait(0.001)
+                    continue
+            t0 = time.perf_counter()
+            ids = []
+            try:
+                for j in jobs:
+                    j.t_start = t0
+                    ids.append(self.cache.acquire(j.dims))
+                h.set_batch_meshes(ids)
+                phi = np.concatenate([j.phi for j in jobs])
+                out, ifn, _ = h.reset_phi(phi, **self.kw)
+                counts = h.field_counts()
+                off = 0
+                t1 = time.perf_counter()
+                for j, c in zip(jobs, counts):
+                    j.out = out[off:off + j.n]
+                    j.ifn = ifn[off:off + j.n]
+                    j.counts = c
+                    j.t_done = t1
+                    off += j.n
+                self.batches.append((len(jobs), off, t1 - t0))
+            except Exception as e:        # reported on every job of the batch
+                for j in jobs:
+                    j.error = e
+                    j.t_done = time.perf_counter()
+            finally:
+                for j in jobs[:len(ids)]:
+                    self.cache.release(j.dims)
+                with self.cv:
+                    self.inflight_bytes -= extra
+                    self.cv.notify_all()
+                for j in jobs:
+                    j.done.set()
+
+    def close(self):
+        with self.cv:
+            self.stop = True
+            self.cv.notify_all()
+        for t in self.threads:
+            t.join()
+        for h in self.handles:
+            h.close()
+        self.cache.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def latency_stats(jobs):
+    lat = np.array([j.latency for j in jobs])
+    return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                mean_ms=float(lat.mean() * 1e3))

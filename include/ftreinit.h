@@ -71,7 +71,9 @@ typedef struct ft_opts {
     float   bin_scale;   /* bin edge in units of the mean node spacing; 0 =
                             default                                             */
     int32_t want_nearest;/* 1: keep the per-node nearest primitive index       */
-    int32_t reserved[6];
+    int32_t query_block; /* threads per block of the distance query: 64, 128,
+                            256 (waves per workgroup 1, 2, 4); 0 = default     */
+    int32_t reserved[5];
 } ft_opts;
 
 typedef struct ft_counts {
@@ -140,6 +142,24 @@ int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges,
  * concatenation of the fields' interfaces in global numbering. */
 int ft_reinit_batch(ft_handle h, double* d_phi, double* d_phi_ifn,
                     const ft_opts* o, ft_counts* per_field, void* stream);
+
+/* Resident meshes (no counterpart in the reference): a mesh uploaded or
+ * generated once, node-sorted once, kept in HBM and shared by any number of
+ * batches on its device.  Ids are process-wide.  The Kuhn form is generated
+ * on the device exactly as ft_kuhn_mesh. */
+int ft_mesh_create_kuhn(int32_t device, int32_t nx, int32_t ny, int32_t nz,
+                        double jitter, uint64_t seed, int32_t* mesh_id);
+int ft_mesh_create(int32_t device, const float* xyz, int64_t n_nodes,
+                   const int32_t* tets, int64_t n_tets, const int32_t* edges,
+                   int64_t n_edges, const int32_t* t2e, int32_t* mesh_id);
+int ft_mesh_info(int32_t mesh_id, int64_t* n_nodes, int64_t* n_tets,
+                 int64_t* n_edges, int64_t* bytes);
+int ft_mesh_release(int32_t mesh_id);
+
+/* Makes the handle's batch the fields of the given resident meshes, in order
+ * (a mesh may appear more than once); device copies only, skipped when the
+ * list equals the current one. */
+int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids);
 
 /* Per-field counts of the last reinit (n_fields entries). */
 int ft_get_field_counts(ft_handle h, ft_counts* per_field);

@@ -90,3 +90,56 @@ def test_batch_many_fields_against_oracle():
         assert cnts[k].n_ifp == len(ref["ifp"]) and cnts[k].n_tri == len(ref["tris"])
         assert np.array_equal(ifn[no[k]:no[k + 1]].view(np.uint64), ref["phi_ifn"].view(np.uint64))
     b.close()
+
+
+def test_served_results_equal_alone():
+    """Jobs served concurrently in mixed batches get exactly their solo result."""
+    from fluidtissue_amd.serve import Server
+    rng = np.random.default_rng(9)
+    dims = [int(n) for n in rng.integers(12, 30, size=14)]
+    kinds = ["star", "spheres", "ellipsoids"]
+    inputs = []
+    for k, n in enumerate(dims):
+        m = fa.kuhn_mesh(n)
+        inputs.append((n, m, fa.phi_field(kinds[k % 3], m.xyz, seed=k)))
+    jobs = []
+    with Server(device=0, lanes=2, max_fields=5, mode="triangles", blend="update") as srv:
+        with srv.cv:                                       # one burst: queue all, then release
+            for n, m, phi in inputs:
+                jobs.append((m, phi, srv.submit((n, n, n), phi)))
+        for _, _, j in jobs:
+            j.wait(120)
+        assert max(b[0] for b in srv.batches) > 1        # some batches mixed jobs
+    for m, phi, j in jobs:
+        a = fa.Reinit(0)
+        a.set_mesh(m)
+        o1, i1, c1 = a.reset_phi(phi, mode="triangles", blend="update")
+        assert np.array_equal(j.out.view(np.uint64), o1.view(np.uint64))
+        assert np.array_equal(j.ifn.view(np.uint64), i1.view(np.uint64))
+        assert (j.counts.n_ifp, j.counts.n_tri) == (c1.n_ifp, c1.n_tri)
+        a.close()
+
+
+def test_resident_mesh_batch_matches_set_mesh():
+    ids = []
+    import ctypes as C
+    from fluidtissue_amd._lib import lib
+    for n in (10, 14, 10):
+        mid = C.c_int32()
+        assert lib.ft_mesh_create_kuhn(0, n, n, n, 0.0, 0, C.byref(mid)) == 0
+        ids.append(mid.value)
+    b = fa.Reinit(0)
+    b.set_batch_meshes(ids)
+    ref = fa.Reinit(0)
+    ref.set_batch([fa.kuhn_mesh(n) for n in (10, 14, 10)])
+    mb, mr = b.mesh(), ref.mesh()
+    for x, y in zip((mb.xyz, mb.tets, mb.edges, mb.t2e), (mr.xyz, mr.tets, mr.edges, mr.t2e)):
+        assert np.array_equal(x, y)
+    phi = fa.phi_field("star", mb.xyz)
+    o1, i1, _ = b.reset_phi(phi, mode="triangles")
+    o2, i2, _ = ref.reset_phi(phi, mode="triangles")
+    assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64))
+    b.close()
+    ref.close()
+    for mid in ids:
+        assert lib.ft_mesh_release(mid) == 0
