@@ -58,15 +58,217 @@ def cpu_baseline(m, phi, nthreads, sample_div=1024, seed=0):
                 seconds=total)
 
 
+def grid_xyz(n):
+    """Node coordinates of an n^3 grid on the unit cube, as the Kuhn generators
+    make them (x-major; float of i*h computed in double)."""
+    h = 1.0 / (n - 1)
+    g = (np.arange(n, dtype=np.float64) * h).astype(np.float32)
+    xyz = np.empty((n, n, n, 3), np.float32)
+    xyz[..., 0] = g[:, None, None]
+    xyz[..., 1] = g[None, :, None]
+    xyz[..., 2] = g[None, None, :]
+    return xyz.reshape(-1, 3)
+
+
+def run_batched(args):
+    """512 fields, N ~ U{24..96}, each a union of 1-16 ellipsoids: one batched
+    reinit (autotuned bin size and workgroup size) against 512 serial plain
+    reinits of the same fields on resident meshes."""
+    import ctypes as C
+    import torch
+    import fluidtissue_amd as fa
+    from fluidtissue_amd import autotune
+    from fluidtissue_amd._lib import check, lib
+
+    rng = np.random.default_rng(7)
+    nfields = args.fields
+    sizes = rng.integers(24, 97, size=nfields)
+    t0 = time.perf_counter()
+    ids = {}
+    for n in sorted(set(int(x) for x in sizes)):
+        mid = C.c_int32()
+        check(lib.ft_mesh_create_kuhn(0, n, n, n, 0.0, 0, C.byref(mid)))
+        ids[n] = mid.value
+    xyz_cache = {}
+    phis = []
+    for k, n in enumerate(sizes):
+        n = int(n)
+        if n not in xyz_cache:
+            xyz_cache[n] = grid_xyz(n)
+        phis.append(fa.phi_field("ellipsoids", xyz_cache[n], seed=1000 + k))
+    N = int(sum(int(n) ** 3 for n in sizes))
+    log(f"batched inputs: {nfields} fields, {N} nodes, {len(ids)} resident meshes "
+        f"({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    b = fa.Reinit(0)
+    b.set_batch_meshes([ids[int(n)] for n in sizes])
+    d_phi0 = torch.from_numpy(np.concatenate(phis)).to(dev)
+    d_phi = torch.empty_like(d_phi0)
+    nbytes = d_phi0.numel() * 8
+
+    def copy(dst, src, nb):
+        d_phi.copy_(d_phi0)
+
+    bs, qb, table = autotune.tune(b, d_phi.data_ptr(), d_phi0.data_ptr(), nbytes, mode=args.mode,
+                                  reps=1, sync=torch.cuda.synchronize, copy=copy, log=log)
+    log(f"autotune picked bin_scale {bs}, query_block {qb}")
+    for _ in range(args.warmup):
+        d_phi.copy_(d_phi0)
+        b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
+                                 query_block=qb, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d_phi.copy_(d_phi0)
+        b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
+                                 query_block=qb, stream=stream)
+    torch.cuda.synchronize()
+    ms_batched = (time.perf_counter() - t0) * 1e3 / args.steps
+    batched_out = d_phi.cpu().numpy()
+    # serial plain: one reinit per field on a single-field handle of its size
+    singles = {}
+    for n, mid in ids.items():
+        h = fa.Reinit(0)
+        h.set_batch_meshes([mid])
+        singles[n] = h
+    d_fields = [torch.from_numpy(p).to(dev) for p in phis]
+    d_work = [torch.empty_like(p) for p in d_fields]
+
+    def serial():
+        for k, n in enumerate(sizes):
+            d_work[k].copy_(d_fields[k])
+            singles[int(n)].reset_phi_device(d_work[k].data_ptr(), None, mode=args.mode,
+                                             bin_scale=bs, query_block=qb, stream=stream)
+    serial()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    serial()
+    torch.cuda.synchronize()
+    ms_serial = (time.perf_counter() - t0) * 1e3
+    serial_out = torch.cat(d_work).cpu().numpy()
+    same = bool(np.array_equal(batched_out.view(np.uint64), serial_out.view(np.uint64)))
+    log(f"batched {ms_batched:.2f} ms, serial plain {ms_serial:.2f} ms, "
+        f"speedup {ms_serial / ms_batched:.2f}x, identical results: {same}")
+    for h in singles.values():
+        h.close()
+    b.close()
+    for mid in ids.values():
+        lib.ft_mesh_release(mid)
+    return {
+        "metric": "Mnodes/s batched reinit (512 fields, one pass) vs serial plain calls",
+        "value": N / (ms_batched * 1e-3) / 1e6, "unit": "Mnodes/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batched,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 distance search; f64 phi and crossing points", "data": "synthetic",
+        "config": {"workload": f"batched: {nfields} fields N~U{{24..96}}, 1-16 ellipsoids, "
+                               f"{args.mode} distance", "nodes": N, "fields": nfields},
+        "serial_plain": {"ms": ms_serial, "Mnodes_per_s": N / (ms_serial * 1e-3) / 1e6},
+        "speedup_vs_serial": ms_serial / ms_batched, "identical_to_serial": same,
+        "autotune": {"bin_scale": bs, "query_block": qb,
+                     "table_ms": [[a, q, round(t, 3)] for a, q, t in table]},
+    }
+
+
+def run_serving(args):
+    """Poisson stream of reinit jobs (sizes log-uniform 32^3..192^3, snapped to
+    multiples of 8; phi from the star and 64-sphere generators) served with
+    continuous batching against serial plain calls, at each rate."""
+    import fluidtissue_amd as fa
+    from fluidtissue_amd.serve import Server, latency_stats
+
+    rng = np.random.default_rng(11)
+    classes = list(range(32, 193, 8))
+    raw = np.exp(rng.uniform(np.log(32), np.log(192), size=args.jobs))
+    dims = [int(min(192, max(32, 8 * round(x / 8)))) for x in raw]
+    gens = rng.integers(0, 2, size=args.jobs)
+    t0 = time.perf_counter()
+    phis = {}
+    for n in sorted(set(dims)):
+        xyz = grid_xyz(n)
+        phis[(n, 0)] = fa.phi_field("star", xyz)
+        phis[(n, 1)] = fa.phi_field("spheres", xyz, seed=0)
+    log(f"serving inputs: {args.jobs} jobs over {len(set(dims))} sizes "
+        f"({time.perf_counter() - t0:.1f} s)")
+    results = {}
+    for rate in args.rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.jobs))
+        res = {}
+        for label, kw in (("served", dict(lanes=2, max_fields=64)),
+                          ("serial_plain", dict(lanes=1, max_fields=1))):
+            with Server(device=0, mode=args.mode, budget_bytes=200e9, **kw) as srv:
+                # warm the mesh cache and the kernels (untimed)
+                warm = [srv.submit((n, n, n), phis[(n, 0)]) for n in sorted(set(dims))]
+                for j in warm:
+                    j.wait(600)
+                srv.batches.clear()
+                start = time.perf_counter()
+                jobs = []
+                for k in range(args.jobs):
+                    ta = start + arrivals[k]
+                    now = time.perf_counter()
+                    if ta > now:
+                        time.sleep(ta - now)
+                    n = dims[k]
+                    jobs.append(srv.submit((n, n, n), phis[(n, int(gens[k]))], t_arrival=ta))
+                for j in jobs:
+                    j.wait(1200)
+                end = max(j.t_done for j in jobs)
+                nodes = sum(j.n for j in jobs)
+                st = latency_stats(jobs)
+                st.update(jobs_per_s=args.jobs / (end - start), Mnodes_per_s=nodes / (end - start) / 1e6,
+                          mean_batch=float(np.mean([b[0] for b in srv.batches])))
+                res[label] = st
+                log(f"rate {rate}/s {label}: {st}")
+        res["throughput_ratio"] = res["served"]["Mnodes_per_s"] / res["serial_plain"]["Mnodes_per_s"]
+        results[str(rate)] = res
+    # saturated capacity: the whole stream queued at once
+    sat = {}
+    for label, kw in (("served", dict(lanes=2, max_fields=64)),
+                      ("serial_plain", dict(lanes=1, max_fields=1))):
+        with Server(device=0, mode=args.mode, budget_bytes=200e9, **kw) as srv:
+            warm = [srv.submit((n, n, n), phis[(n, 0)]) for n in sorted(set(dims))]
+            for j in warm:
+                j.wait(600)
+            start = time.perf_counter()
+            with srv.cv:
+                jobs = [srv.submit((n, n, n), phis[(n, int(g))], t_arrival=start)
+                        for n, g in zip(dims, gens)]
+            for j in jobs:
+                j.wait(1200)
+            end = max(j.t_done for j in jobs)
+            sat[label] = dict(seconds=end - start, jobs_per_s=args.jobs / (end - start),
+                              Mnodes_per_s=sum(j.n for j in jobs) / (end - start) / 1e6)
+            log(f"saturated {label}: {sat[label]}")
+    sat["throughput_ratio"] = sat["served"]["Mnodes_per_s"] / sat["serial_plain"]["Mnodes_per_s"]
+    results["saturated"] = sat
+    return {
+        "metric": "served reinit p50/p99 latency and throughput vs serial plain (Poisson stream)",
+        "value": results[str(args.rates[-1])]["served"]["Mnodes_per_s"], "unit": "Mnodes/s",
+        "n_gpus": 1, "steps": args.jobs, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 distance search; f64 phi and crossing points",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {args.jobs} jobs, sizes log-uniform 32^3-192^3 (multiples "
+                               f"of 8), star/64-sphere phi, {args.mode} distance",
+                   "rates_per_s": args.rates},
+        "rates": results,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS) + ["batched", "serving"])
+    ap.add_argument("--fields", type=int, default=512)
+    ap.add_argument("--jobs", type=int, default=2000)
+    ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 500.0])
     ap.add_argument("--mode", default="triangles", choices=["triangles", "points"])
     ap.add_argument("--bin-scale", type=float, default=0.0)
+    ap.add_argument("--query-block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one field on one GPU; --gpus must be 1 "
@@ -76,6 +278,12 @@ def main():
     import torch
     import fluidtissue_amd as fa
 
+    if args.workload == "batched":
+        print(json.dumps(run_batched(args)), flush=True)
+        return
+    if args.workload == "serving":
+        print(json.dumps(run_serving(args)), flush=True)
+        return
     wl = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     m = fa.kuhn_mesh(wl["n"], jitter=wl["jitter"], seed=1)
@@ -94,8 +302,22 @@ def main():
     def step():
         d_phi.copy_(d_phi0)
         return r.reset_phi_device(d_phi.data_ptr(), None, mode=args.mode, blend="init",
-                                  bin_scale=args.bin_scale, stream=stream)
+                                  bin_scale=args.bin_scale, stream=stream,
+                                  query_block=args.query_block)
 
+    tuned = None
+    if not args.no_tune:
+        from fluidtissue_amd import autotune
+
+        def copy(dst, src, nb):
+            d_phi.copy_(d_phi0)
+        bs, qb, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
+                                      mode=args.mode, reps=1, sync=torch.cuda.synchronize, copy=copy,
+                                      bin_scales=(1.0, 1.5, 2.0), query_blocks=(64, 256))
+        args.bin_scale, args.query_block = bs, qb
+        tuned = {"bin_scale": bs, "query_block": qb,
+                 "table_ms": [[a, q, round(t, 3)] for a, q, t in table]}
+        log(f"autotune picked bin_scale {bs}, query_block {qb}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -158,6 +380,7 @@ def main():
                           "achieved_GBps": b_path / (ms * 1e-3) / 1e9,
                           "frac": b_path / (ms * 1e-3) / 1e9 / peak},
         "cpu_baseline": cpu,
+        "autotune": tuned,
     }
     print(json.dumps(out), flush=True)
     r.close()
