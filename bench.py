@@ -124,8 +124,26 @@ def run_batched(args):
         b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
                                  query_block=qb, stream=stream)
     torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
+    eager_out = d_phi.cpu().numpy()
+    # the same batch as one captured HIP graph per call (captured on the first)
+    for _ in range(max(1, args.warmup)):
+        d_phi.copy_(d_phi0)
+        b.reset_phi_graph_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
+                                 query_block=qb, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d_phi.copy_(d_phi0)
+        b.reset_phi_graph_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
+                                 query_block=qb, stream=stream)
+    torch.cuda.synchronize()
     ms_batched = (time.perf_counter() - t0) * 1e3 / args.steps
     batched_out = d_phi.cpu().numpy()
+    graph_same = bool(np.array_equal(batched_out.view(np.uint64), eager_out.view(np.uint64)))
+    captures, launches = b.graph_stats()
+    log(f"batched eager {ms_eager:.2f} ms, graph {ms_batched:.2f} ms "
+        f"({captures} capture(s), {launches} launches), graph == eager: {graph_same}")
     # serial plain: one reinit per field on a single-field handle of its size
     singles = {}
     for n, mid in ids.items():
@@ -147,7 +165,7 @@ def run_batched(args):
     torch.cuda.synchronize()
     ms_serial = (time.perf_counter() - t0) * 1e3
     serial_out = torch.cat(d_work).cpu().numpy()
-    same = bool(np.array_equal(batched_out.view(np.uint64), serial_out.view(np.uint64)))
+    same = graph_same and bool(np.array_equal(batched_out.view(np.uint64), serial_out.view(np.uint64)))
     log(f"batched {ms_batched:.2f} ms, serial plain {ms_serial:.2f} ms, "
         f"speedup {ms_serial / ms_batched:.2f}x, identical results: {same}")
     for h in singles.values():
@@ -156,7 +174,7 @@ def run_batched(args):
     for mid in ids.values():
         lib.ft_mesh_release(mid)
     return {
-        "metric": "Mnodes/s batched reinit (512 fields, one pass) vs serial plain calls",
+        "metric": "Mnodes/s batched reinit (512 fields, one captured graph) vs serial plain calls",
         "value": N / (ms_batched * 1e-3) / 1e6, "unit": "Mnodes/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batched,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -164,6 +182,8 @@ def run_batched(args):
         "config": {"workload": f"batched: {nfields} fields N~U{{24..96}}, 1-16 ellipsoids, "
                                f"{args.mode} distance", "nodes": N, "fields": nfields},
         "serial_plain": {"ms": ms_serial, "Mnodes_per_s": N / (ms_serial * 1e-3) / 1e6},
+        "eager_batched": {"ms": ms_eager, "Mnodes_per_s": N / (ms_eager * 1e-3) / 1e6},
+        "graph": {"captures": captures, "launches": launches, "identical_to_eager": graph_same},
         "speedup_vs_serial": ms_serial / ms_batched, "identical_to_serial": same,
         "autotune": {"bin_scale": bs, "query_block": qb,
                      "table_ms": [[a, q, round(t, 3)] for a, q, t in table]},

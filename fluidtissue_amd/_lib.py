@@ -105,6 +105,8 @@ _SIGS = {
     "ft_mesh_release": (C.c_int, [C.c_int32]),
     "ft_set_batch_meshes": (C.c_int, [_H, C.c_int32, _P]),
     "ft_reinit_batch": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
+    "ft_reinit_graph": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
+    "ft_graph_stats": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ft_set_timing": (C.c_int, [_H, C.c_int32]),
     "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "ft_kuhn_counts": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64),

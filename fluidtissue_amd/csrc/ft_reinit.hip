@@ -71,6 +71,30 @@ constexpr int kMaxBinsLog2PerAxis = 8;   // <= 256 bins per axis, 2^24 Morton ce
 
 inline unsigned grid1d(int64_t n, int bs = kBS) { return (unsigned)((n + bs - 1) / bs); }
 
+// Scalar slots of a handle (a small device u64 array): stage totals written
+// by the scans, the error word, capacity and traversal overflow flags.
+//   S_BLK    (crossing tets << 32) | triangles
+//   S_EDGES  crossing points (low 32 bits)
+//   S_ERR    ~0, or (code << 40) | first failing tet
+//   S_BINS   (leaves << 32) | primitives
+//   S_OVF    traversal stack/budget overflow of the query
+//   S_CAP    a stage total exceeded the capacity its buffers were sized for
+enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_OVF = 6, S_CAP = 7, S_N = 8 };
+
+// Stage totals of the running reinit as the kernels see them.  Kernels after
+// a scan read the total from the device, so a launch sized for a capacity
+// (a replayed graph) processes exactly this call's elements.
+__device__ __forceinline__ int64_t cnt_xtets(const uint64_t* sc) { return (int64_t)(sc[S_BLK] >> 32); }
+__device__ __forceinline__ int64_t cnt_tris(const uint64_t* sc) { return (int64_t)(sc[S_BLK] & 0xffffffffull); }
+__device__ __forceinline__ int64_t cnt_points(const uint64_t* sc) { return (int64_t)(sc[S_EDGES] & 0xffffffffull); }
+__device__ __forceinline__ int64_t cnt_leaves(const uint64_t* sc) { return (int64_t)(sc[S_BINS] >> 32); }
+// Set once a stage failed (t2e or degenerate tet) or outgrew its buffers:
+// every later kernel of the reinit then does nothing and phi stays as it was.
+__device__ __forceinline__ bool halted(const uint64_t* sc)
+{
+    return sc[S_CAP] != 0ull || sc[S_ERR] != ~0ull;
+}
+
 // ============================================================================
 // Generic kernels: fill and exclusive scan
 // ============================================================================
@@ -252,8 +276,9 @@ __global__ __launch_bounds__(kBS) void k_classify(const int4* tets, int64_t T,
 __global__ __launch_bounds__(kBS) void k_compact(const int4* tets, const uint64_t* sbits,
                                                  const uint64_t* tbits, int64_t nwords,
                                                  const uint64_t* blk_prefix, int32_t* xtet,
-                                                 int32_t* xtri_off)
+                                                 int32_t* xtri_off, const uint64_t* sc)
 {
+    if (halted(sc)) return;
     const int64_t w = (int64_t)blockIdx.x * kBS + threadIdx.x;
     const uint64_t word = w < nwords ? tbits[w] : 0ull;
     uint32_t ct = 0;
@@ -291,6 +316,17 @@ __global__ __launch_bounds__(kBS) void k_compact(const int4* tets, const uint64_
     }
 }
 
+// Capacity check of a reinit whose buffers were sized in advance (graph
+// replay): raises S_CAP when a stage total exceeds what was allocated, which
+// halts every later kernel.  One thread.
+__global__ void k_check_caps(uint64_t* sc, int64_t x_cap, int64_t f_cap, int64_t p_cap)
+{
+    uint64_t v = 0;
+    if (x_cap >= 0 && (cnt_xtets(sc) > x_cap || cnt_tris(sc) > f_cap)) v |= 1ull;
+    if (p_cap >= 0 && cnt_points(sc) > p_cap) v |= 2ull;
+    if (v) sc[S_CAP] |= v;
+}
+
 // Error word: (code << 40) | tet, min-reduced so the lowest failing tet wins
 // within a code; codes: 1 = t2e (FT_ERR_T2E), 2 = degenerate tet.
 __device__ __forceinline__ void flag_error(unsigned long long* err, int code, int64_t t)
@@ -303,12 +339,13 @@ __device__ __forceinline__ void flag_error(unsigned long long* err, int code, in
 // triangle offsets), mark them in the edge bitmap.
 __global__ __launch_bounds__(kBS) void k_xtet(const int4* tets, const int2* t2e3,
                                               const int2* edges, const uint64_t* sbits,
-                                              const int32_t* xtet, int64_t X,
-                                              unsigned long long* ebits, int4* xedges,
-                                              unsigned long long* err)
+                                              const int32_t* xtet, uint64_t* sc,
+                                              unsigned long long* ebits, int4* xedges)
 {
+    if (halted(sc)) return;
     const int64_t x = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (x >= X) return;
+    if (x >= cnt_xtets(sc)) return;
+    unsigned long long* err = (unsigned long long*)(sc + S_ERR);
     const int64_t t = xtet[x];
     const int2 r0 = t2e3[3 * t], r1 = t2e3[3 * t + 1], r2 = t2e3[3 * t + 2];
     const int te[6] = {r0.x, r0.y, r1.x, r1.y, r2.x, r2.y};
@@ -351,8 +388,9 @@ __global__ __launch_bounds__(kBS) void k_points(const uint64_t* ebits, const uin
                                                 int64_t nwords, const int2* edges,
                                                 const float* xyz, const double* phi,
                                                 float* ifp, int32_t* ifp2e,
-                                                unsigned long long* retain)
+                                                unsigned long long* retain, const uint64_t* sc)
 {
+    if (halted(sc)) return;
     const int64_t w = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (w >= nwords) return;
     uint64_t m = ebits[w];
@@ -411,13 +449,15 @@ __device__ __forceinline__ void emit_tri(int32_t n0, int32_t n1, int32_t n2, int
 __global__ __launch_bounds__(kBS) void k_tris(const int4* tets, const float* xyz,
                                               const double* phi, const int32_t* xtet,
                                               const int32_t* xtri_off, const int4* xedges,
-                                              int64_t X, const uint64_t* ebits,
+                                              const uint64_t* ebits,
                                               const uint32_t* eprefix, const float* ifp,
                                               int4* tris, float* tri_norm, int32_t* if_elems,
-                                              unsigned long long* err)
+                                              uint64_t* sc)
 {
+    if (halted(sc)) return;
     const int64_t x = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (x >= X) return;
+    if (x >= cnt_xtets(sc)) return;
+    unsigned long long* err = (unsigned long long*)(sc + S_ERR);
     const int64_t t = xtet[x];
     if_elems[x] = (int32_t)t;
     const int4 ce = xedges[x];
@@ -441,22 +481,24 @@ __global__ __launch_bounds__(kBS) void k_tris(const int4* tets, const float* xyz
     }
 }
 
-__global__ __launch_bounds__(kBS) void k_p2t_count(const int4* tris, int64_t F, uint32_t* deg)
+__global__ __launch_bounds__(kBS) void k_p2t_count(const int4* tris, const uint64_t* sc, uint32_t* deg)
 {
+    if (halted(sc)) return;
     const int64_t f = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (f >= F) return;
+    if (f >= cnt_tris(sc)) return;
     const int4 t = tris[f];
     atomicAdd(&deg[t.x], 1u);
     atomicAdd(&deg[t.y], 1u);
     atomicAdd(&deg[t.z], 1u);
 }
 
-__global__ __launch_bounds__(kBS) void k_p2t_fill(const int4* tris, int64_t F,
+__global__ __launch_bounds__(kBS) void k_p2t_fill(const int4* tris, const uint64_t* sc,
                                                   const uint32_t* off, uint32_t* cursor,
                                                   int32_t* p2t)
 {
+    if (halted(sc)) return;
     const int64_t f = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (f >= F) return;
+    if (f >= cnt_tris(sc)) return;
     const int4 t = tris[f];
     const int v[3] = {t.x, t.y, t.z};
     for (int k = 0; k < 3; k++) {
@@ -468,12 +510,13 @@ __global__ __launch_bounds__(kBS) void k_p2t_fill(const int4* tris, int64_t F,
 // Per point: sort its triangle list ascending (= emission order, the order of
 // p2t in the reference) and sum the areas in that order, each step as
 // float(double(area) + 0.5 * double(norm)) (LevelSet.cpp:356).
-__global__ __launch_bounds__(kBS) void k_point_area(const uint32_t* off, int64_t P,
+__global__ __launch_bounds__(kBS) void k_point_area(const uint32_t* off, const uint64_t* sc,
                                                     int32_t* p2t, const float* tri_norm,
                                                     float* area)
 {
+    if (halted(sc)) return;
     const int64_t v = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (v >= P) return;
+    if (v >= cnt_points(sc)) return;
     const uint32_t b = off[v], e = off[v + 1];
     for (uint32_t i = b + 1; i < e; i++) {
         const int32_t key = p2t[i];
@@ -519,12 +562,18 @@ struct IndexGeom {
 };
 
 // mode 0: points (ifp), mode 1: triangles (centroid).
-__global__ __launch_bounds__(kBS) void k_prim_bin(int mode, int64_t n, const float* ifp,
+__device__ __forceinline__ int64_t cnt_prims(int mode, const uint64_t* sc)
+{
+    return mode == 0 ? cnt_points(sc) : cnt_tris(sc);
+}
+
+__global__ __launch_bounds__(kBS) void k_prim_bin(int mode, const uint64_t* sc, const float* ifp,
                                                   const int4* tris, const int32_t* ifp2e,
                                                   IndexGeom ig, uint32_t* code, uint32_t* count)
 {
+    if (halted(sc)) return;
     const int64_t j = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (j >= n) return;
+    if (j >= cnt_prims(mode, sc)) return;
     float c[3];
     int32_t v0;
     if (mode == 0) {
@@ -578,14 +627,15 @@ __global__ void k_field_roots(int nf, const int64_t* cnt_off, const uint64_t* pr
 }
 
 // Scatter primitives into bin order and write their search records.
-__global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, int64_t n, const float* ifp,
+__global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, const uint64_t* sc, const float* ifp,
                                                       const int4* tris, const uint32_t* code,
                                                       uint32_t* count, const uint64_t* prefix,
                                                       float4* prec, TriRec* trec,
                                                       int32_t* pidx)
 {
+    if (halted(sc)) return;
     const int64_t j = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (j >= n) return;
+    if (j >= cnt_prims(mode, sc)) return;
     const uint32_t c = code[j];
     const uint32_t slot = (uint32_t)prefix[c] + atomicSub(&count[c], 1u) - 1u;
     pidx[slot] = (int32_t)j;
@@ -649,12 +699,13 @@ __global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, int64_t n, const
 
 // Leaf bounds: box of the vertices and, for triangles, a cylinder about the
 // area-weighted mean normal of the leaf's triangles.
-__global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, int64_t L, const uint32_t* leaf_start,
+__global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, const uint64_t* sc, const uint32_t* leaf_start,
                                                      const uint32_t* leaf_cnt, const int32_t* pidx,
                                                      const float* ifp, const int4* tris, Bound* leaf_b)
 {
+    if (halted(sc)) return;
     const int64_t l = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (l >= L) return;
+    if (l >= cnt_leaves(sc)) return;
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     double ns[3] = {0.0, 0.0, 0.0};
     const uint32_t b = leaf_start[l], e = b + leaf_cnt[l];
@@ -715,10 +766,12 @@ __device__ __forceinline__ int kdelta(const uint32_t* keys, int L, int i, int j)
     return __clz(keys[i] ^ keys[j]);
 }
 
-__global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, int L, BNode* nodes,
+__global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, const uint64_t* sc, BNode* nodes,
                                                 int32_t* leaf_parent, int code_bits,
                                                 const int32_t* leaf_off, int32_t* roots)
 {
+    if (halted(sc)) return;
+    const int L = (int)cnt_leaves(sc);
     const int i = blockIdx.x * kBS + threadIdx.x;
     if (i >= L - 1) return;
     const int d = (kdelta(keys, L, i, i + 1) - kdelta(keys, L, i, i - 1)) >= 0 ? 1 : -1;
@@ -812,11 +865,13 @@ __device__ inline Bound merge_bounds(const Bound& A, const Bound& Bb)
 }
 
 // Bottom-up fitting: the second child to arrive at a node merges both.
-__global__ __launch_bounds__(kBS) void k_refit(int L, const int32_t* leaf_parent,
+__global__ __launch_bounds__(kBS) void k_refit(const uint64_t* sc, const int32_t* leaf_parent,
                                                const Bound* leaf_b, BNode* nodes, unsigned* flags)
 {
+    if (halted(sc)) return;
+    const int L = (int)cnt_leaves(sc);
     const int l = blockIdx.x * kBS + threadIdx.x;
-    if (l >= L) return;
+    if (L < 2 || l >= L) return;
     Bound cur_b = leaf_b[l];
     int cur = ~l;
     int p = leaf_parent[l];
@@ -869,6 +924,7 @@ struct QueryArgs {
     unsigned long long* stats;       // nullable: per distance bucket
     float h_est;                     // node spacing (statistics only)
     int probe;                       // greedy first candidate
+    const uint64_t* sc;              // stage totals and halt flags
 };
 
 // Margin of the box prune: a box is skipped only if its lower bound exceeds
@@ -977,6 +1033,7 @@ template <int MODE, int BS>
 __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
 {
     __shared__ int32_t wstack[BS / 64][kStack];
+    if (halted(a.sc)) return;                         // an earlier stage failed: phi untouched
     const int w = threadIdx.x >> 6;
     const int64_t base = (int64_t)blockIdx.x * BS + (threadIdx.x & ~63);
     if (base >= a.n_slots) return;                    // whole wave out of range
@@ -1081,11 +1138,11 @@ __device__ inline uint32_t edge_rank_at(int64_t e, int64_t E, const uint64_t* eb
 __global__ void k_field_counts(int nf, const int64_t* tet_off, const int64_t* edge_off, int64_t T,
                                int64_t E, const uint64_t* tbits, const uint64_t* blk_prefix,
                                const int32_t* xtri_off, const uint64_t* ebits,
-                               const uint32_t* eprefix, uint32_t X, uint32_t F, uint32_t P,
-                               int64_t* out)
+                               const uint32_t* eprefix, const uint64_t* sc, int64_t* out)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
+    if (f >= nf || halted(sc)) return;
+    const uint32_t X = (uint32_t)cnt_xtets(sc), F = (uint32_t)cnt_tris(sc), P = (uint32_t)cnt_points(sc);
     const uint32_t x0 = tet_rank(tet_off[f], T, tbits, blk_prefix, X);
     const uint32_t x1 = tet_rank(tet_off[f + 1], T, tbits, blk_prefix, X);
     const uint32_t f0 = x0 < X ? (uint32_t)xtri_off[x0] : F;
@@ -1240,6 +1297,25 @@ struct FieldHost {
     float h;                         // mean node spacing estimate
 };
 
+struct Caps {
+    int64_t X = 0;   // crossing tets
+    int64_t F = 0;   // triangles
+    int64_t P = 0;   // crossing points
+};
+
+// A captured reinit and what it was captured for.
+struct GraphState {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double* phi = nullptr;
+    double* phi_ifn = nullptr;
+    ft_opts opts{};
+    uint64_t layout = 0;
+    Caps caps;
+    std::vector<void*> bufs;         // device buffers the graph's kernels address
+    int64_t captures = 0, launches = 0;
+};
+
 struct ft_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1248,6 +1324,7 @@ struct ft_ctx {
     std::vector<int64_t> node_off, tet_off, edge_off, slot_off;
     std::vector<FieldHost> fh;
     int64_t N = 0, T = 0, E = 0, n_slots = 0;
+    uint64_t layout_gen = 0;         // bumped whenever the fields change
     bool has_conn = false;
     float h_est = 0.0f;
     DevBuf xyz, tets, edges, t2e, perm, sx, sy, sz, tile_field;
@@ -1261,6 +1338,11 @@ struct ft_ctx {
     DevBuf bin_count, bin_prefix, pcode, prec, trec, pidx, leaf_key, leaf_start, leaf_cnt;
     DevBuf leaf_b, leaf_parent, bnodes, bflags, stats;
     DevBuf d_grids, d_cnt_off, leaf_off, roots;
+    uint64_t idx_layout = ~0ull;     // layout and bin scale the grids were made for
+    float idx_bin_scale = 0.0f;
+    int64_t idx_G = 0;               // bins over all fields
+    int idx_code_bits = 0;
+    GraphState graph;
     // scan scratch and small scalars
     DevBuf scan_bsum, scalars;
     uint64_t* h_scalars = nullptr;   // pinned mirror
@@ -1277,6 +1359,27 @@ struct ft_ctx {
 };
 
 namespace {
+
+std::vector<DevBuf*> all_buffers(ft_ctx* h)
+{
+    return {&h->xyz, &h->tets, &h->edges, &h->t2e, &h->perm, &h->sx, &h->sy, &h->sz,
+            &h->tile_field, &h->d_node_off, &h->d_tet_off, &h->d_edge_off, &h->d_slot_off,
+            &h->fcounts, &h->fclamp, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots,
+            &h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk,
+            &h->xtet, &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp,
+            &h->ifp2e, &h->tris, &h->tri_norm, &h->if_elems, &h->p2t_off,
+            &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
+            &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
+            &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
+            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars};
+}
+
+std::vector<void*> buffer_addresses(ft_ctx* c)
+{
+    std::vector<void*> a;
+    for (DevBuf* b : all_buffers(c)) a.push_back(b->p);
+    return a;
+}
 
 int ensure(DevBuf& b, size_t bytes)
 {
@@ -1375,184 +1478,10 @@ void mark(ft_ctx* c, int k, hipStream_t s)
     if (c->timing) (void)hipEventRecord(c->ev[k], s);
 }
 
-// Scalar slots in ctx->scalars.
-enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_OVF = 6, S_N = 8 };
-
 int check_handle(ft_handle h)
 {
     if (!h) return ft::set_error(FT_ERR_ARG, "null handle");
     FT_HIP(hipSetDevice(h->device));
-    return FT_OK;
-}
-
-// Builds the bin index over `n` primitives (mode 0: points in ifp, mode 1:
-// triangles tris/ifp), one uniform grid per field, and runs the query with the
-// epilogue over every node.
-int search(ft_ctx* c, hipStream_t s, int mode, int64_t n, const float* ifp, const int4* tris,
-           const int32_t* ifp2e, float bin_scale, float band, const double* phi_old,
-           double* phi_out, double* phi_ifn, const uint64_t* retain, int epi, double w_phi,
-           double w_ifn, bool want_nearest, int query_block = 0)
-{
-    uint64_t* sc = c->scalars.as<uint64_t>();
-    const int nf = c->nf;
-    int64_t L = 0;
-    FT_TRY(ensure(c->roots, sizeof(int32_t) * nf));
-    FT_TRY(ensure(c->leaf_off, sizeof(int32_t) * (nf + 1)));
-    FT_TRY(ensure(c->fclamp, sizeof(uint64_t) * nf));
-    if (n > 0) {
-        // per-field grids; keys (field << code_bits) | code stay below 2^30
-        const int fb = field_bits(nf);
-        const int max_lg = std::min(kMaxBinsLog2PerAxis, (30 - fb) / 3);
-        std::vector<Grid> grids(nf);
-        std::vector<int64_t> cnt_off(nf + 1, 0);
-        int lg_max = 0;
-        for (int k = 0; k < nf; k++) {
-            int lg = 0;
-            const float edge = (bin_scale > 0.0f ? bin_scale : 1.0f) * c->fh[k].h;
-            grids[k] = make_grid(c->fh[k], edge, max_lg, &lg);
-            lg_max = std::max(lg_max, lg);
-            cnt_off[k + 1] = cnt_off[k] + ((int64_t)1 << (3 * lg));
-        }
-        const int64_t G = cnt_off[nf];
-        if (G > ((int64_t)1 << 32))
-            return ft::set_error(FT_ERR_ARG, "index: %lld bins exceed 2^32; raise bin_scale", (long long)G);
-        FT_TRY(ensure(c->d_grids, sizeof(Grid) * nf));
-        FT_TRY(ensure(c->d_cnt_off, sizeof(int64_t) * (nf + 1)));
-        FT_HIP(hipMemcpyAsync(c->d_grids.p, grids.data(), sizeof(Grid) * nf, hipMemcpyHostToDevice, s));
-        FT_HIP(hipMemcpyAsync(c->d_cnt_off.p, cnt_off.data(), sizeof(int64_t) * (nf + 1),
-                              hipMemcpyHostToDevice, s));
-        IndexGeom ig{c->d_edge_off.as<int64_t>(), c->d_cnt_off.as<int64_t>(), c->d_grids.as<Grid>(), nf,
-                     3 * lg_max};
-        FT_TRY(ensure(c->bin_count, sizeof(uint32_t) * G));
-        FT_TRY(ensure(c->bin_prefix, sizeof(uint64_t) * G));
-        FT_TRY(ensure(c->pcode, sizeof(uint32_t) * n));
-        FT_TRY(ensure(c->pidx, sizeof(int32_t) * n));
-        if (mode == 0) FT_TRY(ensure(c->prec, sizeof(float4) * n));
-        else FT_TRY(ensure(c->trec, sizeof(TriRec) * n));
-        uint32_t* cnt = c->bin_count.as<uint32_t>();
-        uint64_t* pre = c->bin_prefix.as<uint64_t>();
-        FT_TRY(fill<uint32_t>(s, cnt, G, 0u));
-        hipLaunchKernelGGL(k_prim_bin, dim3(grid1d(n)), dim3(kBS), 0, s, mode, n, ifp, tris,
-                           nf > 1 ? ifp2e : nullptr, ig, c->pcode.as<uint32_t>(), cnt);
-        FT_TRY(scan<uint32_t, uint64_t>(c, s, cnt, pre, G, sc + S_BINS, LoadNonEmpty{}));
-        hipLaunchKernelGGL(k_field_roots, dim3((nf + 1 + 63) / 64), dim3(64), 0, s, nf,
-                           c->d_cnt_off.as<int64_t>(), pre, sc + S_BINS, G,
-                           c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
-        FT_TRY(read_scalars(c, s, S_N));
-        L = (int64_t)(c->h_scalars[S_BINS] >> 32);
-        FT_TRY(ensure(c->leaf_key, sizeof(uint32_t) * L));
-        FT_TRY(ensure(c->leaf_start, sizeof(uint32_t) * L));
-        FT_TRY(ensure(c->leaf_cnt, sizeof(uint32_t) * L));
-        FT_TRY(ensure(c->leaf_b, sizeof(Bound) * L));
-        FT_TRY(ensure(c->leaf_parent, sizeof(int32_t) * L));
-        FT_TRY(ensure(c->bnodes, sizeof(BNode) * std::max<int64_t>(1, L - 1)));
-        FT_TRY(ensure(c->bflags, sizeof(unsigned) * std::max<int64_t>(1, L - 1)));
-        hipLaunchKernelGGL(k_leaves, dim3(grid1d(G)), dim3(kBS), 0, s, cnt, pre, G, ig,
-                           c->leaf_key.as<uint32_t>(), c->leaf_start.as<uint32_t>(),
-                           c->leaf_cnt.as<uint32_t>());
-        hipLaunchKernelGGL(k_prim_scatter, dim3(grid1d(n)), dim3(kBS), 0, s, mode, n, ifp, tris,
-                           c->pcode.as<uint32_t>(), cnt, pre, c->prec.as<float4>(),
-                           c->trec.as<TriRec>(), c->pidx.as<int32_t>());
-        hipLaunchKernelGGL(k_leaf_bounds, dim3(grid1d(L)), dim3(kBS), 0, s, mode, L,
-                           c->leaf_start.as<uint32_t>(), c->leaf_cnt.as<uint32_t>(),
-                           c->pidx.as<int32_t>(), ifp, tris, c->leaf_b.as<Bound>());
-        if (L > 1) {
-            FT_TRY(fill<unsigned>(s, c->bflags.as<unsigned>(), L - 1, 0u));
-            hipLaunchKernelGGL(k_karras, dim3(grid1d(L - 1)), dim3(kBS), 0, s,
-                               c->leaf_key.as<uint32_t>(), (int)L, c->bnodes.as<BNode>(),
-                               c->leaf_parent.as<int32_t>(), 3 * lg_max,
-                               c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
-            hipLaunchKernelGGL(k_refit, dim3(grid1d(L)), dim3(kBS), 0, s, (int)L,
-                               c->leaf_parent.as<int32_t>(), c->leaf_b.as<Bound>(),
-                               c->bnodes.as<BNode>(), c->bflags.as<unsigned>());
-        }
-        FT_TRY(check_launch());
-    } else {
-        FT_TRY(fill<int32_t>(s, c->roots.as<int32_t>(), nf, INT32_MIN));
-    }
-    FT_TRY(fill<uint64_t>(s, c->fclamp.as<uint64_t>(), nf, 0ull));
-    FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
-    QueryArgs q{};
-    q.n_slots = c->n_slots;
-    q.sx = c->sx.as<float>(); q.sy = c->sy.as<float>(); q.sz = c->sz.as<float>();
-    q.perm = c->perm.as<int32_t>();
-    q.tile_field = c->tile_field.as<int32_t>();
-    q.roots = c->roots.as<int32_t>();
-    q.prec = c->prec.as<float4>();
-    q.trec = c->trec.as<TriRec>();
-    q.pidx = c->pidx.as<int32_t>();
-    q.leaf_start = c->leaf_start.as<uint32_t>();
-    q.leaf_cnt = c->leaf_cnt.as<uint32_t>();
-    q.leaf_b = c->leaf_b.as<Bound>();
-    q.nodes = c->bnodes.as<BNode>();
-    q.L = (int)L;
-    q.band = band;
-    q.ifp = ifp;
-    q.tris = tris;
-    q.phi_old = phi_old;
-    q.phi_out = phi_out;
-    q.phi_ifn = phi_ifn;
-    q.retain = retain;
-    if (want_nearest) {
-        FT_TRY(ensure(c->nearest, sizeof(int32_t) * c->N));
-        q.nearest = c->nearest.as<int32_t>();
-    }
-    q.epi = epi;
-    q.w_phi = w_phi;
-    q.w_ifn = w_ifn;
-    q.n_clamped = c->fclamp.as<unsigned long long>();
-    static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
-    q.h_est = c->h_est;
-    if (want_stats) {
-        FT_TRY(ensure(c->stats, 8 * 32));
-        FT_TRY(fill<uint64_t>(s, c->stats.as<uint64_t>(), 32, 0ull));
-        q.stats = c->stats.as<unsigned long long>();
-    }
-    q.overflow = (unsigned*)(sc + S_OVF);
-    q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
-    mark(c, 2, s);
-    if (c->n_slots > 0) {
-        const int bs = query_block == 64 || query_block == 128 ? query_block : 256;
-        const dim3 g(grid1d(c->n_slots, bs)), b(bs);
-        if (mode == 0) {
-            if (bs == 64) hipLaunchKernelGGL((k_query_wave<0, 64>), g, b, 0, s, q);
-            else if (bs == 128) hipLaunchKernelGGL((k_query_wave<0, 128>), g, b, 0, s, q);
-            else hipLaunchKernelGGL((k_query_wave<0, 256>), g, b, 0, s, q);
-        } else {
-            if (bs == 64) hipLaunchKernelGGL((k_query_wave<1, 64>), g, b, 0, s, q);
-            else if (bs == 128) hipLaunchKernelGGL((k_query_wave<1, 128>), g, b, 0, s, q);
-            else hipLaunchKernelGGL((k_query_wave<1, 256>), g, b, 0, s, q);
-        }
-        FT_TRY(check_launch());
-    }
-    mark(c, 3, s);
-    FT_TRY(read_scalars(c, s, S_N));
-    if (want_stats && c->N > 0) {
-        uint64_t st[32];
-        FT_HIP(hipMemcpy(st, c->stats.p, sizeof st, hipMemcpyDeviceToHost));
-        static const char* names[8] = {"out <2h", "out 2-8h", "out 8-32h", "out >32h",
-                                       "in <2h", "in 2-8h", "in 8-32h", "in >32h"};
-        double tn = 0, tl = 0, tp = 0;
-        for (int b = 0; b < 8; b++) { tn += st[4 * b + 1]; tl += st[4 * b + 2]; tp += st[4 * b + 3]; }
-        fprintf(stderr, "[ftreinit] query mode %d: N=%lld L=%lld per node: tree nodes %.1f, "
-                "leaves %.1f, primitives %.1f\n", mode, (long long)c->N, (long long)L,
-                tn / c->N, tl / c->N, tp / c->N);
-        for (int b = 0; b < 8; b++) {
-            const double nn = (double)st[4 * b];
-            if (nn > 0)
-                fprintf(stderr, "[ftreinit]   %-9s %5.1f%% of nodes: tree %.1f leaves %.1f prims %.1f\n",
-                        names[b], 100.0 * nn / c->N, st[4 * b + 1] / nn, st[4 * b + 2] / nn,
-                        st[4 * b + 3] / nn);
-        }
-    }
-    if (c->h_scalars[S_OVF])
-        return ft::set_error(FT_ERR_HIP, "search: traversal exceeded its bounds (%llu)",
-                             (unsigned long long)c->h_scalars[S_OVF]);
-    std::vector<uint64_t> clamp(nf);
-    FT_HIP(hipMemcpy(clamp.data(), c->fclamp.p, 8 * nf, hipMemcpyDeviceToHost));
-    c->field_counts.resize(nf);
-    for (int k = 0; k < nf; k++) c->field_counts[k].n_clamped = (int64_t)clamp[k];
-    c->have_nearest = want_nearest;
     return FT_OK;
 }
 
@@ -1562,6 +1491,7 @@ int set_fields(ft_ctx* c, int nf, const int64_t* node_off, const int64_t* tet_of
                const int64_t* edge_off)
 {
     c->batch_meshes.clear();
+    c->layout_gen++;
     c->nf = nf;
     c->node_off.assign(node_off, node_off + nf + 1);
     c->tet_off.assign(tet_off, tet_off + nf + 1);
@@ -1657,130 +1587,423 @@ int sort_nodes(ft_ctx* c)
     return FT_OK;
 }
 
-// Marching tets on device: fills ifp, ifp2e, tris, if_elems, ifp_area, p2t
-// and the retained-node bitmap.  Returns counts in `cnt`.
-int marching_tets(ft_ctx* c, hipStream_t s, const double* phi, ft_counts* cnt)
+// ----------------------------------------------------------------------------
+// One reinit: stage buffers, the kernel sequence, and the readback
+// ----------------------------------------------------------------------------
+// Buffer capacities of one reinit, in elements.  An eager reinit sizes them
+// to the exact stage totals it reads back between stages.  A captured graph
+// fixes them when it is captured; the device then raises S_CAP when a total
+// outgrows them, every later kernel does nothing, and the host captures again
+// with more room (phi is only written by the last kernel, so nothing is lost).
+// Per-field bin grids of the index for one bin_scale, on the device; cached
+// per field layout.
+int index_setup(ft_ctx* c, hipStream_t s, float bin_scale)
+{
+    const float bs = bin_scale > 0.0f ? bin_scale : 1.0f;
+    if (c->idx_layout == c->layout_gen && c->idx_bin_scale == bs) return FT_OK;
+    const int nf = c->nf;
+    const int fb = field_bits(nf);
+    // keys (field << code_bits) | code stay below 2^30
+    const int max_lg = std::min(kMaxBinsLog2PerAxis, (30 - fb) / 3);
+    std::vector<Grid> grids(nf);
+    std::vector<int64_t> cnt_off(nf + 1, 0);
+    int lg_max = 0;
+    for (int k = 0; k < nf; k++) {
+        int lg = 0;
+        grids[k] = make_grid(c->fh[k], bs * c->fh[k].h, max_lg, &lg);
+        lg_max = std::max(lg_max, lg);
+        cnt_off[k + 1] = cnt_off[k] + ((int64_t)1 << (3 * lg));
+    }
+    const int64_t G = cnt_off[nf];
+    if (G > ((int64_t)1 << 32))
+        return ft::set_error(FT_ERR_ARG, "index: %lld bins exceed 2^32; raise bin_scale", (long long)G);
+    FT_TRY(ensure(c->d_grids, sizeof(Grid) * nf));
+    FT_TRY(ensure(c->d_cnt_off, sizeof(int64_t) * (nf + 1)));
+    FT_HIP(hipMemcpyAsync(c->d_grids.p, grids.data(), sizeof(Grid) * nf, hipMemcpyHostToDevice, s));
+    FT_HIP(hipMemcpyAsync(c->d_cnt_off.p, cnt_off.data(), sizeof(int64_t) * (nf + 1),
+                          hipMemcpyHostToDevice, s));
+    FT_HIP(hipStreamSynchronize(s));
+    c->idx_G = G;
+    c->idx_code_bits = 3 * lg_max;
+    c->idx_layout = c->layout_gen;
+    c->idx_bin_scale = bs;
+    return FT_OK;
+}
+
+inline int64_t at_least1(int64_t v) { return std::max<int64_t>(1, v); }
+inline int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile); }
+
+// Device buffers of the stages of one reinit.  Stage 0: sized by the mesh;
+// 1: by crossing tets and triangles; 2: by crossing points; 3: the index over
+// n primitives (leaves <= min(bins, n)).
+int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_nearest)
+{
+    const int64_t N = c->N, T = c->T, E = c->E, nf = c->nf;
+    const int64_t nw_nodes = (N + 63) / 64, nw_tets = (T + 63) / 64, nw_edges = (E + 63) / 64;
+    const int64_t nblk = (T + kBS - 1) / kBS;
+    if (stage == 0) {
+        FT_TRY(ensure(c->sbits, 8 * at_least1(nw_nodes)));
+        FT_TRY(ensure(c->retain, 8 * at_least1(nw_nodes)));
+        FT_TRY(ensure(c->tbits, 8 * at_least1(nw_tets)));
+        FT_TRY(ensure(c->blk, 8 * at_least1(nblk)));
+        FT_TRY(ensure(c->ebits, 8 * at_least1(nw_edges)));
+        FT_TRY(ensure(c->eprefix, 4 * at_least1(nw_edges)));
+        FT_TRY(ensure(c->fcounts, sizeof(int64_t) * 3 * nf));
+        FT_TRY(ensure(c->fclamp, sizeof(uint64_t) * nf));
+        FT_TRY(ensure(c->roots, sizeof(int32_t) * nf));
+        FT_TRY(ensure(c->leaf_off, sizeof(int32_t) * (nf + 1)));
+        if (want_nearest) FT_TRY(ensure(c->nearest, sizeof(int32_t) * at_least1(N)));
+    } else if (stage == 1) {
+        FT_TRY(ensure(c->xtet, 4 * at_least1(cp.X)));
+        FT_TRY(ensure(c->xtri_off, 4 * at_least1(cp.X)));
+        FT_TRY(ensure(c->xedges, 16 * at_least1(cp.X)));
+        FT_TRY(ensure(c->if_elems, 4 * at_least1(cp.X)));
+        FT_TRY(ensure(c->tris, 16 * at_least1(cp.F)));
+        FT_TRY(ensure(c->tri_norm, 4 * at_least1(cp.F)));
+    } else if (stage == 2) {
+        FT_TRY(ensure(c->ifp, 12 * at_least1(cp.P)));
+        FT_TRY(ensure(c->ifp2e, 4 * at_least1(cp.P)));
+        FT_TRY(ensure(c->ifp_area, 4 * at_least1(cp.P)));
+        FT_TRY(ensure(c->p2t_off, 4 * (cp.P + 1)));
+        FT_TRY(ensure(c->p2t_cursor, 4 * at_least1(cp.P)));
+        FT_TRY(ensure(c->p2t, 4 * at_least1(3 * cp.F)));
+    } else {
+        const int64_t n = mode == 0 ? cp.P : cp.F;
+        const int64_t G = c->idx_G;
+        const int64_t L = std::min(G, n);
+        FT_TRY(ensure(c->bin_count, sizeof(uint32_t) * at_least1(G)));
+        FT_TRY(ensure(c->bin_prefix, sizeof(uint64_t) * at_least1(G)));
+        FT_TRY(ensure(c->pcode, sizeof(uint32_t) * at_least1(n)));
+        FT_TRY(ensure(c->pidx, sizeof(int32_t) * at_least1(n)));
+        if (mode == 0) FT_TRY(ensure(c->prec, sizeof(float4) * at_least1(n)));
+        else FT_TRY(ensure(c->trec, sizeof(TriRec) * at_least1(n)));
+        FT_TRY(ensure(c->leaf_key, sizeof(uint32_t) * at_least1(L)));
+        FT_TRY(ensure(c->leaf_start, sizeof(uint32_t) * at_least1(L)));
+        FT_TRY(ensure(c->leaf_cnt, sizeof(uint32_t) * at_least1(L)));
+        FT_TRY(ensure(c->leaf_b, sizeof(Bound) * at_least1(L)));
+        FT_TRY(ensure(c->leaf_parent, sizeof(int32_t) * at_least1(L)));
+        FT_TRY(ensure(c->bnodes, sizeof(BNode) * at_least1(L - 1)));
+        FT_TRY(ensure(c->bflags, sizeof(unsigned) * at_least1(L - 1)));
+        // scan scratch of the largest scan of a reinit
+        const int64_t nb = std::max({scan_blocks(nblk), scan_blocks(nw_edges), scan_blocks(cp.P),
+                                     scan_blocks(G)});
+        FT_TRY(ensure(c->scan_bsum, sizeof(uint64_t) * nb));
+    }
+    return FT_OK;
+}
+
+// The error word of the reinit (S_ERR) as a status.
+int stage_error(uint64_t ev, ft_counts* cnt)
+{
+    if (ev == ~0ull) return FT_OK;
+    cnt->bad_tet = (int64_t)(ev & ((1ull << 40) - 1));
+    if ((ev >> 40) == 1)
+        return ft::set_error(FT_ERR_T2E,
+                             "tet %lld: crossing edges from t2e do not match its node signs "
+                             "(LevelSet.cpp:400-405)", (long long)cnt->bad_tet);
+    return ft::set_error(FT_ERR_DEGENERATE,
+                         "tet %lld crosses the interface but all its nodes have |phi| <= "
+                         "FLT_MIN (LevelSet.cpp:338-341)", (long long)cnt->bad_tet);
+}
+
+// What the search of a reinit reads and writes.
+struct SearchSpec {
+    int mode = 0;                    // 0: crossing points, 1: triangles
+    float band = 0.0f;
+    const double* phi_old = nullptr;
+    double* phi_out = nullptr;
+    double* phi_ifn = nullptr;
+    bool use_retain = false;         // keep phi_old at crossing-edge nodes (update blend)
+    int epi = EPI_RAW;
+    double w_phi = 0.0, w_ifn = 0.0;
+    bool want_nearest = false;
+    int query_block = 0;
+};
+
+// Enqueues the marching tets of one reinit on stream s.  eager: reads the
+// stage totals back after the two counting scans (sizing the buffers exactly,
+// and stopping early on a failed stage).  !eager (graph capture): no host
+// reads and no allocation; the buffers were sized for `cp` beforehand and
+// k_check_caps guards them.  counts_only: stops once the totals are known
+// (phi is not written).
+int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_nearest, Caps& cp,
+                     bool eager, bool counts_only, ft_counts* cnt)
 {
     const int64_t N = c->N, T = c->T, E = c->E;
     uint64_t* sc = c->scalars.as<uint64_t>();
     const int64_t nw_nodes = (N + 63) / 64, nw_tets = (T + 63) / 64, nw_edges = (E + 63) / 64;
     const int64_t nblk = (T + kBS - 1) / kBS;
-    FT_TRY(ensure(c->sbits, 8 * nw_nodes));
-    FT_TRY(ensure(c->retain, 8 * nw_nodes));
-    FT_TRY(ensure(c->tbits, 8 * nw_tets));
-    FT_TRY(ensure(c->blk, 8 * std::max<int64_t>(1, nblk)));
-    FT_TRY(ensure(c->ebits, 8 * nw_edges));
-    FT_TRY(ensure(c->eprefix, 4 * nw_edges));
+    const bool timed = eager && c->timing && !counts_only;
+    FT_TRY(ensure_stage(c, 0, cp, mode, want_nearest));
     uint64_t* sbits = c->sbits.as<uint64_t>();
-    mark(c, 0, s);
+    if (timed) (void)hipEventRecord(c->ev[0], s);
     hipLaunchKernelGGL(k_node_signs, dim3(grid1d(N)), dim3(kBS), 0, s, phi, N, sbits);
     FT_TRY(fill<uint64_t>(s, c->retain.as<uint64_t>(), nw_nodes, 0ull));
     FT_TRY(fill<uint64_t>(s, c->ebits.as<uint64_t>(), nw_edges, 0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_ERR, 1, ~0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_CAP, 1, 0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_EDGES, 1, 0ull));   // the edge scan writes the low 32 bits
     if (T > 0)
         hipLaunchKernelGGL(k_classify, dim3((unsigned)nblk), dim3(kBS), 0, s, c->tets.as<int4>(), T,
                            sbits, c->tbits.as<uint64_t>(), c->blk.as<uint64_t>());
     FT_TRY(scan<uint64_t, uint64_t>(c, s, c->blk.as<uint64_t>(), c->blk.as<uint64_t>(), nblk,
                                     sc + S_BLK, LoadSame{}));
-    FT_TRY(read_scalars(c, s, S_N));
-    const int64_t X = (int64_t)(c->h_scalars[S_BLK] >> 32);
-    const int64_t F = (int64_t)(c->h_scalars[S_BLK] & 0xffffffffull);
-    FT_TRY(ensure(c->xtet, 4 * std::max<int64_t>(1, X)));
-    FT_TRY(ensure(c->xtri_off, 4 * std::max<int64_t>(1, X)));
-    FT_TRY(ensure(c->xedges, 16 * std::max<int64_t>(1, X)));
-    FT_TRY(ensure(c->if_elems, 4 * std::max<int64_t>(1, X)));
-    FT_TRY(ensure(c->tris, 16 * std::max<int64_t>(1, F)));
-    FT_TRY(ensure(c->tri_norm, 4 * std::max<int64_t>(1, F)));
+    if (eager) {
+        FT_TRY(read_scalars(c, s, S_N));
+        cp.X = (int64_t)(c->h_scalars[S_BLK] >> 32);
+        cp.F = (int64_t)(c->h_scalars[S_BLK] & 0xffffffffull);
+        cnt->n_if_elems = cp.X;
+        cnt->n_tri = cp.F;
+    } else {
+        hipLaunchKernelGGL(k_check_caps, dim3(1), dim3(1), 0, s, sc, cp.X, cp.F, (int64_t)-1);
+    }
+    FT_TRY(ensure_stage(c, 1, cp, mode, want_nearest));
     if (nw_tets > 0)
         hipLaunchKernelGGL(k_compact, dim3(grid1d(nw_tets)), dim3(kBS), 0, s, c->tets.as<int4>(), sbits,
                            c->tbits.as<uint64_t>(), nw_tets, c->blk.as<uint64_t>(),
-                           c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>());
-    if (X > 0)
-        hipLaunchKernelGGL(k_xtet, dim3(grid1d(X)), dim3(kBS), 0, s, c->tets.as<int4>(),
-                           c->t2e.as<int2>(), c->edges.as<int2>(), sbits, c->xtet.as<int32_t>(), X,
-                           c->ebits.as<unsigned long long>(), c->xedges.as<int4>(),
-                           (unsigned long long*)(sc + S_ERR));
-    FT_TRY(fill<uint64_t>(s, sc + S_EDGES, 1, 0ull));   // the scan writes the low 32 bits
+                           c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), sc);
+    if (cp.X > 0)
+        hipLaunchKernelGGL(k_xtet, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tets.as<int4>(),
+                           c->t2e.as<int2>(), c->edges.as<int2>(), sbits, c->xtet.as<int32_t>(), sc,
+                           c->ebits.as<unsigned long long>(), c->xedges.as<int4>());
     FT_TRY(scan<uint64_t, uint32_t>(c, s, c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(),
                                     nw_edges, reinterpret_cast<uint32_t*>(sc + S_EDGES), LoadPopc{}));
-    FT_TRY(read_scalars(c, s, S_N));
-    const int64_t P = (int64_t)(c->h_scalars[S_EDGES] & 0xffffffffull);
-    cnt->n_ifp = P;
-    cnt->n_tri = F;
-    cnt->n_if_elems = X;
-    cnt->bad_tet = -1;
-    if (c->h_scalars[S_ERR] != ~0ull) {
-        const uint64_t ev = c->h_scalars[S_ERR];
-        cnt->bad_tet = (int64_t)(ev & ((1ull << 40) - 1));
-        return ft::set_error(FT_ERR_T2E,
-                             "tet %lld: crossing edges from t2e do not match its node signs "
-                             "(LevelSet.cpp:400-405)", (long long)cnt->bad_tet);
+    if (eager) {
+        FT_TRY(read_scalars(c, s, S_N));
+        cp.P = (int64_t)(c->h_scalars[S_EDGES] & 0xffffffffull);
+        cnt->n_ifp = cp.P;
+        FT_TRY(stage_error(c->h_scalars[S_ERR], cnt));
+        if (counts_only) return FT_OK;
+    } else {
+        hipLaunchKernelGGL(k_check_caps, dim3(1), dim3(1), 0, s, sc, (int64_t)-1, (int64_t)-1, cp.P);
     }
-    FT_TRY(ensure(c->ifp, 12 * std::max<int64_t>(1, P)));
-    FT_TRY(ensure(c->ifp2e, 4 * std::max<int64_t>(1, P)));
-    FT_TRY(ensure(c->ifp_area, 4 * std::max<int64_t>(1, P)));
-    FT_TRY(ensure(c->p2t_off, 4 * (P + 1)));
-    FT_TRY(ensure(c->p2t_cursor, 4 * std::max<int64_t>(1, P)));
-    FT_TRY(ensure(c->p2t, 4 * std::max<int64_t>(1, 3 * F)));
+    FT_TRY(ensure_stage(c, 2, cp, mode, want_nearest));
     if (nw_edges > 0)
         hipLaunchKernelGGL(k_points, dim3(grid1d(nw_edges)), dim3(kBS), 0, s, c->ebits.as<uint64_t>(),
                            c->eprefix.as<uint32_t>(), nw_edges, c->edges.as<int2>(), c->xyz.as<float>(),
                            phi, c->ifp.as<float>(), c->ifp2e.as<int32_t>(),
-                           c->retain.as<unsigned long long>());
-    if (X > 0)
-        hipLaunchKernelGGL(k_tris, dim3(grid1d(X)), dim3(kBS), 0, s, c->tets.as<int4>(), c->xyz.as<float>(),
-                           phi, c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), c->xedges.as<int4>(), X,
+                           c->retain.as<unsigned long long>(), sc);
+    if (cp.X > 0)
+        hipLaunchKernelGGL(k_tris, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tets.as<int4>(), c->xyz.as<float>(),
+                           phi, c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), c->xedges.as<int4>(),
                            c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(), c->ifp.as<float>(),
-                           c->tris.as<int4>(), c->tri_norm.as<float>(), c->if_elems.as<int32_t>(),
-                           (unsigned long long*)(sc + S_ERR));
-    // p2t and areas
+                           c->tris.as<int4>(), c->tri_norm.as<float>(), c->if_elems.as<int32_t>(), sc);
+    // p2t and areas; entries past the point count stay empty
     uint32_t* deg = c->p2t_cursor.as<uint32_t>();
-    FT_TRY(fill<uint32_t>(s, deg, P, 0u));
-    if (F > 0)
-        hipLaunchKernelGGL(k_p2t_count, dim3(grid1d(F)), dim3(kBS), 0, s, c->tris.as<int4>(), F, deg);
-    FT_TRY(scan<uint32_t, uint32_t>(c, s, deg, c->p2t_off.as<uint32_t>(), P,
-                                    c->p2t_off.as<uint32_t>() + P, LoadSame{}));
-    FT_TRY(fill<uint32_t>(s, deg, P, 0u));
-    if (F > 0)
-        hipLaunchKernelGGL(k_p2t_fill, dim3(grid1d(F)), dim3(kBS), 0, s, c->tris.as<int4>(), F,
+    FT_TRY(fill<uint32_t>(s, deg, cp.P, 0u));
+    if (cp.F > 0)
+        hipLaunchKernelGGL(k_p2t_count, dim3(grid1d(cp.F)), dim3(kBS), 0, s, c->tris.as<int4>(), sc, deg);
+    FT_TRY(scan<uint32_t, uint32_t>(c, s, deg, c->p2t_off.as<uint32_t>(), cp.P,
+                                    c->p2t_off.as<uint32_t>() + cp.P, LoadSame{}));
+    FT_TRY(fill<uint32_t>(s, deg, cp.P, 0u));
+    if (cp.F > 0)
+        hipLaunchKernelGGL(k_p2t_fill, dim3(grid1d(cp.F)), dim3(kBS), 0, s, c->tris.as<int4>(), sc,
                            c->p2t_off.as<uint32_t>(), deg, c->p2t.as<int32_t>());
-    if (P > 0)
-        hipLaunchKernelGGL(k_point_area, dim3(grid1d(P)), dim3(kBS), 0, s, c->p2t_off.as<uint32_t>(), P,
-                           c->p2t.as<int32_t>(), c->tri_norm.as<float>(), c->ifp_area.as<float>());
-    FT_TRY(check_launch());
-    mark(c, 1, s);
-    FT_TRY(read_scalars(c, s, S_N));
-    if (c->h_scalars[S_ERR] != ~0ull) {
-        const uint64_t ev = c->h_scalars[S_ERR];
-        cnt->bad_tet = (int64_t)(ev & ((1ull << 40) - 1));
-        return ft::set_error(FT_ERR_DEGENERATE,
-                             "tet %lld crosses the interface but all its nodes have |phi| <= "
-                             "FLT_MIN (LevelSet.cpp:338-341)", (long long)cnt->bad_tet);
-    }
-    // per-field counts (one field: the totals)
-    if (c->nf > 1) {
-        FT_TRY(ensure(c->fcounts, sizeof(int64_t) * 3 * c->nf));
+    if (cp.P > 0)
+        hipLaunchKernelGGL(k_point_area, dim3(grid1d(cp.P)), dim3(kBS), 0, s, c->p2t_off.as<uint32_t>(),
+                           sc, c->p2t.as<int32_t>(), c->tri_norm.as<float>(), c->ifp_area.as<float>());
+    if (c->nf > 1)
         hipLaunchKernelGGL(k_field_counts, dim3((c->nf + 63) / 64), dim3(64), 0, s, c->nf,
                            c->d_tet_off.as<int64_t>(), c->d_edge_off.as<int64_t>(), T, E,
                            c->tbits.as<uint64_t>(), c->blk.as<uint64_t>(), c->xtri_off.as<int32_t>(),
-                           c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(), (uint32_t)X,
-                           (uint32_t)F, (uint32_t)P, c->fcounts.as<int64_t>());
-        FT_TRY(check_launch());
-        std::vector<int64_t> fc(3 * c->nf);
-        FT_HIP(hipMemcpyAsync(fc.data(), c->fcounts.p, sizeof(int64_t) * 3 * c->nf, hipMemcpyDeviceToHost, s));
-        FT_HIP(hipStreamSynchronize(s));
-        for (int k = 0; k < c->nf; k++) {
-            c->field_counts[k] = ft_counts{};
-            c->field_counts[k].n_ifp = fc[3 * k];
-            c->field_counts[k].n_tri = fc[3 * k + 1];
-            c->field_counts[k].n_if_elems = fc[3 * k + 2];
-            c->field_counts[k].bad_tet = -1;
-        }
-    } else {
-        c->field_counts.assign(1, *cnt);
+                           c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(), sc,
+                           c->fcounts.as<int64_t>());
+    FT_TRY(check_launch());
+    if (timed) (void)hipEventRecord(c->ev[1], s);
+    if (eager) {
+        FT_TRY(read_scalars(c, s, S_N));
+        FT_TRY(stage_error(c->h_scalars[S_ERR], cnt));
     }
-    c->have_interface = true;
     return FT_OK;
+}
+
+// Enqueues the bin index over the primitives (crossing points c->ifp, or
+// triangles c->tris) and the distance query with its epilogue.  The number of
+// primitives is read on the device (S_EDGES or S_BLK); cp bounds it.
+int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& cp, bool eager)
+{
+    const int mode = sp.mode;
+    uint64_t* sc = c->scalars.as<uint64_t>();
+    const bool timed = eager && c->timing;
+    const int64_t n_cap = mode == 0 ? cp.P : cp.F;
+    const float* ifp = c->ifp.as<float>();
+    const int4* tris = mode == 0 ? nullptr : c->tris.as<int4>();
+    const int nf = c->nf;
+    if (c->idx_layout != c->layout_gen)
+        return ft::set_error(FT_ERR_STATE, "index grids not set up for this field layout");
+    FT_TRY(ensure_stage(c, 3, cp, mode, sp.want_nearest));
+    const int64_t G = c->idx_G;
+    const int64_t L_cap = std::min(G, n_cap);
+    IndexGeom ig{c->d_edge_off.as<int64_t>(), c->d_cnt_off.as<int64_t>(), c->d_grids.as<Grid>(), nf,
+                 c->idx_code_bits};
+    uint32_t* bcnt = c->bin_count.as<uint32_t>();
+    uint64_t* pre = c->bin_prefix.as<uint64_t>();
+    FT_TRY(fill<uint32_t>(s, bcnt, G, 0u));
+    if (n_cap > 0)
+        hipLaunchKernelGGL(k_prim_bin, dim3(grid1d(n_cap)), dim3(kBS), 0, s, mode, sc, ifp, tris,
+                           nf > 1 ? c->ifp2e.as<int32_t>() : nullptr, ig, c->pcode.as<uint32_t>(), bcnt);
+    FT_TRY(scan<uint32_t, uint64_t>(c, s, bcnt, pre, G, sc + S_BINS, LoadNonEmpty{}));
+    hipLaunchKernelGGL(k_field_roots, dim3((nf + 1 + 63) / 64), dim3(64), 0, s, nf,
+                       c->d_cnt_off.as<int64_t>(), pre, sc + S_BINS, G,
+                       c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
+    if (n_cap > 0) {
+        hipLaunchKernelGGL(k_leaves, dim3(grid1d(G)), dim3(kBS), 0, s, bcnt, pre, G, ig,
+                           c->leaf_key.as<uint32_t>(), c->leaf_start.as<uint32_t>(),
+                           c->leaf_cnt.as<uint32_t>());
+        hipLaunchKernelGGL(k_prim_scatter, dim3(grid1d(n_cap)), dim3(kBS), 0, s, mode, sc, ifp, tris,
+                           c->pcode.as<uint32_t>(), bcnt, pre, c->prec.as<float4>(),
+                           c->trec.as<TriRec>(), c->pidx.as<int32_t>());
+        hipLaunchKernelGGL(k_leaf_bounds, dim3(grid1d(L_cap)), dim3(kBS), 0, s, mode, sc,
+                           c->leaf_start.as<uint32_t>(), c->leaf_cnt.as<uint32_t>(),
+                           c->pidx.as<int32_t>(), ifp, tris, c->leaf_b.as<Bound>());
+    }
+    if (L_cap > 1) {
+        FT_TRY(fill<unsigned>(s, c->bflags.as<unsigned>(), L_cap - 1, 0u));
+        hipLaunchKernelGGL(k_karras, dim3(grid1d(L_cap - 1)), dim3(kBS), 0, s,
+                           c->leaf_key.as<uint32_t>(), sc, c->bnodes.as<BNode>(),
+                           c->leaf_parent.as<int32_t>(), c->idx_code_bits,
+                           c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
+        hipLaunchKernelGGL(k_refit, dim3(grid1d(L_cap)), dim3(kBS), 0, s, sc,
+                           c->leaf_parent.as<int32_t>(), c->leaf_b.as<Bound>(),
+                           c->bnodes.as<BNode>(), c->bflags.as<unsigned>());
+    }
+    FT_TRY(check_launch());
+
+    // ---- distance query with the fused epilogue --------------------------------
+    FT_TRY(fill<uint64_t>(s, c->fclamp.as<uint64_t>(), nf, 0ull));
+    QueryArgs q{};
+    q.n_slots = c->n_slots;
+    q.sx = c->sx.as<float>(); q.sy = c->sy.as<float>(); q.sz = c->sz.as<float>();
+    q.perm = c->perm.as<int32_t>();
+    q.tile_field = c->tile_field.as<int32_t>();
+    q.roots = c->roots.as<int32_t>();
+    q.prec = c->prec.as<float4>();
+    q.trec = c->trec.as<TriRec>();
+    q.pidx = c->pidx.as<int32_t>();
+    q.leaf_start = c->leaf_start.as<uint32_t>();
+    q.leaf_cnt = c->leaf_cnt.as<uint32_t>();
+    q.leaf_b = c->leaf_b.as<Bound>();
+    q.nodes = c->bnodes.as<BNode>();
+    q.L = (int)L_cap;
+    q.band = sp.band;
+    q.ifp = ifp;
+    q.tris = tris;
+    q.phi_old = sp.phi_old;
+    q.phi_out = sp.phi_out;
+    q.phi_ifn = sp.phi_ifn;
+    q.retain = sp.use_retain ? c->retain.as<uint64_t>() : nullptr;   // resolved after allocation
+    q.nearest = sp.want_nearest ? c->nearest.as<int32_t>() : nullptr;
+    q.epi = sp.epi;
+    q.w_phi = sp.w_phi;
+    q.w_ifn = sp.w_ifn;
+    q.n_clamped = c->fclamp.as<unsigned long long>();
+    static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
+    q.h_est = c->h_est;
+    if (want_stats && eager) {
+        FT_TRY(ensure(c->stats, 8 * 32));
+        FT_TRY(fill<uint64_t>(s, c->stats.as<uint64_t>(), 32, 0ull));
+        q.stats = c->stats.as<unsigned long long>();
+    }
+    q.overflow = (unsigned*)(sc + S_OVF);
+    q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
+    q.sc = sc;
+    if (timed) (void)hipEventRecord(c->ev[2], s);
+    if (c->n_slots > 0) {
+        const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
+        const dim3 g(grid1d(c->n_slots, qb)), b(qb);
+        if (mode == 0) {
+            if (qb == 64) hipLaunchKernelGGL((k_query_wave<0, 64>), g, b, 0, s, q);
+            else if (qb == 128) hipLaunchKernelGGL((k_query_wave<0, 128>), g, b, 0, s, q);
+            else hipLaunchKernelGGL((k_query_wave<0, 256>), g, b, 0, s, q);
+        } else {
+            if (qb == 64) hipLaunchKernelGGL((k_query_wave<1, 64>), g, b, 0, s, q);
+            else if (qb == 128) hipLaunchKernelGGL((k_query_wave<1, 128>), g, b, 0, s, q);
+            else hipLaunchKernelGGL((k_query_wave<1, 256>), g, b, 0, s, q);
+        }
+        FT_TRY(check_launch());
+    }
+    if (timed) (void)hipEventRecord(c->ev[3], s);
+    return FT_OK;
+}
+
+// After the reinit's kernels: totals, errors and per-field counts from the
+// device (one readback).  Returns FT_ERR_BUDGET internally when a stage
+// outgrew the capacities of a captured graph.
+int collect(ft_ctx* c, hipStream_t s, bool want_nearest, ft_counts* cnt)
+{
+    const int nf = c->nf;
+    FT_HIP(hipMemcpyAsync(c->h_scalars, c->scalars.p, sizeof(uint64_t) * S_N, hipMemcpyDeviceToHost, s));
+    std::vector<uint64_t> clamp(nf);
+    std::vector<int64_t> fc(nf > 1 ? 3 * nf : 0);
+    FT_HIP(hipMemcpyAsync(clamp.data(), c->fclamp.p, 8 * nf, hipMemcpyDeviceToHost, s));
+    if (nf > 1)
+        FT_HIP(hipMemcpyAsync(fc.data(), c->fcounts.p, sizeof(int64_t) * 3 * nf, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
+    const uint64_t* hs = c->h_scalars;
+    *cnt = ft_counts{};
+    cnt->n_if_elems = (int64_t)(hs[S_BLK] >> 32);
+    cnt->n_tri = (int64_t)(hs[S_BLK] & 0xffffffffull);
+    cnt->n_ifp = (int64_t)(hs[S_EDGES] & 0xffffffffull);
+    cnt->bad_tet = -1;
+    if (hs[S_CAP]) return FT_ERR_BUDGET;
+    FT_TRY(stage_error(hs[S_ERR], cnt));
+    c->have_interface = true;
+    if (hs[S_OVF])
+        return ft::set_error(FT_ERR_HIP, "search: traversal exceeded its bounds (%llu)",
+                             (unsigned long long)hs[S_OVF]);
+    c->field_counts.assign(nf, ft_counts{});
+    for (int k = 0; k < nf; k++) {
+        ft_counts& f = c->field_counts[k];
+        if (nf > 1) {
+            f.n_ifp = fc[3 * k];
+            f.n_tri = fc[3 * k + 1];
+            f.n_if_elems = fc[3 * k + 2];
+        } else {
+            f = *cnt;
+        }
+        f.bad_tet = -1;
+        f.n_clamped = (int64_t)clamp[k];
+        cnt->n_clamped += f.n_clamped;
+    }
+    c->have_nearest = want_nearest;
+    return FT_OK;
+}
+
+void print_query_stats(ft_ctx* c, int mode)
+{
+    uint64_t st[32];
+    if (hipMemcpy(st, c->stats.p, sizeof st, hipMemcpyDeviceToHost) != hipSuccess) return;
+    static const char* names[8] = {"out <2h", "out 2-8h", "out 8-32h", "out >32h",
+                                   "in <2h", "in 2-8h", "in 8-32h", "in >32h"};
+    double tn = 0, tl = 0, tp = 0;
+    for (int b = 0; b < 8; b++) { tn += st[4 * b + 1]; tl += st[4 * b + 2]; tp += st[4 * b + 3]; }
+    fprintf(stderr, "[ftreinit] query mode %d: N=%lld per node: tree nodes %.1f, leaves %.1f, "
+            "primitives %.1f\n", mode, (long long)c->N, tn / c->N, tl / c->N, tp / c->N);
+    for (int b = 0; b < 8; b++) {
+        const double nn = (double)st[4 * b];
+        if (nn > 0)
+            fprintf(stderr, "[ftreinit]   %-9s %5.1f%% of nodes: tree %.1f leaves %.1f prims %.1f\n",
+                    names[b], 100.0 * nn / c->N, st[4 * b + 1] / nn, st[4 * b + 2] / nn,
+                    st[4 * b + 3] / nn);
+    }
+}
+
+SearchSpec reinit_search_spec(ft_ctx* c, double* phi, double* phi_ifn, const ft_opts* o)
+{
+    SearchSpec sp;
+    sp.mode = o->mode == FT_DIST_POINTS ? 0 : 1;
+    sp.band = (float)o->band;
+    sp.phi_old = phi;
+    sp.phi_out = phi;                // read and written per node in place
+    sp.phi_ifn = phi_ifn;
+    sp.use_retain = true;
+    sp.epi = (o->blend == FT_BLEND_INIT) ? EPI_INIT : EPI_UPDATE;
+    sp.w_phi = o->w_phi;
+    sp.w_ifn = o->w_ifn;
+    sp.want_nearest = o->want_nearest != 0;
+    sp.query_block = o->query_block;
+    return sp;
 }
 
 int validate_opts(const ft_opts* o)
@@ -1806,24 +2029,19 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     if (!c->has_conn) return ft::set_error(FT_ERR_STATE, "reinit before a mesh was set");
     if (!d_phi) return ft::set_error(FT_ERR_ARG, "null phi");
     c->have_interface = false;
+    c->have_nearest = false;
     ft_counts cnt{};
-    int rc = marching_tets(c, s, d_phi, &cnt);
+    cnt.bad_tet = -1;
+    int rc = index_setup(c, s, o->bin_scale);
+    Caps cp;
+    const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
+    if (rc == FT_OK) rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, cp, true, false, &cnt);
+    if (rc == FT_OK) rc = enqueue_search(c, s, sp, cp, true);
+    if (rc == FT_OK) rc = collect(c, s, sp.want_nearest, &cnt);
     if (counts) *counts = cnt;
     c->last = cnt;
-    if (rc) return rc;
-    // the query reads phi_old and writes phi_out in place per node
-    const int epi = (o->blend == FT_BLEND_INIT) ? EPI_INIT : EPI_UPDATE;
-    if (o->mode == FT_DIST_POINTS)
-        rc = search(c, s, 0, cnt.n_ifp, c->ifp.as<float>(), nullptr, c->ifp2e.as<int32_t>(),
-                    o->bin_scale, (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(),
-                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0, o->query_block);
-    else
-        rc = search(c, s, 1, cnt.n_tri, c->ifp.as<float>(), c->tris.as<int4>(), c->ifp2e.as<int32_t>(),
-                    o->bin_scale, (float)o->band, d_phi, d_phi, d_phi_ifn, c->retain.as<uint64_t>(),
-                    epi, o->w_phi, o->w_ifn, o->want_nearest != 0, o->query_block);
-    for (const ft_counts& f : c->field_counts) cnt.n_clamped += f.n_clamped;
-    if (counts) *counts = cnt;
-    c->last = cnt;
+    static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
+    if (rc == FT_OK && want_stats && c->N > 0) print_query_stats(c, sp.mode);
     if (rc == FT_OK && c->timing) {
         float ms[3] = {0, 0, 0};
         (void)hipEventSynchronize(c->ev[3]);
@@ -1832,6 +2050,121 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
         c->timed_calls++;
     }
     return rc;
+}
+
+// ----------------------------------------------------------------------------
+// Graph form: the whole reinit of the resident fields as one captured graph
+// ----------------------------------------------------------------------------
+inline int64_t with_room(int64_t v) { return v + v / 4 + 256; }
+
+bool same_opts(const ft_opts& a, const ft_opts& b)
+{
+    return a.mode == b.mode && a.blend == b.blend && a.w_phi == b.w_phi && a.w_ifn == b.w_ifn &&
+           a.band == b.band && a.bin_scale == b.bin_scale && a.want_nearest == b.want_nearest &&
+           a.query_block == b.query_block;
+}
+
+void drop_graph(ft_ctx* c)
+{
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
+    c->graph.exec = nullptr;
+    c->graph.graph = nullptr;
+}
+
+// Captures the reinit of (d_phi, d_phi_ifn, o) with capacities cp on the
+// handle's stream.  Everything it touches is allocated before the capture.
+int capture_reinit(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, const Caps& cp)
+{
+    drop_graph(c);
+    const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
+    for (int st = 0; st < 4; st++) FT_TRY(ensure_stage(c, st, cp, sp.mode, sp.want_nearest));
+    FT_HIP(hipStreamSynchronize(c->stream));
+    FT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    Caps cc = cp;
+    ft_counts unused{};
+    int rc = enqueue_marching(c, c->stream, d_phi, sp.mode, sp.want_nearest, cc, false, false, &unused);
+    if (rc == FT_OK) rc = enqueue_search(c, c->stream, sp, cc, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != FT_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return ft::set_error(FT_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiateWithFlags(&exec, graph, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return ft::set_error(FT_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    GraphState& g = c->graph;
+    g.graph = graph;
+    g.exec = exec;
+    g.phi = d_phi;
+    g.phi_ifn = d_phi_ifn;
+    g.opts = *o;
+    g.layout = c->layout_gen;
+    g.caps = cp;
+    g.bufs = buffer_addresses(c);
+    g.captures++;
+    return FT_OK;
+}
+
+// Reinit of the resident fields through a captured graph: the first call (or
+// a call whose pointers, options, fields or buffer sizes differ from the
+// captured ones) sizes the stages with a counting pass over phi, captures
+// with headroom and launches; later calls launch the same graph.  A call
+// whose interface outgrows the headroom is detected on the device (nothing
+// is written), captured again with room for it, and launched again.
+int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, ft_counts* counts,
+                 hipStream_t s)
+{
+    FT_TRY(validate_opts(o));
+    if (!c->has_conn) return ft::set_error(FT_ERR_STATE, "reinit before a mesh was set");
+    if (!d_phi) return ft::set_error(FT_ERR_ARG, "null phi");
+    c->have_interface = false;
+    c->have_nearest = false;
+    GraphState& g = c->graph;
+    ft_counts cnt{};
+    cnt.bad_tet = -1;
+    FT_TRY(index_setup(c, s, o->bin_scale));
+    const bool same = g.exec && g.phi == d_phi && g.phi_ifn == d_phi_ifn && same_opts(g.opts, *o) &&
+                      g.layout == c->layout_gen;
+    Caps cp = g.caps;
+    bool fresh = !same;
+    if (fresh) {
+        Caps exact;
+        const int mode = o->mode == FT_DIST_POINTS ? 0 : 1;
+        const int rc = enqueue_marching(c, s, d_phi, mode, o->want_nearest != 0, exact, true, true, &cnt);
+        if (rc != FT_OK) {
+            if (counts) *counts = cnt;
+            c->last = cnt;
+            return rc;
+        }
+        cp.X = with_room(exact.X);
+        cp.F = with_room(exact.F);
+        cp.P = with_room(exact.P);
+    }
+    for (int attempt = 0; attempt < 4; attempt++) {
+        if (fresh || buffer_addresses(c) != g.bufs) FT_TRY(capture_reinit(c, d_phi, d_phi_ifn, o, cp));
+        FT_HIP(hipGraphLaunch(g.exec, s));
+        g.launches++;
+        const int rc = collect(c, s, o->want_nearest != 0, &cnt);
+        if (rc != FT_ERR_BUDGET) {
+            if (counts) *counts = cnt;
+            c->last = cnt;
+            return rc;
+        }
+        // Outgrown: the totals of the stages that ran are this call's.
+        const uint64_t why = c->h_scalars[S_CAP];
+        cp.X = std::max(cp.X, with_room(cnt.n_if_elems));
+        cp.F = std::max(cp.F, with_room(cnt.n_tri));
+        if (why & 2ull) cp.P = std::max(cp.P, with_room(cnt.n_ifp));
+        fresh = true;
+    }
+    return ft::set_error(FT_ERR_HIP, "graph reinit: stage capacities did not settle");
 }
 
 std::mutex g_create_mu;
@@ -1899,17 +2232,8 @@ int ft_destroy(ft_handle h)
     if (!h) return FT_OK;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->xyz, &h->tets, &h->edges, &h->t2e, &h->perm, &h->sx, &h->sy, &h->sz,
-                      &h->tile_field, &h->d_node_off, &h->d_tet_off, &h->d_edge_off, &h->d_slot_off,
-                      &h->fcounts, &h->fclamp, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots,
-                      &h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk,
-                      &h->xtet, &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp,
-                      &h->ifp2e, &h->tris, &h->tri_norm, &h->if_elems, &h->p2t_off,
-                      &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
-                      &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
-                      &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
-                      &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars};
-    for (DevBuf* b : bufs) release(*b);
+    drop_graph(h);
+    for (DevBuf* b : all_buffers(h)) release(*b);
     if (h->h_scalars) (void)hipHostFree(h->h_scalars);
     for (int k = 0; k < 4; k++)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
@@ -2265,10 +2589,30 @@ int ft_reinit_batch(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts
     return rc;
 }
 
+int ft_reinit_graph(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts* o,
+                    ft_counts* per_field, void* stream)
+{
+    FT_TRY(check_handle(h));
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    ft_counts tot{};
+    const int rc = reinit_graph(h, d_phi, d_phi_ifn, o, &tot, s);
+    if (per_field)
+        for (int k = 0; k < h->nf && k < (int)h->field_counts.size(); k++) per_field[k] = h->field_counts[k];
+    return rc;
+}
+
+int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches)
+{
+    FT_TRY(check_handle(h));
+    if (captures) *captures = h->graph.captures;
+    if (launches) *launches = h->graph.launches;
+    return FT_OK;
+}
+
 int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* phi_n)
 {
     FT_TRY(check_handle(h));
-    if (n_ifp < 0 || (n_ifp > 0 && !ifp) || (h->N > 0 && !phi_n))
+    if (n_ifp < 0 || n_ifp > INT32_MAX || (n_ifp > 0 && !ifp) || (h->N > 0 && !phi_n))
         return ft::set_error(FT_ERR_ARG, "ft_reset_phi_points: bad arguments");
     if (h->nf != 1) return ft::set_error(FT_ERR_STATE, "ft_reset_phi_points needs a single field");
     hipStream_t s = h->stream;
@@ -2277,9 +2621,28 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     if (n_ifp > 0) FT_HIP(hipMemcpyAsync(h->ifp.p, ifp, 12 * n_ifp, hipMemcpyHostToDevice, s));
     if (h->N > 0) FT_HIP(hipMemcpyAsync(h->phi_tmp.p, phi_n, 8 * h->N, hipMemcpyHostToDevice, s));
     h->have_interface = false;
-    FT_TRY(search(h, s, 0, n_ifp, h->ifp.as<float>(), nullptr, nullptr, 0.0f, 0.0f,
-                  h->phi_tmp.as<double>(), h->phi_tmp.as<double>(), nullptr, nullptr, EPI_RAW, 0.0,
-                  0.0, true));
+    // the crossing points are given: their count goes where the edge scan
+    // would have put it, and the search runs as in a reinit
+    Caps cp;
+    cp.P = n_ifp;
+    SearchSpec sp;
+    sp.mode = 0;
+    sp.phi_old = h->phi_tmp.as<double>();
+    sp.phi_out = h->phi_tmp.as<double>();
+    sp.epi = EPI_RAW;
+    sp.want_nearest = true;
+    uint64_t* sc = h->scalars.as<uint64_t>();
+    FT_TRY(index_setup(h, s, 0.0f));
+    FT_TRY(ensure_stage(h, 0, cp, 0, true));
+    FT_TRY(fill<uint64_t>(s, sc + S_BLK, 1, 0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_EDGES, 1, (uint64_t)n_ifp));
+    FT_TRY(fill<uint64_t>(s, sc + S_ERR, 1, ~0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_CAP, 1, 0ull));
+    FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
+    FT_TRY(enqueue_search(h, s, sp, cp, true));
+    ft_counts cnt{};
+    FT_TRY(collect(h, s, true, &cnt));
+    h->have_interface = false;        // points only: no triangles, p2t or areas
     if (h->N > 0) FT_HIP(hipMemcpyAsync(phi_n, h->phi_tmp.p, 8 * h->N, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
     return FT_OK;

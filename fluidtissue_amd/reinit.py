@@ -253,10 +253,40 @@ class Reinit(_Handle):
     def reset_phi_batch_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
                                bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
         o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
-        check(lib.ft_reinit_batch(self._h, C.c_void_p(phi_ptr),
-                                  C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
-                                  C.c_void_p(stream) if stream else None))
-        return self.field_counts()
+        rc = lib.ft_reinit_batch(self._h, C.c_void_p(phi_ptr),
+                                 C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
+                                 C.c_void_p(stream) if stream else None)
+        return self._batch_done(rc)
+
+    def _batch_done(self, rc):
+        """Per-field counts of a batch call; the totals become self.counts."""
+        check(rc)
+        per = self.field_counts()
+        tot = _lib.FtCounts()
+        for c in per:
+            tot.n_ifp += c.n_ifp
+            tot.n_tri += c.n_tri
+            tot.n_if_elems += c.n_if_elems
+            tot.n_clamped += c.n_clamped
+        tot.bad_tet = -1
+        self.counts = tot
+        return per
+
+    def reset_phi_graph_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
+                               bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
+        """reset_phi_batch_device through one captured HIP graph per call
+        (captured on the first call for these buffers and options)."""
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
+        rc = lib.ft_reinit_graph(self._h, C.c_void_p(phi_ptr),
+                                 C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
+                                 C.c_void_p(stream) if stream else None)
+        return self._batch_done(rc)
+
+    def graph_stats(self):
+        """(captures, launches) of the graph form on this handle."""
+        a, b = C.c_int64(), C.c_int64()
+        check(lib.ft_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_timing(self, on=True):
         check(lib.ft_set_timing(self._h, 1 if on else 0))

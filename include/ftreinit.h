@@ -143,6 +143,21 @@ int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges,
 int ft_reinit_batch(ft_handle h, double* d_phi, double* d_phi_ifn,
                     const ft_opts* o, ft_counts* per_field, void* stream);
 
+/* ft_reinit_batch as one captured HIP graph (no counterpart in the
+ * reference): every kernel of the reinit of all resident fields, with no
+ * host round trip between them, replayed by one hipGraphLaunch per call.
+ * The first call, or one whose d_phi, d_phi_ifn, options, fields or buffers
+ * differ from the captured ones, runs a counting pass over phi (reads only),
+ * captures with headroom over its stage sizes and launches.  A later call
+ * whose interface outgrows the headroom is caught on the device before phi
+ * is written, captured again with room and launched again.  Results are
+ * those of ft_reinit_batch, bit for bit.  Returns after the graph has run. */
+int ft_reinit_graph(ft_handle h, double* d_phi, double* d_phi_ifn,
+                    const ft_opts* o, ft_counts* per_field, void* stream);
+
+/* Captures and launches of ft_reinit_graph on h so far (nullable). */
+int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches);
+
 /* Resident meshes (no counterpart in the reference): a mesh uploaded or
  * generated once, node-sorted once, kept in HBM and shared by any number of
  * batches on its device.  Ids are process-wide.  The Kuhn form is generated

@@ -62,6 +62,26 @@ def test_points_mode_bit_exact_full(rein, name):
     assert np.array_equal(out_u.view(np.uint64), ref_u["phi"].view(np.uint64))
 
 
+@pytest.mark.parametrize("name", ["cfg1", "ellipsoids"])
+def test_update_blend_on_first_and_grown_calls(name):
+    """The retained values of the update blend (LevelSet.cpp:443-447, 148) on
+    the first call of a fresh handle, and after its buffers were regrown by
+    a larger mesh, bitwise against the oracle."""
+    small = fa.kuhn_mesh(9)
+    m, phi, h = workload(name)
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=1)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, _ = r.reset_phi(phi, mode="points", blend="update")
+    assert np.array_equal(out.view(np.uint64), ref["phi"].view(np.uint64))
+    r.set_mesh(small)
+    r.reset_phi(fa.phi_field("star", small.xyz), mode="points", blend="update")
+    r.set_mesh(m)
+    out2, _, _ = r.reset_phi(phi, mode="points", blend="update")
+    assert np.array_equal(out2.view(np.uint64), ref["phi"].view(np.uint64))
+    r.close()
+
+
 @pytest.mark.parametrize("name", ["cfg2", "cfg3s"])
 def test_points_mode_bit_exact_sampled(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="points", want_nearest=True)
