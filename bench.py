@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 WORKLOADS = {
     "cfg1": dict(n=32, jitter=0.0, phi="sphere_wave", desc="32^3 Kuhn, sphere+wave"),
-    "cfg2": dict(n=128, jitter=0.0, phi="star", desc="128^3 Kuhn, star surface"),
+    "cfg2": dict(n=128, jitter=0.0, phi="star", band_h=6.0, desc="128^3 Kuhn, star surface"),
     "cfg3": dict(n=256, jitter=0.2, phi="spheres",
                  desc="256^3 Kuhn, interior nodes jittered +-0.2h, 64 spheres"),
 }
@@ -193,85 +193,158 @@ def run_batched(args):
 def run_serving(args):
     """Poisson stream of reinit jobs (sizes log-uniform 32^3..192^3, snapped to
     multiples of 8; phi from the star and 64-sphere generators) served with
-    continuous batching against serial plain calls, at each rate."""
+    continuous batching against serial plain calls, at each rate.
+
+    Each job's phi lives in pinned host memory (the simulation's own buffer)
+    and is reinitialised in place.  Serial plain: one reinit call per job, in
+    arrival order, on a handle holding that job's resident mesh, with the
+    same transfers."""
+    import ctypes as C
+    import torch
     import fluidtissue_amd as fa
     from fluidtissue_amd.serve import Server, latency_stats
+    from fluidtissue_amd._lib import check, lib
 
     rng = np.random.default_rng(11)
-    classes = list(range(32, 193, 8))
     raw = np.exp(rng.uniform(np.log(32), np.log(192), size=args.jobs))
     dims = [int(min(192, max(32, 8 * round(x / 8)))) for x in raw]
     gens = rng.integers(0, 2, size=args.jobs)
     t0 = time.perf_counter()
-    phis = {}
+    fields = {}
     for n in sorted(set(dims)):
         xyz = grid_xyz(n)
-        phis[(n, 0)] = fa.phi_field("star", xyz)
-        phis[(n, 1)] = fa.phi_field("spheres", xyz, seed=0)
-    log(f"serving inputs: {args.jobs} jobs over {len(set(dims))} sizes "
-        f"({time.perf_counter() - t0:.1f} s)")
-    results = {}
-    for rate in args.rates:
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.jobs))
-        res = {}
-        for label, kw in (("served", dict(lanes=2, max_fields=64)),
-                          ("serial_plain", dict(lanes=1, max_fields=1))):
-            with Server(device=0, mode=args.mode, budget_bytes=200e9, **kw) as srv:
-                # warm the mesh cache and the kernels (untimed)
-                warm = [srv.submit((n, n, n), phis[(n, 0)]) for n in sorted(set(dims))]
-                for j in warm:
-                    j.wait(600)
-                srv.batches.clear()
-                start = time.perf_counter()
-                jobs = []
-                for k in range(args.jobs):
-                    ta = start + arrivals[k]
-                    now = time.perf_counter()
-                    if ta > now:
-                        time.sleep(ta - now)
-                    n = dims[k]
-                    jobs.append(srv.submit((n, n, n), phis[(n, int(gens[k]))], t_arrival=ta))
-                for j in jobs:
-                    j.wait(1200)
-                end = max(j.t_done for j in jobs)
-                nodes = sum(j.n for j in jobs)
-                st = latency_stats(jobs)
-                st.update(jobs_per_s=args.jobs / (end - start), Mnodes_per_s=nodes / (end - start) / 1e6,
-                          mean_batch=float(np.mean([b[0] for b in srv.batches])))
-                res[label] = st
-                log(f"rate {rate}/s {label}: {st}")
-        res["throughput_ratio"] = res["served"]["Mnodes_per_s"] / res["serial_plain"]["Mnodes_per_s"]
-        results[str(rate)] = res
-    # saturated capacity: the whole stream queued at once
-    sat = {}
-    for label, kw in (("served", dict(lanes=2, max_fields=64)),
-                      ("serial_plain", dict(lanes=1, max_fields=1))):
-        with Server(device=0, mode=args.mode, budget_bytes=200e9, **kw) as srv:
-            warm = [srv.submit((n, n, n), phis[(n, 0)]) for n in sorted(set(dims))]
-            for j in warm:
-                j.wait(600)
-            start = time.perf_counter()
+        fields[(n, 0)] = torch.from_numpy(fa.phi_field("star", xyz))
+        fields[(n, 1)] = torch.from_numpy(fa.phi_field("spheres", xyz, seed=0))
+    sizes = [n ** 3 for n in dims]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    store = torch.empty(int(offs[-1]), dtype=torch.float64, pin_memory=True)
+    bufs = [store[int(offs[k]):int(offs[k + 1])] for k in range(args.jobs)]
+
+    def refill():
+        for k in range(args.jobs):
+            bufs[k].copy_(fields[(dims[k], int(gens[k]))])
+
+    refill()
+    log(f"serving inputs: {args.jobs} jobs over {len(set(dims))} sizes, "
+        f"{store.numel() * 8 / 1e9:.1f} GB pinned ({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda:0")
+    kw = dict(mode=args.mode, blend="init")
+
+    # serial plain: a handle per size with its resident mesh, a device buffer per size
+    singles = {}
+    for n in sorted(set(dims)):
+        mid = C.c_int32()
+        check(lib.ft_mesh_create_kuhn(0, n, n, n, 0.0, 0, C.byref(mid)))
+        h = fa.Reinit(0)
+        h.set_batch_meshes([mid.value])
+        singles[n] = (h, mid.value, torch.empty(n ** 3, dtype=torch.float64, device=dev))
+    sstream = torch.cuda.Stream(device=dev)
+
+    def serial_job(k):
+        h, _, d = singles[dims[k]]
+        with torch.cuda.stream(sstream):
+            d.copy_(bufs[k], non_blocking=True)
+        h.reset_phi_batch_device(d.data_ptr(), None, stream=sstream.cuda_stream, **kw)
+        with torch.cuda.stream(sstream):
+            bufs[k].copy_(d, non_blocking=True)
+        sstream.synchronize()
+
+    for n in sorted(set(dims)):                      # warm the kernels (untimed)
+        serial_job(dims.index(n))
+    call_ms = {}
+    for n in sorted(set(dims)):                      # one plain call per size, median of 5
+        k = dims.index(n)
+        ts = []
+        for _ in range(5):
+            t1 = time.perf_counter()
+            serial_job(k)
+            ts.append((time.perf_counter() - t1) * 1e3)
+        call_ms[n] = float(np.median(ts))
+    log("serial plain call ms by size: " + ", ".join(f"{n}^3 {t:.2f}" for n, t in call_ms.items()))
+    refill()
+
+    def run_serial(arrivals):
+        start = time.perf_counter()
+        lat = []
+        t_end = start
+        for k in range(args.jobs):
+            ta = start + arrivals[k]
+            now = time.perf_counter()
+            if ta > now:
+                time.sleep(ta - now)
+            serial_job(k)
+            t_end = time.perf_counter()
+            lat.append(t_end - ta)
+        lat = np.array(lat)
+        return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                    mean_ms=float(lat.mean() * 1e3), jobs_per_s=args.jobs / (t_end - start),
+                    Mnodes_per_s=sum(sizes) / (t_end - start) / 1e6, mean_batch=1.0)
+
+    def run_served(srv, arrivals):
+        srv.batches.clear()
+        start = time.perf_counter()
+        jobs = []
+        if arrivals is None:                          # saturated: the whole stream at once
             with srv.cv:
-                jobs = [srv.submit((n, n, n), phis[(n, int(g))], t_arrival=start)
-                        for n, g in zip(dims, gens)]
-            for j in jobs:
-                j.wait(1200)
-            end = max(j.t_done for j in jobs)
-            sat[label] = dict(seconds=end - start, jobs_per_s=args.jobs / (end - start),
-                              Mnodes_per_s=sum(j.n for j in jobs) / (end - start) / 1e6)
-            log(f"saturated {label}: {sat[label]}")
-    sat["throughput_ratio"] = sat["served"]["Mnodes_per_s"] / sat["serial_plain"]["Mnodes_per_s"]
-    results["saturated"] = sat
+                jobs = [srv.submit((n, n, n), bufs[k], t_arrival=start, inplace=True)
+                        for k, n in enumerate(dims)]
+        else:
+            for k in range(args.jobs):
+                ta = start + arrivals[k]
+                now = time.perf_counter()
+                if ta > now:
+                    time.sleep(ta - now)
+                n = dims[k]
+                jobs.append(srv.submit((n, n, n), bufs[k], t_arrival=ta, inplace=True))
+        for j in jobs:
+            j.wait(1200)
+        end = max(j.t_done for j in jobs)
+        st = latency_stats(jobs)
+        st.update(jobs_per_s=args.jobs / (end - start), Mnodes_per_s=sum(sizes) / (end - start) / 1e6,
+                  mean_batch=float(np.mean([b[0] for b in srv.batches])))
+        return st
+
+    results = {}
+    with Server(device=0, lanes=2, max_fields=64, budget_bytes=200e9, want_ifn=False, **kw) as srv:
+        warm = [srv.submit((n, n, n), fields[(n, 0)].clone()) for n in sorted(set(dims))]
+        for j in warm:
+            j.wait(600)
+        for rate in args.rates:
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.jobs))
+            res = {}
+            refill()
+            res["served"] = run_served(srv, arrivals)
+            log(f"rate {rate}/s served: {res['served']}")
+            refill()
+            res["serial_plain"] = run_serial(arrivals)
+            log(f"rate {rate}/s serial_plain: {res['serial_plain']}")
+            res["throughput_ratio"] = res["served"]["Mnodes_per_s"] / res["serial_plain"]["Mnodes_per_s"]
+            results[str(rate)] = res
+        # saturated capacity: the whole stream queued at once
+        sat = {}
+        refill()
+        sat["served"] = run_served(srv, None)
+        log(f"saturated served: {sat['served']}")
+        refill()
+        sat["serial_plain"] = run_serial(np.zeros(args.jobs))
+        log(f"saturated serial_plain: {sat['serial_plain']}")
+        sat["throughput_ratio"] = sat["served"]["Mnodes_per_s"] / sat["serial_plain"]["Mnodes_per_s"]
+        results["saturated"] = sat
+    for h, mid, _ in singles.values():
+        h.close()
+        lib.ft_mesh_release(mid)
     return {
         "metric": "served reinit p50/p99 latency and throughput vs serial plain (Poisson stream)",
-        "value": results[str(args.rates[-1])]["served"]["Mnodes_per_s"], "unit": "Mnodes/s",
+        "value": results["saturated"]["served"]["Mnodes_per_s"], "unit": "Mnodes/s",
         "n_gpus": 1, "steps": args.jobs, "warmup": 0, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 distance search; f64 phi and crossing points",
         "data": "synthetic",
         "config": {"workload": f"serving: {args.jobs} jobs, sizes log-uniform 32^3-192^3 (multiples "
-                               f"of 8), star/64-sphere phi, {args.mode} distance",
+                               f"of 8), star/64-sphere phi, {args.mode} distance, phi in pinned host "
+                               f"memory reinitialised in place",
                    "rates_per_s": args.rates},
         "rates": results,
+        "serial_call_ms_by_size": {str(n): t for n, t in call_ms.items()},
     }
 
 
@@ -309,6 +382,7 @@ def main():
     m = fa.kuhn_mesh(wl["n"], jitter=wl["jitter"], seed=1)
     phi = fa.phi_field(wl["phi"], m.xyz, seed=0)
     h = 1.0 / (wl["n"] - 1)
+    band = wl.get("band_h", 0.0) * h
     log(f"inputs: {m.n_nodes} nodes {len(m.tets)} tets {len(m.edges)} edges "
         f"({time.perf_counter() - t0:.1f} s)")
 
@@ -322,7 +396,7 @@ def main():
     def step():
         d_phi.copy_(d_phi0)
         return r.reset_phi_device(d_phi.data_ptr(), None, mode=args.mode, blend="init",
-                                  bin_scale=args.bin_scale, stream=stream,
+                                  band=band, bin_scale=args.bin_scale, stream=stream,
                                   query_block=args.query_block)
 
     tuned = None
@@ -333,7 +407,7 @@ def main():
             d_phi.copy_(d_phi0)
         bs, qb, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
                                       mode=args.mode, reps=1, sync=torch.cuda.synchronize, copy=copy,
-                                      bin_scales=(1.0, 1.5, 2.0), query_blocks=(64, 256))
+                                      bin_scales=(1.0, 1.5, 2.0), query_blocks=(64, 256), band=band)
         args.bin_scale, args.query_block = bs, qb
         tuned = {"bin_scale": bs, "query_block": qb,
                  "table_ms": [[a, q, round(t, 3)] for a, q, t in table]}
@@ -389,9 +463,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 distance search; f64 phi, crossing points and thin triangles",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {wl['desc']}, {args.mode} distance, full domain",
+        "config": {"workload": f"{args.workload}: {wl['desc']}, {args.mode} distance, "
+                               + (f"narrow band {wl['band_h']:g}h" if band > 0 else "full domain"),
                    "nodes": N, "tets": T, "edges": len(m.edges), "crossing_points": P,
-                   "triangles": F, "crossing_tets": X, "h": h},
+                   "triangles": F, "crossing_tets": X, "h": h, "band": band,
+                   "clamped_nodes": cnt.n_clamped},
         "phases_ms": {"marching_tets": ph[0], "index_build": ph[1], "query": ph[2]},
         "roofline": {"kernel": "k_query (nearest-triangle search + epilogue)", "bound": "hbm",
                      "achieved": achieved, "peak": peak, "unit": "GB/s",

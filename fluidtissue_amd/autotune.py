@@ -18,12 +18,12 @@ _cache = {}
 
 
 def tune(handle, phi_ptr, phi_backup_ptr, nbytes, mode="triangles", reps=2, sync=None, copy=None,
-         signature=None, bin_scales=BIN_SCALES, query_blocks=QUERY_BLOCKS, log=None):
+         signature=None, bin_scales=BIN_SCALES, query_blocks=QUERY_BLOCKS, log=None, band=0.0):
     """handle: a Reinit with its batch set; phi_ptr: device phi (overwritten);
     phi_backup_ptr: device copy of the input, restored before every run by
     copy(dst, src, nbytes); sync(): waits for the device.  Returns
     (bin_scale, query_block, table of (bin_scale, query_block, ms))."""
-    key = (mode, signature)
+    key = (mode, band, signature)
     if signature is not None and key in _cache:
         return _cache[key]
     table = []
@@ -35,7 +35,8 @@ def tune(handle, phi_ptr, phi_backup_ptr, nbytes, mode="triangles", reps=2, sync
                 copy(phi_ptr, phi_backup_ptr, nbytes)
                 sync()
                 t0 = time.perf_counter()
-                handle.reset_phi_batch_device(phi_ptr, None, mode=mode, bin_scale=bs, query_block=qb)
+                handle.reset_phi_batch_device(phi_ptr, None, mode=mode, band=band, bin_scale=bs,
+                                              query_block=qb)
                 sync()
                 ts.append(time.perf_counter() - t0)
             ms = float(np.median(ts[1:]) * 1e3)

@@ -162,6 +162,13 @@ struct Bound {
     float pad[2];
 };
 
+// Square root for lower bounds only: the hardware v_sqrt_f32 (within 1 ulp),
+// not the correctly rounded sequence the build flags give sqrtf.  Every bound
+// that uses it subtracts a relative slack of 2^-20 or more afterwards, which
+// covers the 2^-23 relative error; a flushed tiny input gives 0, still a
+// lower bound.
+__device__ __forceinline__ float sqrt_lb(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ float cyl_dist2(float x, float y, float z, const Bound& b)
 {
     if (b.tau < 0.0f) return 0.0f;
@@ -170,7 +177,7 @@ __device__ __forceinline__ float cyl_dist2(float x, float y, float z, const Boun
     const float cx = __fmaf_rn(b.n[1], dz, -b.n[2] * dy);
     const float cy = __fmaf_rn(b.n[2], dx, -b.n[0] * dz);
     const float cz = __fmaf_rn(b.n[0], dy, -b.n[1] * dx);
-    const float lat = sqrtf(__fmaf_rn(cz, cz, __fmaf_rn(cy, cy, cx * cx)));
+    const float lat = sqrt_lb(__fmaf_rn(cz, cz, __fmaf_rn(cy, cy, cx * cx)));
     const float slack = 0x1.0p-20f * (fabsf(hn) + lat);     // rounding of hn and lat
     const float a = fmaxf(fabsf(hn) - b.tau - slack, 0.0f);
     const float r = fmaxf(lat - b.rho - slack, 0.0f);
@@ -229,7 +236,7 @@ __device__ __forceinline__ TriProj tri_project(float x, float y, float z, const 
 __device__ __forceinline__ float tri_disk_lb2(const TriProj& p, const TriRec& r)
 {
     const float rho = r.F0.w * (1.0f + 0x1.0p-18f);
-    const float lat = sqrtf(__fmaf_rn(p.Y, p.Y, p.X * p.X));
+    const float lat = sqrt_lb(__fmaf_rn(p.Y, p.Y, p.X * p.X));
     const float g = fmaxf(lat - rho, 0.0f) * (1.0f - 0x1.0p-18f);
     return __fmaf_rn(g, g, p.Z * p.Z) * (1.0f - 0x1.0p-18f);
 }

@@ -7,7 +7,15 @@ worker lane owns a handle and a HIP stream; whenever a lane is free it admits
 the queued jobs that fit (by count, nodes and memory budget) as one batch and
 reinitialises all of them in one pass (ft_reinit_batch).  Jobs that arrive
 while batches are on the GPU wait for the next free lane, so two lanes keep
-two batches in flight and their work overlaps on the device.
+two batches in flight and their work overlaps on the device: one lane's
+transfers run on the copy engines while the other lane's kernels run.
+
+A job's phi is a numpy array or a torch tensor.  Host phi is copied straight
+into the lane's device buffer at the job's offset (asynchronously when it is
+in pinned memory) and the result comes back into pinned host memory; phi
+already on the device is copied device to device and the result stays there.
+With inplace=True the result is written back into the job's own phi (as
+LevelSet::resetPhi_ updates phi_ in place, src/fem/LevelSet.cpp:448).
 
 A batch gives every job exactly the result the job would get alone (the
 batched kernels keep each field's data apart); tests check it bit for bit.
@@ -95,13 +103,25 @@ class MeshCache:
 
 class Job:
     __slots__ = ("dims", "phi", "t_arrival", "t_start", "t_done", "out", "ifn", "counts",
-                 "error", "done", "n")
+                 "error", "done", "n", "on_device", "inplace", "numpy_in")
 
-    def __init__(self, dims, phi, t_arrival=None):
+    def __init__(self, dims, phi, t_arrival=None, inplace=False):
+        import torch
         self.dims = tuple(int(d) for d in dims)
         self.n = self.dims[0] * self.dims[1] * self.dims[2]
-        self.phi = np.ascontiguousarray(phi, np.float64)
-        if self.phi.shape != (self.n,):
+        self.inplace = bool(inplace)
+        self.numpy_in = not isinstance(phi, torch.Tensor)
+        if self.inplace and self.numpy_in and not (isinstance(phi, np.ndarray) and phi.dtype == np.float64
+                                                   and phi.flags.c_contiguous and phi.flags.writeable):
+            raise ValueError("inplace needs a writeable contiguous float64 array")
+        if isinstance(phi, torch.Tensor):
+            if phi.dtype != torch.float64 or not phi.is_contiguous():
+                raise ValueError("phi must be a contiguous float64 tensor")
+            self.phi = phi
+        else:
+            self.phi = torch.from_numpy(np.ascontiguousarray(phi, np.float64))
+        self.on_device = self.phi.is_cuda
+        if tuple(self.phi.shape) != (self.n,):
             raise ValueError("phi does not match dims")
         self.t_arrival = time.perf_counter() if t_arrival is None else t_arrival
         self.t_start = self.t_done = None
@@ -124,7 +144,10 @@ class Server:
     """Continuous-batching reinit server (see module docstring)."""
 
     def __init__(self, device=0, lanes=2, budget_bytes=200e9, max_fields=64, max_nodes=64_000_000,
-                 mode="triangles", blend="init", band=0.0, bin_scale=0.0, query_block=0):
+                 mode="triangles", blend="init", band=0.0, bin_scale=0.0, query_block=0,
+                 want_ifn=True):
+        self.device = device
+        self.want_ifn = bool(want_ifn)
         self.cache = MeshCache(device, budget_bytes)
         self.budget = float(budget_bytes)
         self.max_fields = int(max_fields)
@@ -142,8 +165,8 @@ class Server:
         for t in self.threads:
             t.start()
 
-    def submit(self, dims, phi, t_arrival=None):
-        job = Job(dims, phi, t_arrival)
+    def submit(self, dims, phi, t_arrival=None, inplace=False):
+        job = Job(dims, phi, t_arrival, inplace)
         with self.cv:                  # re-entrant: a caller may hold cv to submit a burst
             self.queue.append(job)
             self.cv.notify()
@@ -167,6 +190,10 @@ class Server:
         return jobs, extra
 
     def _lane(self, h):
+        import torch
+        dev = torch.device("cuda", self.device)
+        stream = torch.cuda.Stream(device=dev)      # default priority
+        d_phi = d_ifn = None
         while True:
             with self.cv:
                 while not self.queue and not self.stop:
@@ -184,18 +211,47 @@ class Server:
                     j.t_start = t0
                     ids.append(self.cache.acquire(j.dims))
                 h.set_batch_meshes(ids)
-                phi = np.concatenate([j.phi for j in jobs])
-                out, ifn, _ = h.reset_phi(phi, **self.kw)
-                counts = h.field_counts()
-                off = 0
+                total = sum(j.n for j in jobs)
+                with torch.cuda.stream(stream):
+                    if d_phi is None or d_phi.numel() < total:
+                        d_phi = d_ifn = None
+                        d_phi = torch.empty(total, dtype=torch.float64, device=dev)
+                        if self.want_ifn:
+                            d_ifn = torch.empty_like(d_phi)
+                    off = 0
+                    for j in jobs:
+                        d_phi[off:off + j.n].copy_(j.phi, non_blocking=True)
+                        off += j.n
+                counts = h.reset_phi_batch_device(
+                    d_phi.data_ptr(), d_ifn.data_ptr() if d_ifn is not None else None,
+                    stream=stream.cuda_stream, **self.kw)
+                with torch.cuda.stream(stream):
+                    off = 0
+                    for j in jobs:
+                        for name, src in (("out", d_phi), ("ifn", d_ifn)):
+                            if src is None:
+                                continue
+                            part = src[off:off + j.n]
+                            if name == "out" and j.inplace:
+                                dst = j.phi
+                                dst.copy_(part, non_blocking=True)
+                            elif j.on_device:
+                                dst = part.clone()
+                            else:
+                                dst = torch.empty(j.n, dtype=torch.float64, pin_memory=True)
+                                dst.copy_(part, non_blocking=True)
+                            setattr(j, name, dst)
+                        off += j.n
+                stream.synchronize()
                 t1 = time.perf_counter()
                 for j, c in zip(jobs, counts):
-                    j.out = out[off:off + j.n]
-                    j.ifn = ifn[off:off + j.n]
+                    if j.numpy_in:
+                        j.out = j.out.numpy()
+                        if j.ifn is not None:
+                            j.ifn = j.ifn.numpy()
                     j.counts = c
                     j.t_done = t1
-                    off += j.n
-                self.batches.append((len(jobs), off, t1 - t0))
+                self.batches.append((len(jobs), total, t1 - t0))
             except Exception as e:        # reported on every job of the batch
                 for j in jobs:
                     j.error = e

@@ -120,6 +120,44 @@ def test_served_results_equal_alone():
         a.close()
 
 
+def test_served_inplace_pinned_and_device_inputs():
+    """phi given as a numpy array, a pinned host tensor or a device tensor,
+    reinitialised in place or into new storage: all equal the solo result."""
+    import torch
+    from fluidtissue_amd.serve import Server
+    sizes = [14, 22, 18, 14, 26, 20]
+    inputs = []
+    for k, n in enumerate(sizes):
+        m = fa.kuhn_mesh(n)
+        inputs.append((n, m, fa.phi_field("spheres" if k % 2 else "star", m.xyz, seed=k)))
+    with Server(device=0, lanes=2, max_fields=4, mode="triangles", blend="update",
+                want_ifn=False) as srv:
+        jobs = []
+        with srv.cv:
+            for k, (n, m, phi) in enumerate(inputs):
+                kind = k % 3
+                if kind == 0:
+                    src, inplace = phi.copy(), True
+                elif kind == 1:
+                    src, inplace = torch.from_numpy(phi).pin_memory(), True
+                else:
+                    src, inplace = torch.from_numpy(phi).cuda(), False
+                jobs.append((m, phi, src, inplace, srv.submit((n, n, n), src, inplace=inplace)))
+        for *_, j in jobs:
+            j.wait(120)
+    for m, phi, src, inplace, j in jobs:
+        a = fa.Reinit(0)
+        a.set_mesh(m)
+        o1, _, _ = a.reset_phi(phi, mode="triangles", blend="update")
+        a.close()
+        got = j.out.cpu().numpy() if isinstance(j.out, torch.Tensor) else j.out
+        assert np.array_equal(np.asarray(got).view(np.uint64), o1.view(np.uint64))
+        assert j.ifn is None
+        if inplace:
+            back = src.numpy() if isinstance(src, torch.Tensor) else src
+            assert np.array_equal(back.view(np.uint64), o1.view(np.uint64))
+
+
 def test_resident_mesh_batch_matches_set_mesh():
     ids = []
     import ctypes as C
