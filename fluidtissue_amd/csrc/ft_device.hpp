@@ -206,9 +206,11 @@ __device__ inline void cross3d(const double* u, const double* v, double* w)
 // the index is built, with u along ab (the longest edge) and n the unit
 // normal; in it the triangle is (0,0), (B,0), (Cx,Cy), Cy >= 0:
 //   F0 = (a, B)  F1 = (u, Cx)  F2 = (v, Cy)  F3 = (n, 1/B)
-//   F4 = (1/|ac|^2, 1/|bc|^2, Cx - B, interior)
-// (inverse lengths are 0 for zero-length edges; interior = 0 for collinear
-// vertices, whose distance is that of the edge segments).  The point is
+//   F4 = (1/|ac|^2, 1/|bc|^2, Cx - B, +-R)
+// (inverse lengths are 0 for zero-length edges; R is the radius, rounded up,
+// of the disk about the midpoint of ab that holds the triangle, negative for
+// collinear vertices, which have no interior and whose distance is that of
+// the edge segments).  The point is
 // projected once (X, Y, Z); the in-plane distance is 0 inside the triangle,
 // else the nearest clamped edge projection; d^2 = d2D^2 + Z^2.  With ab the
 // longest edge the 2D problem stays well conditioned however thin the
@@ -231,12 +233,13 @@ __device__ __forceinline__ TriProj tri_project(float x, float y, float z, const 
     return p;
 }
 
-// Lower bound from the disk about vertex a that holds the triangle (radius
-// |ab|, the longest edge); costs little once the point is projected.
+// Lower bound from the disk about the midpoint of ab that holds the triangle
+// (radius |F4.w|); costs little once the point is projected.
 __device__ __forceinline__ float tri_disk_lb2(const TriProj& p, const TriRec& r)
 {
-    const float rho = r.F0.w * (1.0f + 0x1.0p-18f);
-    const float lat = sqrt_lb(__fmaf_rn(p.Y, p.Y, p.X * p.X));
+    const float rho = fabsf(r.F4.w) * (1.0f + 0x1.0p-18f);
+    const float mx = __fmaf_rn(-0.5f, r.F0.w, p.X);
+    const float lat = sqrt_lb(__fmaf_rn(p.Y, p.Y, mx * mx));
     const float g = fmaxf(lat - rho, 0.0f) * (1.0f - 0x1.0p-18f);
     return __fmaf_rn(g, g, p.Z * p.Z) * (1.0f - 0x1.0p-18f);
 }

@@ -692,8 +692,11 @@ __global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, const uint64_t* 
     r.F1 = make_float4((float)u[0], (float)u[1], (float)u[2], (float)Cx);
     r.F2 = make_float4((float)v[0], (float)v[1], (float)v[2], (float)Cy);
     r.F3 = make_float4((float)nu[0], (float)nu[1], (float)nu[2], L0 > 0.0 ? (float)(1.0 / L0) : 0.0f);
+    // disk about the midpoint of ab holding the triangle (ab is the longest
+    // edge), radius rounded up; its sign carries the interior flag
+    const double Rm = fmax(0.5 * L0, sqrt((Cx - 0.5 * L0) * (Cx - 0.5 * L0) + Cy * Cy)) * (1.0 + 1e-6);
     r.F4 = make_float4(lac > 0.0 ? (float)(1.0 / lac) : 0.0f, lbc > 0.0 ? (float)(1.0 / lbc) : 0.0f,
-                       (float)dCx, interior ? 1.0f : 0.0f);
+                       (float)dCx, interior ? (float)Rm : -(float)Rm);
     trec[slot] = r;
 }
 
@@ -925,7 +928,28 @@ struct QueryArgs {
     float h_est;                     // node spacing (statistics only)
     int probe;                       // greedy first candidate
     const uint64_t* sc;              // stage totals and halt flags
+    unsigned xcd_chunk;              // tiles per XCD chunk (0: dispatch order)
 };
+
+// The dispatcher hands workgroup b to XCD b % 8, each XCD with its own L2.
+// Remapped, XCD x processes runs of `chunk` consecutive tiles (in Morton
+// order, a compact region whose triangle and tree records its L2 keeps)
+// instead of every 8th tile; the runs are dealt to the XCDs in turn, so costly
+// regions still spread over all of them (one contiguous run per XCD was
+// slower: the XCD holding the costliest region finished last).  A bijection
+// on [0, nb).
+constexpr unsigned kXcds = 8;
+constexpr unsigned kXcdChunk = 256;     // measured: 0 -> 22.9 ms, 64/256 -> 22.2 ms at 256^3
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned chunk)
+{
+    if (chunk == 0) return b;
+    // chunks of `chunk` consecutive tiles, dealt to the XCDs in turn
+    const unsigned span = kXcds * chunk;
+    const unsigned full = nb - nb % span;          // blocks in whole spans
+    if (b >= full) return b;                       // the ragged tail keeps its order
+    const unsigned x = b % kXcds, i = b / kXcds;
+    return (i / chunk) * span + x * chunk + i % chunk;
+}
 
 // Margin of the box prune: a box is skipped only if its lower bound exceeds
 // the best distance by more than the float rounding of both (relative) plus
@@ -1035,7 +1059,7 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
     __shared__ int32_t wstack[BS / 64][kStack];
     if (halted(a.sc)) return;                         // an earlier stage failed: phi untouched
     const int w = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * BS + (threadIdx.x & ~63);
+    const int64_t base = (int64_t)xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk) * BS + (threadIdx.x & ~63);
     if (base >= a.n_slots) return;                    // whole wave out of range
     const int64_t slot = base + (threadIdx.x & 63);
     const int32_t node = a.perm[slot];                 // -1: padding lane
@@ -1908,6 +1932,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     q.overflow = (unsigned*)(sc + S_OVF);
     q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     q.sc = sc;
+    q.xcd_chunk = kXcdChunk;
     if (timed) (void)hipEventRecord(c->ev[2], s);
     if (c->n_slots > 0) {
         const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
