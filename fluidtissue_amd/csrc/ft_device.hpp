@@ -201,6 +201,51 @@ __device__ inline void cross3d(const double* u, const double* v, double* w)
     w[2] = u[0] * v[1] - u[1] * v[0];
 }
 
+// ---- FP64 point-triangle distance (the search's final word) -----------------
+// Squared distance from x to triangle abc: the plane distance when the
+// projection of x falls inside (edge functions n.(e x (x - v)) >= 0 for the
+// three edges), else the nearest of the three edge segments; nearly
+// collinear vertices (|n|^2 <= 1e-14 lmax^2) use the segments only.
+__device__ inline double seg_dist2_f64(const double* x, const double* a, const double* b)
+{
+    double ab[3], ax[3];
+    for (int k = 0; k < 3; k++) { ab[k] = b[k] - a[k]; ax[k] = x[k] - a[k]; }
+    const double l2 = dot3d(ab, ab);
+    double t = l2 > 0.0 ? dot3d(ax, ab) / l2 : 0.0;
+    t = fmin(fmax(t, 0.0), 1.0);
+    double r = 0.0;
+    for (int k = 0; k < 3; k++) { const double q = ax[k] - t * ab[k]; r += q * q; }
+    return r;
+}
+
+__device__ inline double tri_dist2_f64(const double* x, const double* a, const double* b,
+                                       const double* c)
+{
+    double e0[3], e1[3], e2[3], n[3];
+    for (int k = 0; k < 3; k++) { e0[k] = b[k] - a[k]; e1[k] = c[k] - b[k]; e2[k] = a[k] - c[k]; }
+    cross3d(e0, e1, n);
+    const double nn = dot3d(n, n);
+    const double lmax = fmax(dot3d(e0, e0), fmax(dot3d(e1, e1), dot3d(e2, e2)));
+    if (nn > 1e-14 * lmax * lmax) {
+        const double* v[3] = {a, b, c};
+        const double* e[3] = {e0, e1, e2};
+        bool inside = true;
+        for (int q = 0; q < 3; q++) {
+            double xv[3], t[3];
+            for (int k = 0; k < 3; k++) xv[k] = x[k] - v[q][k];
+            cross3d(e[q], xv, t);
+            inside = inside && dot3d(n, t) >= 0.0;
+        }
+        if (inside) {
+            double xa[3];
+            for (int k = 0; k < 3; k++) xa[k] = x[k] - a[k];
+            const double hgt = dot3d(n, xa);
+            return hgt * hgt / nn;
+        }
+    }
+    return fmin(seg_dist2_f64(x, a, b), fmin(seg_dist2_f64(x, b, c), seg_dist2_f64(x, c, a)));
+}
+
 // ---- FP32 point-triangle distance -------------------------------------------
 // Triangle record (80 B), an orthonormal frame at vertex a, built in FP64 when
 // the index is built, with u along ab (the longest edge) and n the unit

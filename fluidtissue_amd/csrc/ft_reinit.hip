@@ -929,6 +929,7 @@ struct QueryArgs {
     int probe;                       // greedy first candidate
     const uint64_t* sc;              // stage totals and halt flags
     unsigned xcd_chunk;              // tiles per XCD chunk (0: dispatch order)
+    int2* near2;                     // mode 1: (nearest, close runner-up or -1) per node
 };
 
 // The dispatcher hands workgroup b to XCD b % 8, each XCD with its own L2.
@@ -974,16 +975,32 @@ struct Searcher {
 
     __device__ __forceinline__ float best2() const { return MODE == 0 ? best * best : best; }
 
+    // Mode 0 keeps the nearest point (ties: lowest id, the reference's strict
+    // < over ascending ids).  Mode 1 keeps the best two triangles by FP32
+    // distance, so that the epilogue can settle near-ties in FP64.
     __device__ __forceinline__ void consider(float d, uint32_t s)
     {
-        if (d < best || (d == best && d < INFINITY)) {
+        if (MODE == 0) {
+            if (d < best || (d == best && d < INFINITY)) {
+                const int32_t j = pidx[s];
+                if (d < best || bj < 0 || j < bj) { best = d; bj = j; }
+            }
+        } else if (d < sec || (d == sec && d < INFINITY)) {
             const int32_t j = pidx[s];
-            if (d < best || bj < 0 || j < bj) { best = d; bj = j; bslot = (int32_t)s; }
+            if (d < best || (d == best && (bj < 0 || j < bj))) {
+                if (bj >= 0 && j != bj) { sec = best; sj = bj; }
+                best = d;
+                bj = j;
+            } else if (j != bj && (d < sec || sj < 0 || j < sj)) {
+                sec = d;
+                sj = j;
+            }
         }
     }
 
     uint32_t n_leaves = 0, n_prims = 0;
-    int32_t bslot = -1;
+    float sec;       // mode 1: second best squared distance and its triangle
+    int32_t sj = -1;
 
     // Leaf test at a wave-uniform leaf: records read through scalar loads.
     __device__ void test_leaf_uniform(int l)
@@ -1012,23 +1029,73 @@ struct Searcher {
 
 // Epilogue of one node (EuclideanDistance.cu:54-57, LevelSet.cpp:438-448, 74,
 // 148): sign from phi_old < 0, retained value at crossing-edge nodes, blend.
-template <int MODE>
-__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t node)
+__device__ __forceinline__ void node_epilogue(const QueryArgs& a, int64_t node, double d, int32_t near)
 {
-    float d;
-    if (MODE == 0) d = S.best;
-    else d = S.bj >= 0 ? sqrtf(S.best) : (a.band > 0.0f ? a.band : FLT_MAX);
-    if (a.band > 0.0f && S.bj < 0) d = a.band;
     const double po = a.phi_old[node];
-    const double dn = (po < 0.0) ? -(double)d : (double)d;
+    const double dn = (po < 0.0) ? -d : d;
     if (a.phi_ifn) a.phi_ifn[node] = dn;
-    if (a.nearest) a.nearest[node] = S.bj;
+    if (a.nearest) a.nearest[node] = near;
     double out = dn;
     if (a.epi == EPI_UPDATE) {
         const double kept = (a.retain && ftd::get_bit(a.retain, node)) ? po : dn;
         out = a.w_phi * kept + a.w_ifn * dn;
     }
     a.phi_out[node] = out;
+}
+
+// End of the search of one node.  Points: the float distance, as the
+// reference, and the epilogue here.  Triangles: the FP32 search knows the
+// nearest triangle up to its rounding; the best and, when that close, the
+// runner-up go to k_settle, which evaluates them in FP64 (at 256^3 float
+// rounding alone nears 1e-5 h) and runs the epilogue.  Keeping the FP64 work
+// out of this kernel keeps its register count, hence its occupancy.
+template <int MODE>
+__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t node)
+{
+    if (MODE == 0) {
+        const float d = (a.band > 0.0f && S.bj < 0) ? a.band : S.best;
+        node_epilogue(a, node, (double)d, S.bj);
+    } else {
+        const bool close = S.sj >= 0 && S.sec <= S.best * (1.0f + 0x1.0p-16f);
+        a.near2[node] = make_int2(S.bj, close ? S.sj : -1);
+    }
+}
+
+// FP64 squared distance from point (x, y, z) to triangle f of the interface.
+__device__ inline double node_tri_dist2(const QueryArgs& a, float x, float y, float z, int32_t f)
+{
+    const int4 t = a.tris[f];
+    double p[3] = {x, y, z}, va[3], vb[3], vc[3];
+    for (int k = 0; k < 3; k++) {
+        va[k] = a.ifp[3 * (int64_t)t.x + k];
+        vb[k] = a.ifp[3 * (int64_t)t.y + k];
+        vc[k] = a.ifp[3 * (int64_t)t.z + k];
+    }
+    return ftd::tri_dist2_f64(p, va, vb, vc);
+}
+
+// Triangles: per node (node order), the FP64 distance to the nearest
+// triangle (the lower of the two candidates, ties to the lower id), then
+// the epilogue.  No candidate: +-band in a narrow band, else +-FLT_MAX.
+__global__ __launch_bounds__(kBS) void k_settle(QueryArgs a, const float* xyz, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= n || halted(a.sc)) return;
+    const int2 c = a.near2[i];
+    double d;
+    int32_t near = c.x;
+    if (c.x >= 0) {
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        double d2 = node_tri_dist2(a, x, y, z, c.x);
+        if (c.y >= 0) {
+            const double e2 = node_tri_dist2(a, x, y, z, c.y);
+            if (e2 < d2 || (e2 == d2 && c.y < near)) { d2 = e2; near = c.y; }
+        }
+        d = sqrt(d2);
+    } else {
+        d = (double)(a.band > 0.0f ? a.band : FLT_MAX);
+    }
+    node_epilogue(a, i, d, near);
 }
 
 template <int MODE>
@@ -1071,6 +1138,7 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
     S.bj = -1;
     if (MODE == 0) S.best = a.band > 0.0f ? a.band : FLT_MAX;
     else S.best = a.band > 0.0f ? a.band * a.band : INFINITY;
+    S.sec = S.best;
     S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
     S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
     S.ifp = a.ifp; S.tris = a.tris;
@@ -1357,7 +1425,7 @@ struct ft_ctx {
     DevBuf phi_tmp, phi_ifn_tmp;
     DevBuf sbits, retain, tbits, blk, xtet, xtri_off, xedges, ebits, eprefix;
     DevBuf ifp, ifp2e, tris, tri_norm, if_elems, p2t_off, p2t_cursor, p2t, ifp_area;
-    DevBuf nearest, fcounts, fclamp;
+    DevBuf nearest, near2, fcounts, fclamp;
     // index
     DevBuf bin_count, bin_prefix, pcode, prec, trec, pidx, leaf_key, leaf_start, leaf_cnt;
     DevBuf leaf_b, leaf_parent, bnodes, bflags, stats;
@@ -1377,8 +1445,8 @@ struct ft_ctx {
     bool have_nearest = false;
     // phase timing (ft_set_timing): events at the phase boundaries of a reinit
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double phase_ms[3] = {0, 0, 0};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double phase_ms[4] = {0, 0, 0, 0};
     int64_t timed_calls = 0;
 };
 
@@ -1392,7 +1460,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk,
             &h->xtet, &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp,
             &h->ifp2e, &h->tris, &h->tri_norm, &h->if_elems, &h->p2t_off,
-            &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
+            &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->near2, &h->bin_count,
             &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
             &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars};
@@ -1677,6 +1745,7 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
         FT_TRY(ensure(c->roots, sizeof(int32_t) * nf));
         FT_TRY(ensure(c->leaf_off, sizeof(int32_t) * (nf + 1)));
         if (want_nearest) FT_TRY(ensure(c->nearest, sizeof(int32_t) * at_least1(N)));
+        if (mode == 1) FT_TRY(ensure(c->near2, sizeof(int2) * at_least1(N)));
     } else if (stage == 1) {
         FT_TRY(ensure(c->xtet, 4 * at_least1(cp.X)));
         FT_TRY(ensure(c->xtri_off, 4 * at_least1(cp.X)));
@@ -1933,6 +2002,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     q.sc = sc;
     q.xcd_chunk = kXcdChunk;
+    q.near2 = c->near2.as<int2>();
     if (timed) (void)hipEventRecord(c->ev[2], s);
     if (c->n_slots > 0) {
         const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
@@ -1949,6 +2019,11 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         FT_TRY(check_launch());
     }
     if (timed) (void)hipEventRecord(c->ev[3], s);
+    if (mode == 1 && c->N > 0) {
+        hipLaunchKernelGGL(k_settle, dim3(grid1d(c->N)), dim3(kBS), 0, s, q, c->xyz.as<float>(), c->N);
+        FT_TRY(check_launch());
+    }
+    if (timed) (void)hipEventRecord(c->ev[4], s);
     return FT_OK;
 }
 
@@ -2068,10 +2143,10 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
     if (rc == FT_OK && want_stats && c->N > 0) print_query_stats(c, sp.mode);
     if (rc == FT_OK && c->timing) {
-        float ms[3] = {0, 0, 0};
-        (void)hipEventSynchronize(c->ev[3]);
-        for (int k = 0; k < 3; k++) (void)hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]);
-        for (int k = 0; k < 3; k++) c->phase_ms[k] += ms[k];
+        float ms[4] = {0, 0, 0, 0};
+        (void)hipEventSynchronize(c->ev[4]);
+        for (int k = 0; k < 4; k++) (void)hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]);
+        for (int k = 0; k < 4; k++) c->phase_ms[k] += ms[k];
         c->timed_calls++;
     }
     return rc;
@@ -2260,7 +2335,7 @@ int ft_destroy(ft_handle h)
     drop_graph(h);
     for (DevBuf* b : all_buffers(h)) release(*b);
     if (h->h_scalars) (void)hipHostFree(h->h_scalars);
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -2729,9 +2804,9 @@ int ft_set_timing(ft_handle h, int32_t on)
 {
     FT_TRY(check_handle(h));
     if (on && !h->ev[0])
-        for (int k = 0; k < 4; k++) FT_HIP(hipEventCreate(&h->ev[k]));
+        for (int k = 0; k < 5; k++) FT_HIP(hipEventCreate(&h->ev[k]));
     h->timing = on != 0;
-    for (int k = 0; k < 3; k++) h->phase_ms[k] = 0.0;
+    for (int k = 0; k < 4; k++) h->phase_ms[k] = 0.0;
     h->timed_calls = 0;
     return FT_OK;
 }
@@ -2740,7 +2815,7 @@ int ft_get_timing(ft_handle h, double* phase_ms, int64_t* calls)
 {
     FT_TRY(check_handle(h));
     if (!phase_ms || !calls) return ft::set_error(FT_ERR_ARG, "null output");
-    for (int k = 0; k < 3; k++) phase_ms[k] = h->phase_ms[k];
+    for (int k = 0; k < 4; k++) phase_ms[k] = h->phase_ms[k];
     *calls = h->timed_calls;
     return FT_OK;
 }

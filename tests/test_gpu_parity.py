@@ -124,6 +124,38 @@ def test_triangle_mode_sampled(rein, name):
     assert mism == 0
 
 
+@pytest.mark.parametrize("name,band_h,n_sample", [("cfg3", 0.0, 2000), ("cfg2", 6.0, 3000)])
+def test_full_size_workloads(name, band_h, n_sample):
+    """The benchmark configurations at full size: 256^3 jittered, 64 spheres,
+    full domain; 128^3 star, narrow band 6h.  Interface bit-exact against the
+    oracle; distances on a seeded node sample within 1e-5 h of FP64, nearest
+    triangle equal outside eps-ties."""
+    m, phi, h = workload(name)
+    band = band_h * h
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, cnt = r.reset_phi(phi, mode="triangles", band=band, want_nearest=True)
+    ref = oracle_interface(name)
+    assert_interface_bit_exact(r.interface(), ref)
+    rng = np.random.default_rng(14)
+    nodes = rng.choice(m.n_nodes, n_sample, replace=False)
+    near = r.nearest()[nodes]
+    got = np.abs(ifn[nodes])
+    r.close()
+    if band > 0:
+        d64, _ = oracle.triangle_distance(m.xyz, ref["ifp"], ref["tris"], nodes)
+        outb = d64 > band * (1 + 1e-6)
+        assert np.all(near[outb] == -1) and np.all(got[outb] == np.float64(np.float32(band)))
+        keep = d64 < band * (1 - 1e-6)
+        nodes, near, got = nodes[keep], near[keep], got[keep]
+    emax, erms, ties, mism = triangle_errors(name, got, near, nodes, h, ref=ref)
+    print(f"{name} full size triangles: {len(nodes)} nodes checked, max err {emax:.3e} "
+          f"({emax / h:.2e} h) rms {erms:.3e} ({erms / h:.2e} h), eps-ties {ties}, "
+          f"mismatches {mism}", flush=True)
+    assert emax <= 1e-5 * h
+    assert mism == 0
+
+
 def test_triangle_mode_band(rein):
     """Narrow band 6h: exact inside, clamped to +-band outside."""
     name = "cfg2s"
