@@ -438,10 +438,12 @@ def main():
     t_query = ph[2] * 1e-3
     peak = 8000.0
     achieved = b_query / t_query / 1e9
-    traffic = None
+    traffic = valu = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.mode}.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("query_bytes_per_launch")
+    if os.path.exists(tf):                       # rocprofv3 counters of this kernel (profiles/)
+        prof = json.load(open(tf))
+        traffic = prof.get("query_bytes_per_launch")
+        valu = prof.get("valu_busy_frac")
 
     cpu = None
     if not args.no_cpu:
@@ -470,9 +472,11 @@ def main():
                    "clamped_nodes": cnt.n_clamped},
         "phases_ms": {"marching_tets": ph[0], "index_build": ph[1], "query": ph[2],
                       "settle": ph[3]},
-        "roofline": {"kernel": "k_query (nearest-triangle search + epilogue)", "bound": "hbm",
-                     "achieved": achieved, "peak": peak, "unit": "GB/s",
-                     "frac": achieved / peak, "traffic": traffic},
+        "roofline": {"kernel": f"k_query_wave<{1 if args.mode == 'triangles' else 0},"
+                               f"{args.query_block or 256}> (nearest-primitive search)",
+                     "bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "valu_busy_frac_profiled": valu},
         "path_roofline": {"algorithmic_bytes": b_path,
                           "achieved_GBps": b_path / (ms * 1e-3) / 1e9,
                           "frac": b_path / (ms * 1e-3) / 1e9 / peak},
