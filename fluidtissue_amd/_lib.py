@@ -40,7 +40,8 @@ class FtOpts(C.Structure):
         ("bin_scale", C.c_float),
         ("want_nearest", C.c_int32),
         ("query_block", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("want_normals", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -107,6 +108,8 @@ _SIGS = {
     "ft_reinit_batch": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
     "ft_reinit_graph": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
     "ft_graph_stats": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ft_get_normals": (C.c_int, [_H, _P]),
+    "ft_normals_device": (C.c_int, [_H, C.POINTER(C.c_void_p)]),
     "ft_set_timing": (C.c_int, [_H, C.c_int32]),
     "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "ft_kuhn_counts": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64),

@@ -1205,6 +1205,110 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
 }
 
 // ============================================================================
+// Level-set normals: Assembler::grad / grad_nodes (Assembler.cpp:11-71), the
+// last step of LevelSet::resetPhi_ (LevelSet.cpp:451-453)
+// ============================================================================
+// Node -> tets lists (Mesh3D::p2t, TetraMesh.cpp:84-86, 115-131): counts,
+// scan, scatter, then each list sorted to increasing tet id.
+__global__ __launch_bounds__(kBS) void k_n2t_count(const int4* tets, int64_t T, uint32_t* cnt)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (t >= T) return;
+    const int4 q = tets[t];
+    atomicAdd(&cnt[q.x], 1u);
+    atomicAdd(&cnt[q.y], 1u);
+    atomicAdd(&cnt[q.z], 1u);
+    atomicAdd(&cnt[q.w], 1u);
+}
+
+__global__ __launch_bounds__(kBS) void k_n2t_fill(const int4* tets, int64_t T, const uint64_t* off,
+                                                  uint32_t* cursor, int32_t* ids)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (t >= T) return;
+    const int4 q = tets[t];
+    const int v[4] = {q.x, q.y, q.z, q.w};
+    for (int k = 0; k < 4; k++) ids[off[v[k]] + atomicAdd(&cursor[v[k]], 1u)] = (int32_t)t;
+}
+
+__global__ __launch_bounds__(kBS) void k_n2t_sort(const uint64_t* off, int64_t N, int32_t* ids)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= N) return;
+    const uint64_t b = off[i], e = off[i + 1];
+    for (uint64_t a = b + 1; a < e; a++) {
+        const int32_t key = ids[a];
+        uint64_t j = a;
+        while (j > b && ids[j - 1] > key) { ids[j] = ids[j - 1]; j--; }
+        ids[j] = key;
+    }
+}
+
+// Assembler::grad: per tet, B = [p1-p0, p2-p0, p3-p0] in FP64 from the float
+// nodes; its inverse transposed is cofactor(r,c) / det, det = sum of
+// cofactor(r,0) B(r,0) as x0 + (x1 + x2) (getElementInfo_ij_,
+// Assembler.hpp:429-459); the basis gradients (-1,-1,-1), e1, e2, e3
+// (Tetrahedron.hpp:36-50) mapped by it, the first summed left to right; then
+// g = ((u0 c0 + u1 c1) + u2 c2) + u3 c3, cast to float.  A singular tet (det
+// 0, which the reference only reports) gets a zero gradient.
+__global__ __launch_bounds__(kBS) void k_tet_grad(const int4* tets, int64_t T, const float* xyz,
+                                                  const double* u, float* grad_t, const uint64_t* sc)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (t >= T || halted(sc)) return;
+    const int4 q = tets[t];
+    const int qi[4] = {q.x, q.y, q.z, q.w};
+    double x[4][3];
+    for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 3; k++) x[j][k] = (double)xyz[3 * (int64_t)qi[j] + k];
+    double B[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) B[r][c] = x[c + 1][r] - x[0][r];
+    double C[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            C[r][c] = B[r1][c1] * B[r2][c2] - B[r1][c2] * B[r2][c1];
+        }
+    const double det = C[0][0] * B[0][0] + (C[1][0] * B[1][0] + C[2][0] * B[2][0]);
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    if (det != 0.0) {
+        const double inv = 1.0 / det;
+        const double u0 = u[qi[0]], u1 = u[qi[1]], u2 = u[qi[2]], u3 = u[qi[3]];
+        for (int r = 0; r < 3; r++) {
+            const double b0 = C[r][0] * inv, b1 = C[r][1] * inv, b2 = C[r][2] * inv;
+            const double c0 = (b0 * -1.0 + b1 * -1.0) + b2 * -1.0;
+            g[r] = (float)(((u0 * c0 + u1 * b0) + u2 * b1) + u3 * b2);
+        }
+    }
+    grad_t[3 * t] = g[0];
+    grad_t[3 * t + 1] = g[1];
+    grad_t[3 * t + 2] = g[2];
+}
+
+// Assembler::grad_nodes: per node, the float mean of its tets' gradients,
+// summed in increasing tet id.
+__global__ __launch_bounds__(kBS) void k_node_normals(const uint64_t* off, const int32_t* ids,
+                                                      const float* grad_t, int64_t N, float* out,
+                                                      const uint64_t* sc)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= N || halted(sc)) return;
+    const uint64_t b = off[i], e = off[i + 1];
+    float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+    for (uint64_t j = b; j < e; j++) {
+        const int64_t t = ids[j];
+        n0 += grad_t[3 * t];
+        n1 += grad_t[3 * t + 1];
+        n2 += grad_t[3 * t + 2];
+    }
+    const float cnt = (float)(e - b);
+    out[3 * i] = n0 / cnt;
+    out[3 * i + 1] = n1 / cnt;
+    out[3 * i + 2] = n2 / cnt;
+}
+
+// ============================================================================
 // Per-field interface counts (batches): ranks at the field boundaries
 // ============================================================================
 __device__ __forceinline__ uint64_t low_mask(int b) { return b ? ((1ull << b) - 1ull) : 0ull; }
@@ -1426,6 +1530,10 @@ struct ft_ctx {
     DevBuf sbits, retain, tbits, blk, xtet, xtri_off, xedges, ebits, eprefix;
     DevBuf ifp, ifp2e, tris, tri_norm, if_elems, p2t_off, p2t_cursor, p2t, ifp_area;
     DevBuf nearest, near2, fcounts, fclamp;
+    // normals: node -> tets lists (per field layout), tet gradients, node normals
+    DevBuf n2t_off, n2t_ids, grad_t, normals, phi_ifn_int;
+    uint64_t n2t_layout = ~0ull;
+    bool have_normals = false;
     // index
     DevBuf bin_count, bin_prefix, pcode, prec, trec, pidx, leaf_key, leaf_start, leaf_cnt;
     DevBuf leaf_b, leaf_parent, bnodes, bflags, stats;
@@ -1463,7 +1571,8 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->near2, &h->bin_count,
             &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
-            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars};
+            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
+            &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int};
 }
 
 std::vector<void*> buffer_addresses(ft_ctx* c)
@@ -1728,7 +1837,8 @@ inline int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, (n + kScanTi
 // Device buffers of the stages of one reinit.  Stage 0: sized by the mesh;
 // 1: by crossing tets and triangles; 2: by crossing points; 3: the index over
 // n primitives (leaves <= min(bins, n)).
-int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_nearest)
+int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_nearest,
+                 bool want_normals = false)
 {
     const int64_t N = c->N, T = c->T, E = c->E, nf = c->nf;
     const int64_t nw_nodes = (N + 63) / 64, nw_tets = (T + 63) / 64, nw_edges = (E + 63) / 64;
@@ -1746,6 +1856,11 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
         FT_TRY(ensure(c->leaf_off, sizeof(int32_t) * (nf + 1)));
         if (want_nearest) FT_TRY(ensure(c->nearest, sizeof(int32_t) * at_least1(N)));
         if (mode == 1) FT_TRY(ensure(c->near2, sizeof(int2) * at_least1(N)));
+        if (want_normals) {
+            FT_TRY(ensure(c->normals, 12 * at_least1(N)));
+            FT_TRY(ensure(c->grad_t, 12 * at_least1(T)));
+            FT_TRY(ensure(c->phi_ifn_int, 8 * at_least1(N)));
+        }
     } else if (stage == 1) {
         FT_TRY(ensure(c->xtet, 4 * at_least1(cp.X)));
         FT_TRY(ensure(c->xtri_off, 4 * at_least1(cp.X)));
@@ -1799,6 +1914,31 @@ int stage_error(uint64_t ev, ft_counts* cnt)
                          "FLT_MIN (LevelSet.cpp:338-341)", (long long)cnt->bad_tet);
 }
 
+// Node -> tets lists of the current fields (once per field layout; not in a
+// captured graph: it synchronises).
+int build_node_tets(ft_ctx* c, hipStream_t s)
+{
+    if (c->n2t_layout == c->layout_gen) return FT_OK;
+    const int64_t N = c->N, T = c->T;
+    FT_TRY(ensure(c->n2t_off, sizeof(uint64_t) * (N + 1)));
+    FT_TRY(ensure(c->n2t_ids, sizeof(int32_t) * at_least1(4 * T)));
+    FT_TRY(ensure(c->pcode, sizeof(uint32_t) * at_least1(N)));          // counters
+    uint32_t* cnt = c->pcode.as<uint32_t>();
+    uint64_t* off = c->n2t_off.as<uint64_t>();
+    FT_TRY(fill<uint32_t>(s, cnt, N, 0u));
+    if (T > 0) hipLaunchKernelGGL(k_n2t_count, dim3(grid1d(T)), dim3(kBS), 0, s, c->tets.as<int4>(), T, cnt);
+    FT_TRY(scan<uint32_t, uint64_t>(c, s, cnt, off, N, off + N, LoadSame{}));
+    FT_TRY(fill<uint32_t>(s, cnt, N, 0u));
+    if (T > 0)
+        hipLaunchKernelGGL(k_n2t_fill, dim3(grid1d(T)), dim3(kBS), 0, s, c->tets.as<int4>(), T, off, cnt,
+                           c->n2t_ids.as<int32_t>());
+    if (N > 0) hipLaunchKernelGGL(k_n2t_sort, dim3(grid1d(N)), dim3(kBS), 0, s, off, N, c->n2t_ids.as<int32_t>());
+    FT_TRY(check_launch());
+    FT_HIP(hipStreamSynchronize(s));
+    c->n2t_layout = c->layout_gen;
+    return FT_OK;
+}
+
 // What the search of a reinit reads and writes.
 struct SearchSpec {
     int mode = 0;                    // 0: crossing points, 1: triangles
@@ -1810,6 +1950,8 @@ struct SearchSpec {
     int epi = EPI_RAW;
     double w_phi = 0.0, w_ifn = 0.0;
     bool want_nearest = false;
+    bool want_normals = false;       // Assembler::grad_nodes of phi_ifn after the epilogue
+    bool int_ifn = false;            // phi_ifn into the handle's buffer (normals need it)
     int query_block = 0;
 };
 
@@ -1819,15 +1961,15 @@ struct SearchSpec {
 // reads and no allocation; the buffers were sized for `cp` beforehand and
 // k_check_caps guards them.  counts_only: stops once the totals are known
 // (phi is not written).
-int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_nearest, Caps& cp,
-                     bool eager, bool counts_only, ft_counts* cnt)
+int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_nearest,
+                     bool want_normals, Caps& cp, bool eager, bool counts_only, ft_counts* cnt)
 {
     const int64_t N = c->N, T = c->T, E = c->E;
     uint64_t* sc = c->scalars.as<uint64_t>();
     const int64_t nw_nodes = (N + 63) / 64, nw_tets = (T + 63) / 64, nw_edges = (E + 63) / 64;
     const int64_t nblk = (T + kBS - 1) / kBS;
     const bool timed = eager && c->timing && !counts_only;
-    FT_TRY(ensure_stage(c, 0, cp, mode, want_nearest));
+    FT_TRY(ensure_stage(c, 0, cp, mode, want_nearest, want_normals));
     uint64_t* sbits = c->sbits.as<uint64_t>();
     if (timed) (void)hipEventRecord(c->ev[0], s);
     hipLaunchKernelGGL(k_node_signs, dim3(grid1d(N)), dim3(kBS), 0, s, phi, N, sbits);
@@ -1984,7 +2126,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     q.tris = tris;
     q.phi_old = sp.phi_old;
     q.phi_out = sp.phi_out;
-    q.phi_ifn = sp.phi_ifn;
+    q.phi_ifn = sp.phi_ifn ? sp.phi_ifn : (sp.int_ifn ? c->phi_ifn_int.as<double>() : nullptr);
     q.retain = sp.use_retain ? c->retain.as<uint64_t>() : nullptr;   // resolved after allocation
     q.nearest = sp.want_nearest ? c->nearest.as<int32_t>() : nullptr;
     q.epi = sp.epi;
@@ -2021,6 +2163,16 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     if (timed) (void)hipEventRecord(c->ev[3], s);
     if (mode == 1 && c->N > 0) {
         hipLaunchKernelGGL(k_settle, dim3(grid1d(c->N)), dim3(kBS), 0, s, q, c->xyz.as<float>(), c->N);
+        FT_TRY(check_launch());
+    }
+    if (sp.want_normals && c->N > 0) {
+        if (c->n2t_layout != c->layout_gen)
+            return ft::set_error(FT_ERR_STATE, "node -> tets lists not built for this field layout");
+        if (c->T > 0)
+            hipLaunchKernelGGL(k_tet_grad, dim3(grid1d(c->T)), dim3(kBS), 0, s, c->tets.as<int4>(), c->T,
+                               c->xyz.as<float>(), q.phi_ifn, c->grad_t.as<float>(), sc);
+        hipLaunchKernelGGL(k_node_normals, dim3(grid1d(c->N)), dim3(kBS), 0, s, c->n2t_off.as<uint64_t>(),
+                           c->n2t_ids.as<int32_t>(), c->grad_t.as<float>(), c->N, c->normals.as<float>(), sc);
         FT_TRY(check_launch());
     }
     if (timed) (void)hipEventRecord(c->ev[4], s);
@@ -2102,6 +2254,8 @@ SearchSpec reinit_search_spec(ft_ctx* c, double* phi, double* phi_ifn, const ft_
     sp.w_phi = o->w_phi;
     sp.w_ifn = o->w_ifn;
     sp.want_nearest = o->want_nearest != 0;
+    sp.want_normals = o->want_normals != 0;
+    sp.int_ifn = sp.want_normals && !phi_ifn;
     sp.query_block = o->query_block;
     return sp;
 }
@@ -2130,14 +2284,18 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     if (!d_phi) return ft::set_error(FT_ERR_ARG, "null phi");
     c->have_interface = false;
     c->have_nearest = false;
+    c->have_normals = false;
     ft_counts cnt{};
     cnt.bad_tet = -1;
     int rc = index_setup(c, s, o->bin_scale);
     Caps cp;
     const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
-    if (rc == FT_OK) rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, cp, true, false, &cnt);
+    if (rc == FT_OK && sp.want_normals) rc = build_node_tets(c, s);
+    if (rc == FT_OK)
+        rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cp, true, false, &cnt);
     if (rc == FT_OK) rc = enqueue_search(c, s, sp, cp, true);
     if (rc == FT_OK) rc = collect(c, s, sp.want_nearest, &cnt);
+    c->have_normals = rc == FT_OK && sp.want_normals;
     if (counts) *counts = cnt;
     c->last = cnt;
     static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
@@ -2161,7 +2319,7 @@ bool same_opts(const ft_opts& a, const ft_opts& b)
 {
     return a.mode == b.mode && a.blend == b.blend && a.w_phi == b.w_phi && a.w_ifn == b.w_ifn &&
            a.band == b.band && a.bin_scale == b.bin_scale && a.want_nearest == b.want_nearest &&
-           a.query_block == b.query_block;
+           a.query_block == b.query_block && a.want_normals == b.want_normals;
 }
 
 void drop_graph(ft_ctx* c)
@@ -2178,12 +2336,14 @@ int capture_reinit(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o
 {
     drop_graph(c);
     const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
-    for (int st = 0; st < 4; st++) FT_TRY(ensure_stage(c, st, cp, sp.mode, sp.want_nearest));
+    if (sp.want_normals) FT_TRY(build_node_tets(c, c->stream));
+    for (int st = 0; st < 4; st++) FT_TRY(ensure_stage(c, st, cp, sp.mode, sp.want_nearest, sp.want_normals));
     FT_HIP(hipStreamSynchronize(c->stream));
     FT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     Caps cc = cp;
     ft_counts unused{};
-    int rc = enqueue_marching(c, c->stream, d_phi, sp.mode, sp.want_nearest, cc, false, false, &unused);
+    int rc = enqueue_marching(c, c->stream, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cc, false,
+                              false, &unused);
     if (rc == FT_OK) rc = enqueue_search(c, c->stream, sp, cc, false);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
@@ -2226,6 +2386,7 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
     if (!d_phi) return ft::set_error(FT_ERR_ARG, "null phi");
     c->have_interface = false;
     c->have_nearest = false;
+    c->have_normals = false;
     GraphState& g = c->graph;
     ft_counts cnt{};
     cnt.bad_tet = -1;
@@ -2237,7 +2398,8 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
     if (fresh) {
         Caps exact;
         const int mode = o->mode == FT_DIST_POINTS ? 0 : 1;
-        const int rc = enqueue_marching(c, s, d_phi, mode, o->want_nearest != 0, exact, true, true, &cnt);
+        const int rc = enqueue_marching(c, s, d_phi, mode, o->want_nearest != 0, o->want_normals != 0,
+                                        exact, true, true, &cnt);
         if (rc != FT_OK) {
             if (counts) *counts = cnt;
             c->last = cnt;
@@ -2252,6 +2414,7 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
         FT_HIP(hipGraphLaunch(g.exec, s));
         g.launches++;
         const int rc = collect(c, s, o->want_nearest != 0, &cnt);
+        c->have_normals = rc == FT_OK && o->want_normals != 0;
         if (rc != FT_ERR_BUDGET) {
             if (counts) *counts = cnt;
             c->last = cnt;
@@ -2699,6 +2862,25 @@ int ft_reinit_graph(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts
     if (per_field)
         for (int k = 0; k < h->nf && k < (int)h->field_counts.size(); k++) per_field[k] = h->field_counts[k];
     return rc;
+}
+
+int ft_get_normals(ft_handle h, float* normals)
+{
+    FT_TRY(check_handle(h));
+    if (!normals) return ft::set_error(FT_ERR_ARG, "null output");
+    if (!h->have_normals) return ft::set_error(FT_ERR_STATE, "no normals: reinit with want_normals first");
+    if (h->N > 0) FT_HIP(hipMemcpyAsync(normals, h->normals.p, 12 * h->N, hipMemcpyDeviceToHost, h->stream));
+    FT_HIP(hipStreamSynchronize(h->stream));
+    return FT_OK;
+}
+
+int ft_normals_device(ft_handle h, const float** normals)
+{
+    FT_TRY(check_handle(h));
+    if (!normals) return ft::set_error(FT_ERR_ARG, "null output");
+    if (!h->have_normals) return ft::set_error(FT_ERR_STATE, "no normals: reinit with want_normals first");
+    *normals = h->normals.as<float>();
+    return FT_OK;
 }
 
 int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches)

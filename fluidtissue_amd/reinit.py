@@ -95,7 +95,7 @@ def phi_field(kind, xyz, seed=0):
 
 
 def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest=False,
-              w_phi=0.75, w_ifn=0.25, query_block=0):
+              w_phi=0.75, w_ifn=0.25, query_block=0, want_normals=False):
     o = _lib.FtOpts()
     lib.ft_opts_default(C.byref(o))
     o.mode = MODES[mode]
@@ -106,6 +106,7 @@ def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest
     o.w_phi = float(w_phi)
     o.w_ifn = float(w_ifn)
     o.query_block = int(query_block)
+    o.want_normals = 1 if want_normals else 0
     return o
 
 
@@ -169,20 +170,23 @@ class Reinit(_Handle):
         self.n = xyz.shape[0]
 
     def reset_phi(self, phi, mode="points", blend="init", band=0.0, bin_scale=0.0,
-                  want_ifn=True, want_nearest=False, query_block=0):
+                  want_ifn=True, want_nearest=False, query_block=0, want_normals=False):
         """Returns (phi_out, phi_ifn, counts); phi is not modified."""
         out = np.array(phi, dtype=np.float64, copy=True)
         ifn = np.empty_like(out) if want_ifn else None
-        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
+                      want_normals=want_normals)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit(self._h, _ptr(out), _ptr(ifn), C.byref(o), C.byref(cnt)))
         return out, ifn, cnt
 
     def reset_phi_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
-                         bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
+                         bin_scale=0.0, want_nearest=False, stream=None, query_block=0,
+                         want_normals=False):
         """Device-resident phi (raw device addresses, e.g. tensor.data_ptr())."""
-        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
+                      want_normals=want_normals)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit_device(self._h, C.c_void_p(phi_ptr),
@@ -251,8 +255,10 @@ class Reinit(_Handle):
         return list(arr)
 
     def reset_phi_batch_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
-                               bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
-        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
+                               bin_scale=0.0, want_nearest=False, stream=None, query_block=0,
+                               want_normals=False):
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
+                      want_normals=want_normals)
         rc = lib.ft_reinit_batch(self._h, C.c_void_p(phi_ptr),
                                  C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
                                  C.c_void_p(stream) if stream else None)
@@ -273,10 +279,12 @@ class Reinit(_Handle):
         return per
 
     def reset_phi_graph_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
-                               bin_scale=0.0, want_nearest=False, stream=None, query_block=0):
+                               bin_scale=0.0, want_nearest=False, stream=None, query_block=0,
+                               want_normals=False):
         """reset_phi_batch_device through one captured HIP graph per call
         (captured on the first call for these buffers and options)."""
-        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block)
+        o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
+                      want_normals=want_normals)
         rc = lib.ft_reinit_graph(self._h, C.c_void_p(phi_ptr),
                                  C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
                                  C.c_void_p(stream) if stream else None)
@@ -311,6 +319,12 @@ class Reinit(_Handle):
         check(lib.ft_get_interface(self._h, _ptr(ifp), _ptr(ifp2e), _ptr(tris), _ptr(elems),
                                    _ptr(area), _ptr(off), _ptr(p2t)))
         return Interface(ifp, ifp2e, tris, elems, area, off, p2t)
+
+    def normals(self):
+        """Level-set normals of the last reinit with want_normals (n x 3 float)."""
+        out = np.empty((self.n, 3), np.float32)
+        check(lib.ft_get_normals(self._h, _ptr(out)))
+        return out
 
     def nearest(self):
         out = np.empty(self.n, np.int32)

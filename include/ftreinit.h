@@ -73,7 +73,10 @@ typedef struct ft_opts {
     int32_t want_nearest;/* 1: keep the per-node nearest primitive index       */
     int32_t query_block; /* threads per block of the distance query: 64, 128,
                             256 (waves per workgroup 1, 2, 4); 0 = default     */
-    int32_t reserved[5];
+    int32_t want_normals;/* 1: also the level-set normals of phi_ifn, as
+                            LevelSet::resetPhi_ ends (LevelSet.cpp:451-453,
+                            Assembler::grad_nodes, Assembler.cpp:11-71)        */
+    int32_t reserved[4];
 } ft_opts;
 
 typedef struct ft_counts {
@@ -154,6 +157,15 @@ int ft_reinit_batch(ft_handle h, double* d_phi, double* d_phi_ifn,
  * those of ft_reinit_batch, bit for bit.  Returns after the graph has run. */
 int ft_reinit_graph(ft_handle h, double* d_phi, double* d_phi_ifn,
                     const ft_opts* o, ft_counts* per_field, void* stream);
+
+/* Level-set normals of the last reinit with want_normals = 1 (normals_,
+ * LevelSet.hpp:131): per node the mean of its tets' P1 gradients of phi_ifn,
+ * summed in increasing tet id (Assembler.cpp:46-71); n_nodes x 3 float.  A
+ * tet with a singular map gets a zero gradient (the reference reports it,
+ * Assembler.hpp:444-455, and leaves its inverse unset). */
+int ft_get_normals(ft_handle h, float* normals);
+/* Device address of the same (valid until the next call on h). */
+int ft_normals_device(ft_handle h, const float** normals);
 
 /* Captures and launches of ft_reinit_graph on h so far (nullable). */
 int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches);

@@ -379,3 +379,86 @@ void ora_finish_reset(int64_t n_nodes, const double* phi_old,
             phi_out[i] = w_phi * retained[i] + w_ifn * phi_ifn[i];  /* :148 */
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Level-set normals: Assembler::grad / grad_nodes (Assembler.cpp:11-71),   */
+/* called at the end of LevelSet::resetPhi_ (LevelSet.cpp:451-453).         */
+/* ---------------------------------------------------------------------- */
+void ora_node_tets(int64_t n_nodes, int64_t n_tets, const int32_t* tets,
+                   int64_t* off, int32_t* tet_ids)
+{
+    /* a stable counting sort of the (node, tet) pairs by node; tets are
+     * visited in increasing id, so each node's list comes out sorted, as
+     * std::sort of the pairs gives it (TetraMesh.cpp:117) */
+    memset(off, 0, sizeof(int64_t) * (size_t)(n_nodes + 1));
+    for (int64_t t = 0; t < n_tets; t++)
+        for (int k = 0; k < 4; k++) off[tets[4 * t + k] + 1]++;
+    for (int64_t i = 0; i < n_nodes; i++) off[i + 1] += off[i];
+    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n_nodes > 0 ? n_nodes : 1));
+    memcpy(cur, off, sizeof(int64_t) * (size_t)n_nodes);
+    for (int64_t t = 0; t < n_tets; t++)
+        for (int k = 0; k < 4; k++) {
+            const int32_t v = tets[4 * t + k];
+            tet_ids[cur[v]++] = (int32_t)t;
+        }
+    free(cur);
+}
+
+int64_t ora_tet_grad(int64_t n_tets, const int32_t* tets, const float* p,
+                     const double* u, float* grad_t)
+{
+    int64_t bad = -1;
+    for (int64_t t = 0; t < n_tets; t++) {
+        const int32_t* q = tets + 4 * t;
+        double x[4][3];
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) x[j][k] = (double)p[3 * (int64_t)q[j] + k];
+        double B[3][3];                      /* column c = x[c+1] - x[0] */
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) B[r][c] = x[c + 1][r] - x[0][r];
+        /* cofactors C(r,c) = B(r1,c1) B(r2,c2) - B(r1,c2) B(r2,c1), cyclic */
+        double C[3][3];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                C[r][c] = B[r1][c1] * B[r2][c2] - B[r1][c2] * B[r2][c1];
+            }
+        const double det = C[0][0] * B[0][0] + (C[1][0] * B[1][0] + C[2][0] * B[2][0]);
+        if (det == 0.0) {
+            if (bad < 0) bad = t;
+            grad_t[3 * t] = grad_t[3 * t + 1] = grad_t[3 * t + 2] = 0.0f;
+            continue;
+        }
+        const double inv = 1.0 / det;
+        /* inverse = adjugate / det: inv(r,c) = C(c,r) / det; its transpose
+         * invBt(r,c) = C(r,c) * inv */
+        double Bt[3][3];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) Bt[r][c] = C[r][c] * inv;
+        double g[3];
+        for (int r = 0; r < 3; r++) {
+            /* invBt times the basis gradients (a product with a dynamic-size
+             * matrix, Quadrature.hpp:118-126: its inner sums run left to
+             * right); the unit gradients select a column exactly */
+            const double c0 = (Bt[r][0] * -1.0 + Bt[r][1] * -1.0) + Bt[r][2] * -1.0;
+            const double c1 = Bt[r][0];
+            const double c2 = Bt[r][1];
+            const double c3 = Bt[r][2];
+            g[r] = ((u[q[0]] * c0 + u[q[1]] * c1) + u[q[2]] * c2) + u[q[3]] * c3;
+        }
+        for (int k = 0; k < 3; k++) grad_t[3 * t + k] = (float)g[k];
+    }
+    return bad;
+}
+
+void ora_grad_nodes(int64_t n_nodes, const int64_t* off, const int32_t* tet_ids,
+                    const float* grad_t, float* grad_n)
+{
+    for (int64_t i = 0; i < n_nodes; i++) {
+        float n[3] = {0.0f, 0.0f, 0.0f};
+        for (int64_t j = off[i]; j < off[i + 1]; j++)
+            for (int k = 0; k < 3; k++) n[k] += grad_t[3 * (int64_t)tet_ids[j] + k];
+        const float cnt = (float)(off[i + 1] - off[i]);
+        for (int k = 0; k < 3; k++) grad_n[3 * i + k] = n[k] / cnt;
+    }
+}

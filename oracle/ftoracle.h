@@ -104,6 +104,27 @@ void ora_finish_reset(int64_t n_nodes, const double* phi_old,
                       double w_phi, double w_ifn, double* retained,
                       double* phi_out);
 
+/* Mesh3D::p2t, node -> tetrahedra (TetraMesh.cpp:84-86, 115-131): the
+ * (node, tet) pairs sorted, so each node's tets in increasing id order.
+ * off: n_nodes+1 entries; tet_ids: 4*n_tets entries. */
+void ora_node_tets(int64_t n_nodes, int64_t n_tets, const int32_t* tets,
+                   int64_t* off, int32_t* tet_ids);
+
+/* Assembler::grad (Assembler.cpp:11-42): per tet, the P1 gradient of u from
+ * B = [p1-p0, p2-p0, p3-p0] (FP64 from the float nodes), its inverse
+ * transposed by cofactors / determinant (Assembler.hpp:429-459), the basis
+ * gradients (-1,-1,-1), e1, e2, e3 (Tetrahedron.hpp:36-50) mapped by it,
+ * then g = u0 c0 + u1 c1 + u2 c2 + u3 c3, cast to float (3 per tet).
+ * Returns the first tet with det B == 0 (the reference reports it and
+ * returns -1), or -1. */
+int64_t ora_tet_grad(int64_t n_tets, const int32_t* tets, const float* p,
+                     const double* u, float* grad_t);
+
+/* Assembler::grad_nodes (Assembler.cpp:46-71): per node, the float mean of
+ * the gradients of its tets, summed in p2t order. */
+void ora_grad_nodes(int64_t n_nodes, const int64_t* off, const int32_t* tet_ids,
+                    const float* grad_t, float* grad_n);
+
 #ifdef __cplusplus
 }
 #endif

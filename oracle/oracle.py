@@ -34,6 +34,13 @@ _o.ora_node_triangle_distance.argtypes = [_i64, _P, _P, _P, _i64]
 _o.ora_finish_reset.restype = None
 _o.ora_finish_reset.argtypes = [_i64, _P, _P, _i64, _P, _P, C.c_int, C.c_double, C.c_double, _P, _P]
 
+_o.ora_node_tets.restype = None
+_o.ora_node_tets.argtypes = [_i64, _i64, _P, _P, _P]
+_o.ora_tet_grad.restype = _i64
+_o.ora_tet_grad.argtypes = [_i64, _P, _P, _P, _P]
+_o.ora_grad_nodes.restype = None
+_o.ora_grad_nodes.argtypes = [_i64, _P, _P, _P, _P]
+
 NTHREADS = max(1, min(16, os.cpu_count() or 1))
 
 
@@ -155,3 +162,32 @@ def reinit(mesh_xyz, tets, edges, t2e, phi, blend=0, nthreads=NTHREADS):
     ret, out = finish_reset(phi, ifn, ifp2e, edges, blend)
     return dict(ifp=ifp, ifp2e=ifp2e, tris=tris, if_elems=elems, ifp_area=area, phi_ifn=ifn,
                 retained=ret, phi=out)
+
+
+def node_tets(n_nodes, tets):
+    """Mesh3D::p2t as CSR: (offsets n_nodes+1, tet ids in increasing order)."""
+    tets = _c(tets, np.int32)
+    off = np.empty(n_nodes + 1, np.int64)
+    ids = np.empty(4 * len(tets), np.int32)
+    _o.ora_node_tets(n_nodes, len(tets), _p(tets), _p(off), _p(ids))
+    return off, ids
+
+
+def tet_grad(tets, xyz, u):
+    """Assembler::grad: float gradient per tet; also the first singular tet or -1."""
+    tets = _c(tets, np.int32)
+    xyz = _c(xyz, np.float32)
+    u = _c(u, np.float64)
+    g = np.empty((len(tets), 3), np.float32)
+    bad = _o.ora_tet_grad(len(tets), _p(tets), _p(xyz), _p(u), _p(g))
+    return g, int(bad)
+
+
+def grad_nodes(tets, xyz, u):
+    """Assembler::grad_nodes: per-node float mean of the tet gradients."""
+    n = len(xyz)
+    off, ids = node_tets(n, tets)
+    g, _ = tet_grad(tets, xyz, u)
+    out = np.empty((n, 3), np.float32)
+    _o.ora_grad_nodes(n, _p(off), _p(ids), _p(g), _p(out))
+    return out

@@ -287,3 +287,64 @@ def test_triangle_mode_lipschitz_along_edges():
     e = m.edges
     le = np.linalg.norm(m.xyz[e[:, 0]].astype(np.float64) - m.xyz[e[:, 1]], axis=1)
     assert np.all(np.abs(d[e[:, 0]] - d[e[:, 1]]) <= le * (1 + 1e-12))
+
+
+# ---------------------------------------------------------------------------
+# Normals: Assembler::grad / grad_nodes (Assembler.cpp:11-71)
+# ---------------------------------------------------------------------------
+def test_node_tets_matches_sorted_pairs():
+    import fluidtissue_amd as fa
+    m = fa.kuhn_mesh((5, 4, 6))
+    off, ids = oracle.node_tets(m.n_nodes, m.tets)
+    pairs = sorted((int(v), t) for t in range(len(m.tets)) for v in m.tets[t])   # TetraMesh.cpp:117
+    assert off[-1] == len(pairs)
+    for i in range(m.n_nodes):
+        assert list(ids[off[i]:off[i + 1]]) == [t for v, t in pairs if v == i]
+
+
+def test_tet_grad_single_tet_against_linear_solve():
+    rng = np.random.default_rng(21)
+    for _ in range(200):
+        x = rng.uniform(-1, 1, size=(4, 3)).astype(np.float32)
+        u = rng.uniform(-2, 2, size=4)
+        g, bad = oracle.tet_grad(np.array([[0, 1, 2, 3]], np.int32), x, u)
+        B = (x[1:].astype(np.float64) - x[0]).T          # columns p_k - p_0
+        if abs(np.linalg.det(B)) < 1e-3:
+            continue
+        exact = np.linalg.solve(B.T, u[1:] - u[0])        # grad . (p_k - p_0) = u_k - u_0
+        assert bad == -1
+        assert np.allclose(g[0], exact, rtol=1e-5, atol=1e-6 * np.abs(exact).max())
+
+
+def test_tet_grad_singular_tet_is_reported():
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)   # coplanar
+    g, bad = oracle.tet_grad(np.array([[0, 1, 2, 3]], np.int32), x, np.arange(4.0))
+    assert bad == 0 and np.all(g == 0)
+
+
+def test_grad_nodes_linear_field_is_exact():
+    """A linear field has the same gradient on every tet and every node."""
+    import fluidtissue_amd as fa
+    m = fa.kuhn_mesh(9, jitter=0.25, seed=3)
+    a = np.array([0.3, -1.7, 2.25])
+    u = m.xyz.astype(np.float64) @ a + 0.125
+    g, bad = oracle.tet_grad(m.tets, m.xyz, u)
+    assert bad == -1
+    assert np.allclose(g, a.astype(np.float32), rtol=2e-6, atol=0)
+    n = oracle.grad_nodes(m.tets, m.xyz, u)
+    assert np.allclose(n, a.astype(np.float32), rtol=1e-5, atol=0)
+
+
+def test_grad_nodes_sphere_points_outward():
+    """Normals of a distance field to a sphere: unit and radial, to O(h)."""
+    import fluidtissue_amd as fa
+    m = fa.kuhn_mesh(24)
+    x = m.xyz.astype(np.float64) - 0.5
+    r = np.linalg.norm(x, axis=1)
+    n = oracle.grad_nodes(m.tets, m.xyz, r - 0.25).astype(np.float64)
+    h = 1.0 / 23
+    inner = (r > 4 * h) & (np.abs(x).max(axis=1) < 0.5 - 2 * h)      # off the centre and boundary
+    radial = x[inner] / r[inner, None]
+    norm = np.linalg.norm(n[inner], axis=1)
+    assert np.max(np.abs(norm - 1.0)) < 0.05                         # |grad| = 1 to O(h^2/r^2)
+    assert np.min(np.sum(n[inner] * radial, axis=1) / norm) > 0.999  # radial direction
