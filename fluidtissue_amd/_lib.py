@@ -109,6 +109,9 @@ _SIGS = {
     "ft_reinit_graph": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), _P, _P]),
     "ft_graph_stats": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ft_get_normals": (C.c_int, [_H, _P]),
+    "ft_interface_curvature": (C.c_int, [_H, _P, _P]),
+    "ft_get_counts": (C.c_int, [_H, _P]),
+    "ft_write_interface_obj": (C.c_int, [_H, C.c_char_p, C.c_int32]),
     "ft_normals_device": (C.c_int, [_H, C.POINTER(C.c_void_p)]),
     "ft_set_timing": (C.c_int, [_H, C.c_int32]),
     "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = ["ft_reinit.hip", "ft_batch.hip", "ft_gen.cpp", "EuclideanDistance.cpp"]
+SOURCES = ["ft_reinit.hip", "ft_gen.cpp", "ft_obj.cpp", "EuclideanDistance.cpp"]
 
 
 def _run(cmd):

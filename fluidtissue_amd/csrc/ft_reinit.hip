@@ -1308,6 +1308,51 @@ __global__ __launch_bounds__(kBS) void k_node_normals(const uint64_t* off, const
     out[3 * i + 2] = n2 / cnt;
 }
 
+// ---- interface curvature (LevelSet::getInterfaceCurvature, LevelSet.cpp:215-253)
+// g / |g| per node in float (|g| = sqrt(x^2 + (y^2 + z^2))), kept as three
+// double fields (g_x, g_y, g_z, :224-230).
+__global__ __launch_bounds__(kBS) void k_unit_split(const float* g, int64_t N, double* gx, double* gy,
+                                                    double* gz)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= N) return;
+    const float x = g[3 * i], y = g[3 * i + 1], z = g[3 * i + 2];
+    const float nrm = sqrtf(x * x + (y * y + z * z));
+    gx[i] = (double)(x / nrm);
+    gy[i] = (double)(y / nrm);
+    gz[i] = (double)(z / nrm);
+}
+
+// One component of grad_nodes (the float mean over the node's tets of that
+// component) added into k: k = mean on the first call, k += mean after, so
+// k = gg_x.x + gg_y.y + gg_z.z in that order (:237-238).
+__global__ __launch_bounds__(kBS) void k_node_grad_comp(const uint64_t* off, const int32_t* ids,
+                                                        const float* grad_t, int64_t N, int comp,
+                                                        int first, float* k)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= N) return;
+    const uint64_t b = off[i], e = off[i + 1];
+    float s = 0.0f;
+    for (uint64_t j = b; j < e; j++) s += grad_t[3 * (int64_t)ids[j] + comp];
+    const float m = s / (float)(e - b);
+    k[i] = first ? m : k[i] + m;
+}
+
+// Curvature at the crossing points (:243-250): t = float(phi0 / (phi0 - phi1))
+// from the blended phi, then (1.0 - t) k0 in double plus t k1 in float, the
+// sum in double rounded to float; in crossing-edge order (e2ifp_).
+__global__ __launch_bounds__(kBS) void k_curv_points(const int32_t* ifp2e, int64_t P, const int2* edges,
+                                                     const double* phi, const float* k, float* k_if)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= P) return;
+    const int2 e = edges[ifp2e[i]];
+    const float t = (float)(phi[e.x] / (phi[e.x] - phi[e.y]));
+    const float tk1 = t * k[e.y];
+    k_if[i] = (float)((1.0 - (double)t) * (double)k[e.x] + (double)tk1);
+}
+
 // ============================================================================
 // Per-field interface counts (batches): ranks at the field boundaries
 // ============================================================================
@@ -1532,6 +1577,8 @@ struct ft_ctx {
     DevBuf nearest, near2, fcounts, fclamp;
     // normals: node -> tets lists (per field layout), tet gradients, node normals
     DevBuf n2t_off, n2t_ids, grad_t, normals, phi_ifn_int;
+    DevBuf gsplit, kmesh, k_if;      // curvature: unit-normal components, node and point curvature
+    const double* last_phi = nullptr;   // device phi of the last reinit (blended)
     uint64_t n2t_layout = ~0ull;
     bool have_normals = false;
     // index
@@ -1572,7 +1619,8 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
             &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
-            &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int};
+            &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
+            &h->gsplit, &h->kmesh, &h->k_if};
 }
 
 std::vector<void*> buffer_addresses(ft_ctx* c)
@@ -2296,6 +2344,7 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     if (rc == FT_OK) rc = enqueue_search(c, s, sp, cp, true);
     if (rc == FT_OK) rc = collect(c, s, sp.want_nearest, &cnt);
     c->have_normals = rc == FT_OK && sp.want_normals;
+    c->last_phi = d_phi;
     if (counts) *counts = cnt;
     c->last = cnt;
     static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
@@ -2415,6 +2464,7 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
         g.launches++;
         const int rc = collect(c, s, o->want_nearest != 0, &cnt);
         c->have_normals = rc == FT_OK && o->want_normals != 0;
+        c->last_phi = d_phi;
         if (rc != FT_ERR_BUDGET) {
             if (counts) *counts = cnt;
             c->last = cnt;
@@ -2864,6 +2914,15 @@ int ft_reinit_graph(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts
     return rc;
 }
 
+int ft_get_counts(ft_handle h, ft_counts* total)
+{
+    FT_TRY(check_handle(h));
+    if (!total) return ft::set_error(FT_ERR_ARG, "null output");
+    if (!h->have_interface) return ft::set_error(FT_ERR_STATE, "no interface: reinit first");
+    *total = h->last;
+    return FT_OK;
+}
+
 int ft_get_normals(ft_handle h, float* normals)
 {
     FT_TRY(check_handle(h));
@@ -2880,6 +2939,40 @@ int ft_normals_device(ft_handle h, const float** normals)
     if (!normals) return ft::set_error(FT_ERR_ARG, "null output");
     if (!h->have_normals) return ft::set_error(FT_ERR_STATE, "no normals: reinit with want_normals first");
     *normals = h->normals.as<float>();
+    return FT_OK;
+}
+
+int ft_interface_curvature(ft_handle h, float* k_if, void* stream)
+{
+    FT_TRY(check_handle(h));
+    if (!k_if) return ft::set_error(FT_ERR_ARG, "null output");
+    if (!h->have_normals || !h->have_interface || !h->last_phi)
+        return ft::set_error(FT_ERR_STATE, "curvature needs a reinit with want_normals first");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const int64_t N = h->N, T = h->T, P = h->last.n_ifp;
+    FT_TRY(ensure(h->gsplit, 24 * at_least1(N)));
+    FT_TRY(ensure(h->kmesh, 4 * at_least1(N)));
+    FT_TRY(ensure(h->k_if, 4 * at_least1(P)));
+    if (N > 0) {
+        double* g = h->gsplit.as<double>();
+        const uint64_t* sc = h->scalars.as<uint64_t>();
+        hipLaunchKernelGGL(k_unit_split, dim3(grid1d(N)), dim3(kBS), 0, s, h->normals.as<float>(), N, g,
+                           g + N, g + 2 * N);
+        for (int c = 0; c < 3; c++) {
+            if (T > 0)
+                hipLaunchKernelGGL(k_tet_grad, dim3(grid1d(T)), dim3(kBS), 0, s, h->tets.as<int4>(), T,
+                                   h->xyz.as<float>(), g + c * N, h->grad_t.as<float>(), sc);
+            hipLaunchKernelGGL(k_node_grad_comp, dim3(grid1d(N)), dim3(kBS), 0, s,
+                               h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->grad_t.as<float>(),
+                               N, c, c == 0 ? 1 : 0, h->kmesh.as<float>());
+        }
+        if (P > 0)
+            hipLaunchKernelGGL(k_curv_points, dim3(grid1d(P)), dim3(kBS), 0, s, h->ifp2e.as<int32_t>(), P,
+                               h->edges.as<int2>(), h->last_phi, h->kmesh.as<float>(), h->k_if.as<float>());
+        FT_TRY(check_launch());
+    }
+    if (P > 0) FT_HIP(hipMemcpyAsync(k_if, h->k_if.p, 4 * P, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
     return FT_OK;
 }
 

@@ -326,6 +326,17 @@ class Reinit(_Handle):
         check(lib.ft_get_normals(self._h, _ptr(out)))
         return out
 
+    def curvature(self, stream=None):
+        """LevelSet::getInterfaceCurvature after a reinit with want_normals:
+        one float per crossing point, in crossing-edge order."""
+        out = np.empty(self.counts.n_ifp, np.float32)
+        check(lib.ft_interface_curvature(self._h, _ptr(out), C.c_void_p(stream) if stream else None))
+        return out
+
+    def write_obj(self, fname, unit_normals=True):
+        """fname.obj / fname.mtl of the last reinit's interface (cfish::Write_Wavefront_obj)."""
+        check(lib.ft_write_interface_obj(self._h, str(fname).encode(), 1 if unit_normals else 0))
+
     def nearest(self):
         out = np.empty(self.n, np.int32)
         check(lib.ft_get_nearest(self._h, _ptr(out)))

@@ -167,6 +167,25 @@ int ft_get_normals(ft_handle h, float* normals);
 /* Device address of the same (valid until the next call on h). */
 int ft_normals_device(ft_handle h, const float** normals);
 
+/* LevelSet::getInterfaceCurvature (LevelSet.cpp:215-253) after a reinit with
+ * want_normals = 1: the divergence of the unit normal field (grad_nodes of
+ * its three components, Assembler.cpp:46-71) at the mesh nodes, interpolated
+ * along each crossing edge with t from the reinit's (blended) phi; k_if has
+ * n_ifp entries in crossing-edge order.  The reinit's phi (host: the handle's
+ * device copy; device: the caller's pointer) must still hold that phi. */
+int ft_interface_curvature(ft_handle h, float* k_if, void* stream);
+
+/* Totals of the last successful reinit on h (sizes for ft_get_interface). */
+int ft_get_counts(ft_handle h, ft_counts* total);
+
+/* Wavefront OBJ of the last reinit's interface: fname.obj with the vertices,
+ * vertex normals (per point, the sum over its triangles of (p1-p0)x(p2-p0),
+ * divided by its norm when unit_normals) and 1-based faces, and fname.mtl
+ * with the materials table; the INTERFACE output of cfish::
+ * Write_Wavefront_obj (src/leveque/WriteData.cpp:9-61, :89-118; raw normals
+ * as src/growth/WriteData.cpp:48-82), text identical to its iostream output. */
+int ft_write_interface_obj(ft_handle h, const char* fname, int32_t unit_normals);
+
 /* Captures and launches of ft_reinit_graph on h so far (nullable). */
 int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches);
 

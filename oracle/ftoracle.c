@@ -462,3 +462,40 @@ void ora_grad_nodes(int64_t n_nodes, const int64_t* off, const int32_t* tet_ids,
         for (int k = 0; k < 3; k++) grad_n[3 * i + k] = n[k] / cnt;
     }
 }
+
+void ora_interface_curvature(int64_t n_nodes, const float* p, int64_t n_tets,
+                             const int32_t* tets, const int64_t* off,
+                             const int32_t* tet_ids, const float* g,
+                             const double* phi, int64_t n_ifp,
+                             const int32_t* ifp2e, const int32_t* edges,
+                             float* k_if)
+{
+    double* gc[3];
+    for (int c = 0; c < 3; c++) gc[c] = (double*)malloc(sizeof(double) * (size_t)(n_nodes > 0 ? n_nodes : 1));
+    for (int64_t i = 0; i < n_nodes; i++) {                      /* :224-230 */
+        const float x = g[3 * i], y = g[3 * i + 1], z = g[3 * i + 2];
+        const float nrm = sqrtf(x * x + (y * y + z * z));
+        gc[0][i] = (double)(x / nrm);
+        gc[1][i] = (double)(y / nrm);
+        gc[2][i] = (double)(z / nrm);
+    }
+    float* gt = (float*)malloc(sizeof(float) * 3 * (size_t)(n_tets > 0 ? n_tets : 1));
+    float* gn = (float*)malloc(sizeof(float) * 3 * (size_t)(n_nodes > 0 ? n_nodes : 1));
+    float* k = (float*)malloc(sizeof(float) * (size_t)(n_nodes > 0 ? n_nodes : 1));
+    for (int c = 0; c < 3; c++) {                                /* :233-239 */
+        ora_tet_grad(n_tets, tets, p, gc[c], gt);
+        ora_grad_nodes(n_nodes, off, tet_ids, gt, gn);
+        for (int64_t i = 0; i < n_nodes; i++) k[i] = c == 0 ? gn[3 * i] : k[i] + gn[3 * i + c];
+    }
+    for (int64_t i = 0; i < n_ifp; i++) {                        /* :243-250 */
+        const int32_t e = ifp2e[i];
+        const int32_t a = edges[2 * (int64_t)e], b = edges[2 * (int64_t)e + 1];
+        const float t = (float)(phi[a] / (phi[a] - phi[b]));
+        const float tk1 = t * k[b];
+        k_if[i] = (float)((1.0 - (double)t) * (double)k[a] + (double)tk1);
+    }
+    for (int c = 0; c < 3; c++) free(gc[c]);
+    free(gt);
+    free(gn);
+    free(k);
+}

@@ -125,6 +125,18 @@ int64_t ora_tet_grad(int64_t n_tets, const int32_t* tets, const float* p,
 void ora_grad_nodes(int64_t n_nodes, const int64_t* off, const int32_t* tet_ids,
                     const float* grad_t, float* grad_n);
 
+/* LevelSet::getInterfaceCurvature (LevelSet.cpp:215-253): from the node
+ * normals g (grad_nodes of phi_ifn), the unit-normal components as double
+ * fields, their grad_nodes, k = gg_x.x + gg_y.y + gg_z.z per node, then at
+ * each crossing point (crossing-edge order) t = float(phi0 / (phi0 - phi1))
+ * and k_if = float((1.0 - t) k0 + float(t k1)).  off/tet_ids: node -> tets. */
+void ora_interface_curvature(int64_t n_nodes, const float* p, int64_t n_tets,
+                             const int32_t* tets, const int64_t* off,
+                             const int32_t* tet_ids, const float* g,
+                             const double* phi, int64_t n_ifp,
+                             const int32_t* ifp2e, const int32_t* edges,
+                             float* k_if);
+
 #ifdef __cplusplus
 }
 #endif

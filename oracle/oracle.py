@@ -41,6 +41,9 @@ _o.ora_tet_grad.argtypes = [_i64, _P, _P, _P, _P]
 _o.ora_grad_nodes.restype = None
 _o.ora_grad_nodes.argtypes = [_i64, _P, _P, _P, _P]
 
+_o.ora_interface_curvature.restype = None
+_o.ora_interface_curvature.argtypes = [_i64, _P, _i64, _P, _P, _P, _P, _P, _i64, _P, _P, _P]
+
 NTHREADS = max(1, min(16, os.cpu_count() or 1))
 
 
@@ -190,4 +193,21 @@ def grad_nodes(tets, xyz, u):
     g, _ = tet_grad(tets, xyz, u)
     out = np.empty((n, 3), np.float32)
     _o.ora_grad_nodes(n, _p(off), _p(ids), _p(g), _p(out))
+    return out
+
+
+def interface_curvature(tets, xyz, normals, phi, ifp2e, edges):
+    """LevelSet::getInterfaceCurvature from the node normals (grad_nodes of
+    phi_ifn) and the blended phi; one value per crossing point."""
+    tets = _c(tets, np.int32)
+    xyz = _c(xyz, np.float32)
+    n = len(xyz)
+    off, ids = node_tets(n, tets)
+    g = _c(normals, np.float32)
+    phi = _c(phi, np.float64)
+    ifp2e = _c(ifp2e, np.int32)
+    edges = _c(edges, np.int32)
+    out = np.empty(len(ifp2e), np.float32)
+    _o.ora_interface_curvature(n, _p(xyz), len(tets), _p(tets), _p(off), _p(ids), _p(g), _p(phi),
+                               len(ifp2e), _p(ifp2e), _p(edges), _p(out))
     return out

@@ -67,3 +67,38 @@ def triangle_errors(name, d_gpu, f_gpu, nodes, h, ref=None, eps_tie=None):
         else:
             mismatch += 1
     return float(err.max()), float(np.sqrt((err ** 2).mean())), ties, mismatch
+
+
+def expected_interface_obj(ifp, tris, fname, unit_normals=True):
+    """The INTERFACE text of cfish::Write_Wavefront_obj (src/leveque/
+    WriteData.cpp:9-61) for an oracle interface, formatted as iostream prints
+    floats (%g, 6 digits); vertex normals in float32, products and sums one
+    rounding at a time."""
+    ifp = np.asarray(ifp, np.float32)
+    off, p2t = oracle.p2t_lists(tris, len(ifp))
+    f32 = np.float32
+    lines = [f"mtllib {fname}.mtl", "", "#", "# shape interface", "#", ""]
+    lines += [f"v {float(x):g} {float(y):g} {float(z):g}" for x, y, z in ifp]
+    lines.append("")
+    for i in range(len(ifp)):
+        n = [f32(0), f32(0), f32(0)]
+        for j in range(off[i], off[i + 1]):
+            a, b, c = (ifp[v] for v in tris[p2t[j], :3])
+            e1, e2 = b - a, c - a
+            n[0] = f32(n[0] + f32(f32(e1[1] * e2[2]) - f32(e1[2] * e2[1])))
+            n[1] = f32(n[1] + f32(f32(e1[2] * e2[0]) - f32(e1[0] * e2[2])))
+            n[2] = f32(n[2] + f32(f32(e1[0] * e2[1]) - f32(e1[1] * e2[0])))
+        if unit_normals:
+            nrm = f32(np.sqrt(f32(f32(n[0] * n[0]) + f32(f32(n[1] * n[1]) + f32(n[2] * n[2])))))
+            n = [f32(v / nrm) for v in n]
+        lines.append(f"vn {float(n[0]):g} {float(n[1]):g} {float(n[2]):g}")
+    lines += ["", "g interface", "usemtl 0"]
+    for t in tris:
+        a, b, c = (int(v) + 1 for v in t[:3])
+        lines.append(f"f {a}//{a} {b}//{b} {c}//{c}")
+    return "\n".join(lines) + "\n"
+
+
+EXPECTED_MTL = ("newmtl 0\nKa 0 0 0\nKd 0.6 0.6 0.6\nd 1\nKs 0 0 0\nNs 0 interface\n\n"
+                "newmtl 1\nKa 0 0 0\nKd 0.0 0.0 1.0\nd 0.2\nKs 0 0 0\nNs 0 boundary\n"
+                "newmtl 2\nKa 0 0 0\nKd 1.0 0.0 0.0\nd 1\nKs 0 0 0\nNs 0 knots\n")

@@ -77,3 +77,50 @@ def test_normals_need_a_reinit_with_them():
     with pytest.raises(fa._lib.FtError):
         r.normals()
     r.close()
+
+
+@pytest.mark.parametrize("name,mode,blend", [("cfg1", "points", "update"), ("ellipsoids", "triangles", "init"),
+                                             ("cfg2s", "triangles", "update")])
+def test_interface_curvature_matches_oracle(name, mode, blend):
+    """LevelSet::getInterfaceCurvature (LevelSet.cpp:215-253) on the device,
+    bit for bit against the oracle from the same reinit outputs."""
+    m, phi, h = workload(name)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, cnt = r.reset_phi(phi, mode=mode, blend=blend, want_normals=True)
+    g = r.normals()
+    k = r.curvature()
+    itf = r.interface()
+    r.close()
+    ref = oracle.interface_curvature(m.tets, m.xyz, g, out, itf.ifp2e, m.edges)
+    assert len(k) == cnt.n_ifp
+    assert np.array_equal(k.view(np.uint32), ref.view(np.uint32))
+
+
+def test_interface_curvature_sphere_on_device():
+    m = fa.kuhn_mesh(48, jitter=0.1, seed=5)
+    R = 0.27
+    phi = np.linalg.norm(m.xyz.astype(np.float64) - 0.5, axis=1) - R
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    r.reset_phi(phi, mode="triangles", want_normals=True)
+    k = r.curvature()
+    r.close()
+    assert abs(np.median(k) - 2 / R) < 0.05 * 2 / R
+
+
+@pytest.mark.parametrize("unit", [True, False])
+def test_interface_obj_matches_reference_text(tmp_path, unit):
+    """The OBJ export of the interface, byte for byte the text the reference's
+    writer prints for the oracle's interface."""
+    from tests.helpers import EXPECTED_MTL, expected_interface_obj, oracle_interface
+    m, phi, h = workload("cfg1")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    r.reset_phi(phi, mode="triangles")
+    base = str(tmp_path / "shape")
+    r.write_obj(base, unit_normals=unit)
+    r.close()
+    ref = oracle_interface("cfg1")
+    assert open(base + ".obj").read() == expected_interface_obj(ref["ifp"], ref["tris"], base, unit)
+    assert open(base + ".mtl").read() == EXPECTED_MTL

@@ -348,3 +348,29 @@ def test_grad_nodes_sphere_points_outward():
     norm = np.linalg.norm(n[inner], axis=1)
     assert np.max(np.abs(norm - 1.0)) < 0.05                         # |grad| = 1 to O(h^2/r^2)
     assert np.min(np.sum(n[inner] * radial, axis=1) / norm) > 0.999  # radial direction
+
+
+def test_interface_curvature_sphere():
+    """Curvature of a sphere of radius R: div of the unit normal = 2/R at the
+    crossing points, to O(h) on this mesh."""
+    import fluidtissue_amd as fa
+    m = fa.kuhn_mesh(40)
+    R = 0.3
+    phi = np.linalg.norm(m.xyz.astype(np.float64) - 0.5, axis=1) - R
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=0)
+    g = oracle.grad_nodes(m.tets, m.xyz, ref["phi_ifn"])
+    k = oracle.interface_curvature(m.tets, m.xyz, g, ref["phi"], ref["ifp2e"], m.edges)
+    assert len(k) == len(ref["ifp"])
+    assert abs(np.median(k) - 2 / R) < 0.05 * 2 / R
+    assert np.percentile(np.abs(k - 2 / R), 90) < 0.25 * 2 / R
+
+
+def test_interface_curvature_plane_is_zero():
+    import fluidtissue_amd as fa
+    m = fa.kuhn_mesh(16, jitter=0.2, seed=4)
+    x = m.xyz.astype(np.float64)
+    phi = x @ np.array([0.3, 0.5, 0.8]) / np.linalg.norm([0.3, 0.5, 0.8]) - 0.61
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=0)
+    k = oracle.interface_curvature(m.tets, m.xyz, oracle.grad_nodes(m.tets, m.xyz, phi), ref["phi"],
+                                   ref["ifp2e"], m.edges)
+    assert np.max(np.abs(k)) < 1e-4        # linear phi: constant unit normal, zero divergence
