@@ -250,12 +250,13 @@ __device__ inline double tri_dist2_f64(const double* x, const double* a, const d
 // Triangle record (80 B), an orthonormal frame at vertex a, built in FP64 when
 // the index is built, with u along ab (the longest edge) and n the unit
 // normal; in it the triangle is (0,0), (B,0), (Cx,Cy), Cy >= 0:
-//   F0 = (a, B)  F1 = (u, Cx)  F2 = (v, Cy)  F3 = (n, 1/B)
-//   F4 = (1/|ac|^2, 1/|bc|^2, Cx - B, +-R)
+//   F0 = (a, +-R)  F1 = (u, B)  F2 = (v, n.x)  F3 = (n.y, n.z, Cx, Cy)
+//   F4 = (1/B, 1/|ac|^2, 1/|bc|^2, Cx - B)
 // (inverse lengths are 0 for zero-length edges; R is the radius, rounded up,
 // of the disk about the midpoint of ab that holds the triangle, negative for
 // collinear vertices, which have no interior and whose distance is that of
-// the edge segments).  The point is
+// the edge segments).  The culling test reads the first 14 dwords only, one
+// contiguous run of scalar loads.  The point is
 // projected once (X, Y, Z); the in-plane distance is 0 inside the triangle,
 // else the nearest clamped edge projection; d^2 = d2D^2 + Z^2.  With ab the
 // longest edge the 2D problem stays well conditioned however thin the
@@ -274,16 +275,16 @@ __device__ __forceinline__ TriProj tri_project(float x, float y, float z, const 
     TriProj p;
     p.X = __fmaf_rn(wz, r.F1.z, __fmaf_rn(wy, r.F1.y, wx * r.F1.x));
     p.Y = __fmaf_rn(wz, r.F2.z, __fmaf_rn(wy, r.F2.y, wx * r.F2.x));
-    p.Z = __fmaf_rn(wz, r.F3.z, __fmaf_rn(wy, r.F3.y, wx * r.F3.x));
+    p.Z = __fmaf_rn(wz, r.F3.y, __fmaf_rn(wy, r.F3.x, wx * r.F2.w));
     return p;
 }
 
 // Lower bound from the disk about the midpoint of ab that holds the triangle
-// (radius |F4.w|); costs little once the point is projected.
+// (radius |F0.w|); costs little once the point is projected.
 __device__ __forceinline__ float tri_disk_lb2(const TriProj& p, const TriRec& r)
 {
-    const float rho = fabsf(r.F4.w) * (1.0f + 0x1.0p-18f);
-    const float mx = __fmaf_rn(-0.5f, r.F0.w, p.X);
+    const float rho = fabsf(r.F0.w) * (1.0f + 0x1.0p-18f);
+    const float mx = __fmaf_rn(-0.5f, r.F1.w, p.X);
     const float lat = sqrt_lb(__fmaf_rn(p.Y, p.Y, mx * mx));
     const float g = fmaxf(lat - rho, 0.0f) * (1.0f - 0x1.0p-18f);
     return __fmaf_rn(g, g, p.Z * p.Z) * (1.0f - 0x1.0p-18f);
@@ -292,21 +293,21 @@ __device__ __forceinline__ float tri_disk_lb2(const TriProj& p, const TriRec& r)
 __device__ __forceinline__ float tri_dist2_from(const TriProj& p, const TriRec& r)
 {
     const float X = p.X, Y = p.Y, Z = p.Z;
-    const float B = r.F0.w, Cx = r.F1.w, Cy = r.F2.w, dCx = r.F4.z;
+    const float B = r.F1.w, Cx = r.F3.z, Cy = r.F3.w, dCx = r.F4.w;
     const float xb = X - B;
     const bool inside = (int)(Y >= 0.0f) & (int)(__fmaf_rn(Cy, X, -Cx * Y) >= 0.0f) &
-                        (int)(__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f) & (int)(r.F4.w > 0.0f);
+                        (int)(__fmaf_rn(dCx, Y, -Cy * xb) >= 0.0f) & (int)(r.F0.w > 0.0f);
     // edge ab
-    float t = fminf(fmaxf(X * r.F3.w, 0.0f), 1.0f);
+    float t = fminf(fmaxf(X * r.F4.x, 0.0f), 1.0f);
     float ex = __fmaf_rn(-t, B, X);
     float d = __fmaf_rn(Y, Y, ex * ex);
     // edge ac
-    t = fminf(fmaxf(__fmaf_rn(Y, Cy, X * Cx) * r.F4.x, 0.0f), 1.0f);
+    t = fminf(fmaxf(__fmaf_rn(Y, Cy, X * Cx) * r.F4.y, 0.0f), 1.0f);
     ex = __fmaf_rn(-t, Cx, X);
     float ey = __fmaf_rn(-t, Cy, Y);
     d = fminf(d, __fmaf_rn(ey, ey, ex * ex));
     // edge bc
-    t = fminf(fmaxf(__fmaf_rn(Y, Cy, xb * dCx) * r.F4.y, 0.0f), 1.0f);
+    t = fminf(fmaxf(__fmaf_rn(Y, Cy, xb * dCx) * r.F4.z, 0.0f), 1.0f);
     ex = __fmaf_rn(-t, dCx, xb);
     ey = __fmaf_rn(-t, Cy, Y);
     d = fminf(d, __fmaf_rn(ey, ey, ex * ex));

@@ -688,15 +688,15 @@ __global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, const uint64_t* 
     const double dCx = Cx - L0;
     const double lac = Cx * Cx + Cy * Cy, lbc = dCx * dCx + Cy * Cy;
     TriRec r;
-    r.F0 = make_float4(a[0], a[1], a[2], (float)L0);
-    r.F1 = make_float4((float)u[0], (float)u[1], (float)u[2], (float)Cx);
-    r.F2 = make_float4((float)v[0], (float)v[1], (float)v[2], (float)Cy);
-    r.F3 = make_float4((float)nu[0], (float)nu[1], (float)nu[2], L0 > 0.0 ? (float)(1.0 / L0) : 0.0f);
     // disk about the midpoint of ab holding the triangle (ab is the longest
     // edge), radius rounded up; its sign carries the interior flag
     const double Rm = fmax(0.5 * L0, sqrt((Cx - 0.5 * L0) * (Cx - 0.5 * L0) + Cy * Cy)) * (1.0 + 1e-6);
-    r.F4 = make_float4(lac > 0.0 ? (float)(1.0 / lac) : 0.0f, lbc > 0.0 ? (float)(1.0 / lbc) : 0.0f,
-                       (float)dCx, interior ? (float)Rm : -(float)Rm);
+    r.F0 = make_float4(a[0], a[1], a[2], interior ? (float)Rm : -(float)Rm);
+    r.F1 = make_float4((float)u[0], (float)u[1], (float)u[2], (float)L0);
+    r.F2 = make_float4((float)v[0], (float)v[1], (float)v[2], (float)nu[0]);
+    r.F3 = make_float4((float)nu[1], (float)nu[2], (float)Cx, (float)Cy);
+    r.F4 = make_float4(L0 > 0.0 ? (float)(1.0 / L0) : 0.0f, lac > 0.0 ? (float)(1.0 / lac) : 0.0f,
+                       lbc > 0.0 ? (float)(1.0 / lbc) : 0.0f, (float)dCx);
     trec[slot] = r;
 }
 
