@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-BIN_SCALES = (0.75, 1.0, 1.5, 2.0, 3.0)
+BIN_SCALES = (0.75, 1.0, 1.25, 1.5, 2.0, 3.0)
 QUERY_BLOCKS = (64, 128, 256)
 
 _cache = {}

@@ -986,6 +986,12 @@ struct Searcher {
     const int4* tris;
 
     __device__ __forceinline__ float best2() const { return MODE == 0 ? best * best : best; }
+    float bnd;       // prune_bound(best2()), refreshed whenever best changes
+    __device__ __forceinline__ void set_best(float d)
+    {
+        best = d;
+        bnd = prune_bound(best2());
+    }
 
     // Mode 0 keeps the nearest point (ties: lowest id, the reference's strict
     // < over ascending ids).  Mode 1 keeps the best two triangles by FP32
@@ -995,13 +1001,13 @@ struct Searcher {
         if (MODE == 0) {
             if (d < best || (d == best && d < INFINITY)) {
                 const int32_t j = pidx[s];
-                if (d < best || bj < 0 || j < bj) { best = d; bj = j; }
+                if (d < best || bj < 0 || j < bj) { set_best(d); bj = j; }
             }
         } else if (d < sec || (d == sec && d < INFINITY)) {
             const int32_t j = pidx[s];
             if (d < best || (d == best && (bj < 0 || j < bj))) {
                 if (bj >= 0 && j != bj) { sec = best; sj = bj; }
-                best = d;
+                set_best(d);
                 bj = j;
             } else if (j != bj && (d < sec || sj < 0 || j < sj)) {
                 sec = d;
@@ -1019,7 +1025,7 @@ struct Searcher {
     __device__ __forceinline__ void test_tri(const TriRec& r, uint32_t s)
     {
         const ftd::TriProj pj = ftd::tri_project(x, y, z, r);
-        if (__ballot(ftd::tri_disk_lb2(pj, r) <= prune_bound(best2())) == 0ull) return;
+        if (__ballot(ftd::tri_disk_lb2(pj, r) <= bnd) == 0ull) return;
         consider(ftd::tri_dist2_from(pj, r), s);
     }
 
@@ -1162,8 +1168,8 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
     Searcher<MODE> S;
     S.x = px; S.y = py; S.z = pz;
     S.bj = -1;
-    if (MODE == 0) S.best = a.band > 0.0f ? a.band : FLT_MAX;
-    else S.best = a.band > 0.0f ? a.band * a.band : INFINITY;
+    if (MODE == 0) S.set_best(a.band > 0.0f ? a.band : FLT_MAX);
+    else S.set_best(a.band > 0.0f ? a.band * a.band : INFINITY);
     S.sec = S.best;
     S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
     S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
@@ -1191,7 +1197,7 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
         while (--budget > 0) {
             const BNode nd = ftd::cload(a.nodes, cur);    // scalar loads: cur is uniform
             const int left = nd.left, right = nd.right;
-            const float bnd = prune_bound(S.best2());
+            const float bnd = S.bnd;
             // box bound first; the cylinder only where some lane's box admits it
             float ll = ftd::box_dist2(px, py, pz, nd.l.lo, nd.l.hi);
             float lr = ftd::box_dist2(px, py, pz, nd.r.lo, nd.r.hi);
