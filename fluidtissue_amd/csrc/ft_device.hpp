@@ -155,7 +155,7 @@ __device__ __forceinline__ float box_dist2(float x, float y, float z, const floa
 // A patch of a smooth surface lies in a thin cylinder around its normal, so
 // for a node far away the cylinder bound is close to the true distance where
 // the box bound is loose by about the patch size.  tau < 0: no cylinder.
-struct Bound {
+struct __align__(16) Bound {
     float lo[3], hi[3];
     float c[3], n[3];
     float rho, tau;

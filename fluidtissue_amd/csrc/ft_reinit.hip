@@ -806,21 +806,23 @@ __global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, const uint
     if (leaf_off[f] == lo && leaf_off[f + 1] == hi + 1) roots[f] = i;
 }
 
+// Child bounds of a node (16-byte aligned in BNode), 4 x 128-bit.  The
+// agent-scope fences around the arrival counter in k_refit order them: the
+// writer's fence (release) completes its stores before its increment, the
+// reader's fence (acquire) invalidates its vector L1 before it loads.
 __device__ __forceinline__ void st_bound(Bound* dst, const Bound& v)
 {
-    const unsigned* src = reinterpret_cast<const unsigned*>(&v);
-    unsigned* d = reinterpret_cast<unsigned*>(dst);
-    for (int k = 0; k < 16; k++)
-        __hip_atomic_store(&d[k], src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float4* src = reinterpret_cast<const float4*>(&v);
+    float4* d = reinterpret_cast<float4*>(dst);
+    for (int k = 0; k < 4; k++) d[k] = src[k];
 }
 
 __device__ __forceinline__ Bound ld_bound(const Bound* srcp)
 {
     Bound v;
-    const unsigned* s = reinterpret_cast<const unsigned*>(srcp);
-    unsigned* d = reinterpret_cast<unsigned*>(&v);
-    for (int k = 0; k < 16; k++)
-        d[k] = __hip_atomic_load(&s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float4* s = reinterpret_cast<const float4*>(srcp);
+    float4* d = reinterpret_cast<float4*>(&v);
+    for (int k = 0; k < 4; k++) d[k] = s[k];
     return v;
 }
 
@@ -1016,7 +1018,7 @@ struct Searcher {
         }
     }
 
-    uint32_t n_leaves = 0, n_prims = 0;
+    uint32_t n_leaves = 0, n_prims = 0, n_pairs = 0;   // statistics (FT_QUERY_STATS)
     float sec;       // mode 1: second best squared distance and its triangle
     int32_t sj = -1;
 
@@ -1030,11 +1032,12 @@ struct Searcher {
     }
 
     // Leaf test at a wave-uniform leaf: records read through scalar loads.
-    __device__ void test_leaf_uniform(int l)
+    __device__ void test_leaf_uniform(int l, uint64_t admit = ~0ull)
     {
         const uint32_t b = ftd::cload(leaf_start, l), e = b + ftd::cload(leaf_cnt, l);
         n_leaves++;
         n_prims += e - b;
+        n_pairs += (uint32_t)__popcll(admit) * (e - b);   // lane-triangle pairs whose lane admits the leaf
         if (MODE == 0) {
             for (uint32_t s = b; s < e; s++) {
                 const float4 p = ftd::cload(prec, s);
@@ -1139,11 +1142,12 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
     const float hh = a.h_est;
     int bkt = dd < 2 * hh ? 0 : dd < 8 * hh ? 1 : dd < 32 * hh ? 2 : 3;
     if (a.phi_old[node] < 0.0) bkt += 4;
-    unsigned long long* st = a.stats + 4 * bkt;
+    unsigned long long* st = a.stats + 5 * bkt;
     atomicAdd(&st[0], 1ull);
     atomicAdd(&st[1], (unsigned long long)n_nodes);
     atomicAdd(&st[2], (unsigned long long)S.n_leaves);
     atomicAdd(&st[3], (unsigned long long)S.n_prims);
+    atomicAdd(&st[4], (unsigned long long)S.n_pairs);
 }
 
 // Wave-coherent search: the 64 lanes of a wave hold 64 consecutive slots of
@@ -1211,11 +1215,11 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
             const uint64_t m0 = left_first ? mL : mR, m1 = left_first ? mR : mL;
             int next = -1;
             if (m0) {
-                if (c0 < 0) S.test_leaf_uniform(~c0);
+                if (c0 < 0) S.test_leaf_uniform(~c0, m0);
                 else next = c0;
             }
             if (m1) {
-                if (c1 < 0) S.test_leaf_uniform(~c1);
+                if (c1 < 0) S.test_leaf_uniform(~c1, m1);
                 else if (next < 0) next = c1;
                 else if (sp < kStack) wstack[w][sp++] = c1;
                 else atomicOr(a.overflow, 1u);
@@ -1622,6 +1626,11 @@ struct ft_ctx {
     int64_t idx_G = 0;               // bins over all fields
     int idx_code_bits = 0;
     GraphState graph;
+    // capacities for the next eager call on the same fields and mode
+    Caps spec_caps;
+    uint64_t spec_layout = ~0ull;
+    int spec_mode = -1;
+    int64_t spec_hits = 0;
     // scan scratch and small scalars
     DevBuf scan_bsum, scalars;
     uint64_t* h_scalars = nullptr;   // pinned mirror
@@ -2047,13 +2056,14 @@ struct SearchSpec {
 // k_check_caps guards them.  counts_only: stops once the totals are known
 // (phi is not written).
 int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_nearest,
-                     bool want_normals, Caps& cp, bool eager, bool counts_only, ft_counts* cnt)
+                     bool want_normals, Caps& cp, bool eager, bool counts_only, ft_counts* cnt,
+                     bool timed)
 {
     const int64_t N = c->N, T = c->T, E = c->E;
     uint64_t* sc = c->scalars.as<uint64_t>();
     const int64_t nw_nodes = (N + 63) / 64, nw_tets = (T + 63) / 64, nw_edges = (E + 63) / 64;
     const int64_t nblk = (T + kBS - 1) / kBS;
-    const bool timed = eager && c->timing && !counts_only;
+    timed = timed && !counts_only;
     FT_TRY(ensure_stage(c, 0, cp, mode, want_nearest, want_normals));
     uint64_t* sbits = c->sbits.as<uint64_t>();
     if (timed) (void)hipEventRecord(c->ev[0], s);
@@ -2131,21 +2141,20 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
                            c->fcounts.as<int64_t>());
     FT_TRY(check_launch());
     if (timed) (void)hipEventRecord(c->ev[1], s);
-    if (eager) {
-        FT_TRY(read_scalars(c, s, S_N));
-        FT_TRY(stage_error(c->h_scalars[S_ERR], cnt));
-    }
+    // a degenerate tet found by k_tris halts the rest on the device; the final
+    // readback reports it
     return FT_OK;
 }
 
 // Enqueues the bin index over the primitives (crossing points c->ifp, or
 // triangles c->tris) and the distance query with its epilogue.  The number of
 // primitives is read on the device (S_EDGES or S_BLK); cp bounds it.
-int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& cp, bool eager)
+int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& cp, bool eager,
+                   bool timed)
 {
     const int mode = sp.mode;
     uint64_t* sc = c->scalars.as<uint64_t>();
-    const bool timed = eager && c->timing;
+    (void)eager;
     const int64_t n_cap = mode == 0 ? cp.P : cp.F;
     const float* ifp = c->ifp.as<float>();
     const int4* tris = mode == 0 ? nullptr : c->tris.as<int4>();
@@ -2221,8 +2230,8 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     static const bool want_stats = getenv("FT_QUERY_STATS") != nullptr;
     q.h_est = c->h_est;
     if (want_stats && eager) {
-        FT_TRY(ensure(c->stats, 8 * 32));
-        FT_TRY(fill<uint64_t>(s, c->stats.as<uint64_t>(), 32, 0ull));
+        FT_TRY(ensure(c->stats, 8 * 40));
+        FT_TRY(fill<uint64_t>(s, c->stats.as<uint64_t>(), 40, 0ull));
         q.stats = c->stats.as<unsigned long long>();
     }
     q.overflow = (unsigned*)(sc + S_OVF);
@@ -2309,20 +2318,24 @@ int collect(ft_ctx* c, hipStream_t s, bool want_nearest, ft_counts* cnt)
 
 void print_query_stats(ft_ctx* c, int mode)
 {
-    uint64_t st[32];
+    uint64_t st[40];
     if (hipMemcpy(st, c->stats.p, sizeof st, hipMemcpyDeviceToHost) != hipSuccess) return;
     static const char* names[8] = {"out <2h", "out 2-8h", "out 8-32h", "out >32h",
                                    "in <2h", "in 2-8h", "in 8-32h", "in >32h"};
-    double tn = 0, tl = 0, tp = 0;
-    for (int b = 0; b < 8; b++) { tn += st[4 * b + 1]; tl += st[4 * b + 2]; tp += st[4 * b + 3]; }
-    fprintf(stderr, "[ftreinit] query mode %d: N=%lld per node: tree nodes %.1f, leaves %.1f, "
-            "primitives %.1f\n", mode, (long long)c->N, tn / c->N, tl / c->N, tp / c->N);
+    double tn = 0, tl = 0, tp = 0, tq = 0;
     for (int b = 0; b < 8; b++) {
-        const double nn = (double)st[4 * b];
+        tn += st[5 * b + 1]; tl += st[5 * b + 2]; tp += st[5 * b + 3]; tq += st[5 * b + 4];
+    }
+    fprintf(stderr, "[ftreinit] query mode %d: N=%lld per node: tree nodes %.1f, leaves %.1f, "
+            "primitives %.1f, lanes admitting a tested leaf %.1f%%\n", mode, (long long)c->N,
+            tn / c->N, tl / c->N, tp / c->N, tp > 0 ? 100.0 * tq / (64.0 * tp) : 0.0);
+    for (int b = 0; b < 8; b++) {
+        const double nn = (double)st[5 * b];
         if (nn > 0)
-            fprintf(stderr, "[ftreinit]   %-9s %5.1f%% of nodes: tree %.1f leaves %.1f prims %.1f\n",
-                    names[b], 100.0 * nn / c->N, st[4 * b + 1] / nn, st[4 * b + 2] / nn,
-                    st[4 * b + 3] / nn);
+            fprintf(stderr, "[ftreinit]   %-9s %5.1f%% of nodes: tree %.1f leaves %.1f prims %.1f "
+                    "admitting lanes %.1f%%\n", names[b], 100.0 * nn / c->N, st[5 * b + 1] / nn,
+                    st[5 * b + 2] / nn, st[5 * b + 3] / nn,
+                    st[5 * b + 3] > 0 ? 100.0 * st[5 * b + 4] / (64.0 * st[5 * b + 3]) : 0.0);
     }
 }
 
@@ -2361,6 +2374,9 @@ int validate_opts(const ft_opts* o)
     return FT_OK;
 }
 
+// Capacity with headroom over a measured stage total.
+inline int64_t with_room(int64_t v) { return v + v / 4 + 256; }
+
 int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
                   ft_counts* counts, hipStream_t s)
 {
@@ -2376,10 +2392,37 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     Caps cp;
     const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
     if (rc == FT_OK && sp.want_normals) rc = build_node_tets(c, s);
-    if (rc == FT_OK)
-        rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cp, true, false, &cnt);
-    if (rc == FT_OK) rc = enqueue_search(c, s, sp, cp, true);
-    if (rc == FT_OK) rc = collect(c, s, sp.want_nearest, &cnt);
+    // Speculative pass: the capacities of the last call on these fields, no
+    // readback between stages (the device guards them, and halts before phi is
+    // written if a stage outgrows them); else, or then, the exact pass, which
+    // reads the stage totals back to size the buffers.
+    const bool spec = rc == FT_OK && c->spec_layout == c->layout_gen && c->spec_mode == sp.mode;
+    if (spec) {
+        cp = c->spec_caps;
+        rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cp, false, false,
+                              &cnt, c->timing);
+        if (rc == FT_OK) rc = enqueue_search(c, s, sp, cp, false, c->timing);
+        if (rc == FT_OK) rc = collect(c, s, sp.want_nearest, &cnt);
+        c->spec_hits += rc != FT_ERR_BUDGET;
+    }
+    if (!spec || rc == FT_ERR_BUDGET) {
+        cp = Caps{};
+        cnt = ft_counts{};
+        cnt.bad_tet = -1;
+        rc = index_setup(c, s, o->bin_scale);
+        if (rc == FT_OK)
+            rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cp, true, false,
+                                  &cnt, c->timing);
+        if (rc == FT_OK) rc = enqueue_search(c, s, sp, cp, true, c->timing);
+        if (rc == FT_OK) rc = collect(c, s, sp.want_nearest, &cnt);
+        if (rc == FT_OK) {
+            c->spec_caps.X = with_room(cp.X);
+            c->spec_caps.F = with_room(cp.F);
+            c->spec_caps.P = with_room(cp.P);
+            c->spec_layout = c->layout_gen;
+            c->spec_mode = sp.mode;
+        }
+    }
     c->have_normals = rc == FT_OK && sp.want_normals;
     c->last_phi = d_phi;
     if (counts) *counts = cnt;
@@ -2399,8 +2442,6 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
 // ----------------------------------------------------------------------------
 // Graph form: the whole reinit of the resident fields as one captured graph
 // ----------------------------------------------------------------------------
-inline int64_t with_room(int64_t v) { return v + v / 4 + 256; }
-
 bool same_opts(const ft_opts& a, const ft_opts& b)
 {
     return a.mode == b.mode && a.blend == b.blend && a.w_phi == b.w_phi && a.w_ifn == b.w_ifn &&
@@ -2429,8 +2470,8 @@ int capture_reinit(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o
     Caps cc = cp;
     ft_counts unused{};
     int rc = enqueue_marching(c, c->stream, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cc, false,
-                              false, &unused);
-    if (rc == FT_OK) rc = enqueue_search(c, c->stream, sp, cc, false);
+                              false, &unused, false);
+    if (rc == FT_OK) rc = enqueue_search(c, c->stream, sp, cc, false, false);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc != FT_OK) {
@@ -2485,7 +2526,7 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
         Caps exact;
         const int mode = o->mode == FT_DIST_POINTS ? 0 : 1;
         const int rc = enqueue_marching(c, s, d_phi, mode, o->want_nearest != 0, o->want_normals != 0,
-                                        exact, true, true, &cnt);
+                                        exact, true, true, &cnt, false);
         if (rc != FT_OK) {
             if (counts) *counts = cnt;
             c->last = cnt;
@@ -3051,7 +3092,7 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     FT_TRY(fill<uint64_t>(s, sc + S_ERR, 1, ~0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_CAP, 1, 0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
-    FT_TRY(enqueue_search(h, s, sp, cp, true));
+    FT_TRY(enqueue_search(h, s, sp, cp, true, false));
     ft_counts cnt{};
     FT_TRY(collect(h, s, true, &cnt));
     h->have_interface = false;        // points only: no triangles, p2t or areas

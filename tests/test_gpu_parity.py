@@ -306,3 +306,27 @@ def test_cpp_drop_in_link(tmp_path):
     xyz = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
     ref, _ = oracle.reset_phi_points(xyz, ifp, phi)
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_repeated_calls_outgrowing_capacities():
+    """An eager call sized from the previous call's totals that finds a larger
+    interface halts on the device, leaves phi alone and reruns exactly: every
+    call equals the same call on a fresh handle."""
+    m = fa.kuhn_mesh(30, jitter=0.2, seed=8)
+    x = m.xyz.astype(np.float64)
+    small = np.linalg.norm(x - 0.5, axis=1) - 0.12
+    big = fa.phi_field("spheres", m.xyz, seed=3)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    for k, phi in enumerate([small, small, big, big, small, big]):
+        for mode in ("triangles", "points"):
+            out, ifn, cnt = r.reset_phi(phi, mode=mode, blend="update", want_nearest=True)
+            f = fa.Reinit(0)
+            f.set_mesh(m)
+            o2, i2, c2 = f.reset_phi(phi, mode=mode, blend="update", want_nearest=True)
+            assert (cnt.n_ifp, cnt.n_tri) == (c2.n_ifp, c2.n_tri), (k, mode)
+            assert np.array_equal(out.view(np.uint64), o2.view(np.uint64)), (k, mode)
+            assert np.array_equal(ifn.view(np.uint64), i2.view(np.uint64)), (k, mode)
+            assert np.array_equal(r.nearest(), f.nearest()), (k, mode)
+            f.close()
+    r.close()
