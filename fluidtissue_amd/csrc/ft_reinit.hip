@@ -812,32 +812,41 @@ __global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, const uint
 // reader's fence (acquire) invalidates its vector L1 before it loads.
 __device__ __forceinline__ void st_bound(Bound* dst, const Bound& v)
 {
-    const float4* src = reinterpret_cast<const float4*>(&v);
-    float4* d = reinterpret_cast<float4*>(dst);
-    for (int k = 0; k < 4; k++) d[k] = src[k];
+    const unsigned* src = reinterpret_cast<const unsigned*>(&v);
+    unsigned* d = reinterpret_cast<unsigned*>(dst);
+    for (int k = 0; k < 16; k++)
+        __hip_atomic_store(&d[k], src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ Bound ld_bound(const Bound* srcp)
 {
     Bound v;
-    const float4* s = reinterpret_cast<const float4*>(srcp);
-    float4* d = reinterpret_cast<float4*>(&v);
-    for (int k = 0; k < 4; k++) d[k] = s[k];
+    const unsigned* s = reinterpret_cast<const unsigned*>(srcp);
+    unsigned* d = reinterpret_cast<unsigned*>(&v);
+    for (int k = 0; k < 16; k++)
+        d[k] = __hip_atomic_load(&s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return v;
 }
 
 // Parent bound of two children: box union and a cylinder that contains both
 // child cylinders.  For a child cylinder (c_i, n_i, rho_i, tau_i) seen from
-// the parent axis n at centre c, with cos = |n.n_i| and sin = sqrt(1-cos^2):
+// the parent axis n at centre c, with cos = |n.n_i| and sin = |n x n_i|:
 //   thickness  |n.(c_i-c)| + tau_i cos + rho_i sin
 //   radius     |n x (c_i-c)| + rho_i + tau_i sin
+// In FP32 with hardware square roots (the merge is on the refit's critical
+// path, one per tree level): every term is a sum of non-negative parts with a
+// relative error under 1e-6, the axis is unit to 3e-7, and the results are
+// rounded up by 2^-12 relative plus 2^-18 of the box extent, so the cylinder
+// still contains both children.
 __device__ inline Bound merge_bounds(const Bound& A, const Bound& Bb)
 {
     Bound P;
+    float ext = 0.0f;
     for (int k = 0; k < 3; k++) {
         P.lo[k] = fminf(A.lo[k], Bb.lo[k]);
         P.hi[k] = fmaxf(A.hi[k], Bb.hi[k]);
         P.c[k] = 0.5f * (P.lo[k] + P.hi[k]);
+        ext = fmaxf(ext, P.hi[k] - P.lo[k]);
     }
     P.pad[0] = P.pad[1] = 0.0f;
     if (A.tau < 0.0f || Bb.tau < 0.0f) {
@@ -845,27 +854,35 @@ __device__ inline Bound merge_bounds(const Bound& A, const Bound& Bb)
         P.rho = 0.0f; P.tau = -1.0f;
         return P;
     }
-    const double wa = (double)A.rho * A.rho + 1e-30, wb = (double)Bb.rho * Bb.rho + 1e-30;
-    double n[3];
+    const float wa = A.rho * A.rho + 1e-30f, wb = Bb.rho * Bb.rho + 1e-30f;
+    float n[3];
     for (int k = 0; k < 3; k++) n[k] = wa * A.n[k] + wb * Bb.n[k];
-    double nl = sqrt(ftd::dot3d(n, n));
-    if (!(nl > 1e-12 * (wa + wb))) { for (int k = 0; k < 3; k++) n[k] = A.n[k]; nl = 1.0; }
-    for (int k = 0; k < 3; k++) n[k] /= nl;
-    double tau = 0.0, rho = 0.0;
+    const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (nn > 1e-20f * (wa + wb) * (wa + wb) && nn > 1e-30f) {
+        const float inv = __builtin_amdgcn_rsqf(nn);
+        for (int k = 0; k < 3; k++) n[k] *= inv;
+    } else {
+        for (int k = 0; k < 3; k++) n[k] = A.n[k];
+    }
+    float tau = 0.0f, rho = 0.0f;
     const Bound* ch[2] = {&A, &Bb};
     for (int i = 0; i < 2; i++) {
         const Bound& C = *ch[i];
-        double dc[3], ni[3], cr[3];
-        for (int k = 0; k < 3; k++) { dc[k] = (double)C.c[k] - P.c[k]; ni[k] = C.n[k]; }
-        const double cs = fmin(1.0, fabs(ftd::dot3d(n, ni)));
-        const double sn = sqrt(fmax(0.0, 1.0 - cs * cs));
-        ftd::cross3d(n, dc, cr);
-        tau = fmax(tau, fabs(ftd::dot3d(n, dc)) + C.tau * cs + C.rho * sn);
-        rho = fmax(rho, sqrt(ftd::dot3d(cr, cr)) + C.rho + C.tau * sn);
+        const float dc[3] = {C.c[0] - P.c[0], C.c[1] - P.c[1], C.c[2] - P.c[2]};
+        const float cs = fminf(1.0f, fabsf(n[0] * C.n[0] + n[1] * C.n[1] + n[2] * C.n[2]));
+        const float sx = n[1] * C.n[2] - n[2] * C.n[1], sy = n[2] * C.n[0] - n[0] * C.n[2],
+                    sz = n[0] * C.n[1] - n[1] * C.n[0];
+        const float sn = __builtin_amdgcn_sqrtf(sx * sx + sy * sy + sz * sz);
+        const float ax = fabsf(n[0] * dc[0] + n[1] * dc[1] + n[2] * dc[2]);
+        const float rx = n[1] * dc[2] - n[2] * dc[1], ry = n[2] * dc[0] - n[0] * dc[2],
+                    rz = n[0] * dc[1] - n[1] * dc[0];
+        const float rad = __builtin_amdgcn_sqrtf(rx * rx + ry * ry + rz * rz);
+        tau = fmaxf(tau, ax + C.tau * cs + C.rho * sn);
+        rho = fmaxf(rho, rad + C.rho + C.tau * sn);
     }
-    for (int k = 0; k < 3; k++) P.n[k] = (float)n[k];
-    P.tau = (float)(tau * (1.0 + 1e-6));
-    P.rho = (float)(rho * (1.0 + 1e-6));
+    for (int k = 0; k < 3; k++) P.n[k] = n[k];
+    P.tau = tau * (1.0f + 0x1.0p-12f) + 0x1.0p-18f * ext;
+    P.rho = rho * (1.0f + 0x1.0p-12f) + 0x1.0p-18f * ext;
     return P;
 }
 
