@@ -36,15 +36,33 @@ from .reinit import Reinit
 WORK_BYTES_PER_NODE = 96
 
 
-class MeshCache:
-    """Resident Kuhn meshes keyed by dims, within a byte budget (LRU)."""
+def _create_kuhn(device, dims):
+    """Resident Kuhn mesh on the device: (mesh id, bytes)."""
+    mid = C.c_int32()
+    check(lib.ft_mesh_create_kuhn(device, dims[0], dims[1], dims[2], 0.0, 0, C.byref(mid)))
+    b = C.c_int64()
+    check(lib.ft_mesh_info(mid.value, None, None, None, C.byref(b)))
+    return mid.value, b.value
 
-    def __init__(self, device=0, budget_bytes=200e9):
+
+def _release(mid):
+    check(lib.ft_mesh_release(mid))
+
+
+class MeshCache:
+    """Resident Kuhn meshes keyed by dims, within a byte budget (LRU).
+
+    create(device, dims) -> (id, bytes) and release(id) default to the
+    library's resident meshes."""
+
+    def __init__(self, device=0, budget_bytes=200e9, create=_create_kuhn, release=_release):
         self.device = device
         self.budget = float(budget_bytes)
         self.meshes = collections.OrderedDict()   # dims -> [id, bytes, pins]
         self.bytes = 0
         self.lock = threading.Lock()
+        self._create = create
+        self._release = release
 
     @staticmethod
     def estimate_bytes(dims):
@@ -59,7 +77,7 @@ class MeshCache:
                 return True
             mid, b, pins = self.meshes[key]
             if pins == 0:
-                check(lib.ft_mesh_release(mid))
+                self._release(mid)
                 self.bytes -= b
                 del self.meshes[key]
         return self.bytes + need <= self.budget
@@ -81,13 +99,10 @@ class MeshCache:
                 return self.meshes[key][0]
             if not self._evict_for(self.estimate_bytes(key)):
                 raise _lib.FtError(_lib.FT_ERR_BUDGET, f"mesh {key} does not fit the budget")
-            mid = C.c_int32()
-            check(lib.ft_mesh_create_kuhn(self.device, key[0], key[1], key[2], 0.0, 0, C.byref(mid)))
-            b = C.c_int64()
-            check(lib.ft_mesh_info(mid.value, None, None, None, C.byref(b)))
-            self.meshes[key] = [mid.value, b.value, 1]
-            self.bytes += b.value
-            return mid.value
+            mid, b = self._create(self.device, key)
+            self.meshes[key] = [mid, b, 1]
+            self.bytes += b
+            return mid
 
     def release(self, dims):
         with self.lock:
@@ -96,7 +111,7 @@ class MeshCache:
     def close(self):
         with self.lock:
             for mid, _, _ in self.meshes.values():
-                lib.ft_mesh_release(mid)
+                self._release(mid)
             self.meshes.clear()
             self.bytes = 0
 
@@ -140,6 +155,27 @@ class Job:
         return self.out
 
 
+def admit(queue, max_fields, max_nodes, inflight_bytes, can_admit):
+    """Takes from the front of `queue` the jobs that fit one batch: FIFO with no
+    overtaking, at most max_fields jobs and max_nodes nodes (a single larger
+    job is still admitted alone), and only while can_admit(dims, bytes) says
+    the meshes plus the work of the in-flight and admitted jobs fit the
+    budget.  Returns (jobs, work bytes of the admitted jobs)."""
+    jobs, nodes, extra = [], 0, 0
+    while queue and len(jobs) < max_fields:
+        j = queue[0]
+        work = WORK_BYTES_PER_NODE * j.n
+        if jobs and nodes + j.n > max_nodes:
+            break
+        if not can_admit(j.dims, inflight_bytes + extra + work):
+            break
+        queue.popleft()
+        jobs.append(j)
+        nodes += j.n
+        extra += work
+    return jobs, extra
+
+
 class Server:
     """Continuous-batching reinit server (see module docstring)."""
 
@@ -173,19 +209,8 @@ class Server:
         return job
 
     def _admit(self):
-        """Takes the queued jobs that fit one batch (FIFO, no overtaking)."""
-        jobs, nodes, extra = [], 0, 0
-        while self.queue and len(jobs) < self.max_fields:
-            j = self.queue[0]
-            work = WORK_BYTES_PER_NODE * j.n
-            if jobs and nodes + j.n > self.max_nodes:
-                break
-            if not self.cache.can_admit(j.dims, self.inflight_bytes + extra + work):
-                break
-            self.queue.popleft()
-            jobs.append(j)
-            nodes += j.n
-            extra += work
+        jobs, extra = admit(self.queue, self.max_fields, self.max_nodes, self.inflight_bytes,
+                            self.cache.can_admit)
         self.inflight_bytes += extra
         return jobs, extra
 

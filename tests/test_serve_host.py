@@ -1,0 +1,100 @@
+"""Host logic of the served form (no GPU): mesh cache budget/LRU/pins and
+batch admission."""
+import collections
+import types
+
+import pytest
+
+from fluidtissue_amd.serve import WORK_BYTES_PER_NODE, MeshCache, admit
+
+
+def _fake_cache(budget):
+    made, freed = [], []
+
+    def create(device, dims):
+        made.append(tuple(dims))
+        return len(made), MeshCache.estimate_bytes(dims)
+
+    return MeshCache(0, budget, create=create, release=freed.append), made, freed
+
+
+def test_cache_reuses_and_evicts_least_recently_used():
+    b20 = MeshCache.estimate_bytes((20, 20, 20))
+    c, made, freed = _fake_cache(2.5 * b20)
+    m1 = c.acquire((20, 20, 20)); c.release((20, 20, 20))
+    m2 = c.acquire((21, 20, 20)); c.release((21, 20, 20))
+    assert c.acquire((20, 20, 20)) == m1            # resident: no new mesh
+    c.release((20, 20, 20))
+    assert made == [(20, 20, 20), (21, 20, 20)]
+    c.acquire((22, 20, 20))                         # needs room: evicts (21,...), the LRU one
+    assert freed == [m2] and (20, 20, 20) in c.meshes and (21, 20, 20) not in c.meshes
+    assert c.bytes == sum(b for _, b, _ in c.meshes.values()) <= c.budget
+
+
+def test_cache_never_evicts_pinned_meshes():
+    b = MeshCache.estimate_bytes((20, 20, 20))
+    c, made, freed = _fake_cache(1.5 * b)
+    c.acquire((20, 20, 20))                         # pinned (not released)
+    assert not c.can_admit((20, 21, 20), 0)
+    with pytest.raises(Exception):
+        c.acquire((20, 21, 20))
+    assert freed == []
+    c.release((20, 20, 20))
+    assert c.can_admit((20, 21, 20), 0)
+    c.acquire((20, 21, 20))
+    assert freed == [1]
+
+
+def _job(n):
+    return types.SimpleNamespace(dims=(n, n, n), n=n ** 3)
+
+
+def test_admit_fifo_counts_nodes_and_budget():
+    q = collections.deque(_job(n) for n in (10, 12, 8, 30, 9))
+    always = lambda dims, extra: True
+    jobs, extra = admit(q, max_fields=3, max_nodes=10 ** 9, inflight_bytes=0, can_admit=always)
+    assert [j.n for j in jobs] == [1000, 1728, 512] and len(q) == 2
+    assert extra == WORK_BYTES_PER_NODE * (1000 + 1728 + 512)
+    # node cap: the 30^3 job does not join a batch that would exceed it, but
+    # is admitted alone next time
+    jobs, _ = admit(q, max_fields=8, max_nodes=20000, inflight_bytes=0, can_admit=always)
+    assert [j.n for j in jobs] == [27000]
+    # budget: nothing is taken past a job that does not fit (no overtaking)
+    q = collections.deque(_job(n) for n in (10, 40, 8))
+    seen = []
+
+    def fits(dims, extra):
+        seen.append((dims, extra))
+        return dims[0] != 40
+    jobs, extra = admit(q, 8, 10 ** 9, inflight_bytes=5, can_admit=fits)
+    assert [j.dims[0] for j in jobs] == [10] and [j.dims[0] for j in q] == [40, 8]
+    assert seen[0][1] == 5 + WORK_BYTES_PER_NODE * 1000    # in-flight work counted
+
+
+def test_autotune_picks_fastest_and_caches():
+    from fluidtissue_amd import autotune
+    clock = [0.0]
+    calls = []
+    cost = {(1.0, 64): 3.0, (1.0, 256): 2.0, (1.5, 64): 1.0, (1.5, 256): 4.0}
+
+    class FakeHandle:
+        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block):
+            calls.append((bin_scale, query_block))
+            clock[0] += cost[(bin_scale, query_block)]
+
+    restores = []
+    orig = autotune.time.perf_counter
+    autotune.time.perf_counter = lambda: clock[0]
+    try:
+        args = dict(sync=lambda: None, copy=lambda d, s, n: restores.append((d, s, n)),
+                    signature=("test", 1), bin_scales=(1.0, 1.5), query_blocks=(64, 256), reps=2)
+        bs, qb, table = autotune.tune(FakeHandle(), 11, 22, 33, **args)
+        assert (bs, qb) == (1.5, 64)
+        assert [(b, q) for b, q, _ in table] == [(1.0, 64), (1.0, 256), (1.5, 64), (1.5, 256)]
+        assert [ms for _, _, ms in table] == [3000.0, 2000.0, 1000.0, 4000.0]
+        assert len(calls) == 4 * 3 and restores == [(11, 22, 33)] * 12   # input restored each run
+        assert autotune.tune(FakeHandle(), 11, 22, 33, **args)[:2] == (1.5, 64)
+        assert len(calls) == 12                                        # cached per signature
+    finally:
+        autotune.time.perf_counter = orig
+        autotune._cache.pop(("triangles", 0.0, ("test", 1)), None)
