@@ -741,8 +741,25 @@ __global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, const uint64_t* s
         B.rho = 0.0f; B.tau = -1.0f;                 // box only
     } else {
         for (int k = 0; k < 3; k++) B.n[k] = (float)(ns[k] / nl);
-        double tau = 0.0, rho = 0.0;
         const double n[3] = {B.n[0], B.n[1], B.n[2]};
+        // The slab is centred on the vertices' extent along n, not on the box
+        // centre: a surface patch can pass well off its box's centre, and a
+        // slab centred off the patch is thicker by that offset.
+        double hlo = INFINITY, hhi = -INFINITY;
+        for (uint32_t s = b; s < e; s++) {
+            float v[3][3];
+            tri_vertices(tris, ifp, pidx[s], v[0], v[1], v[2]);
+            for (int q = 0; q < 3; q++) {
+                double d[3];
+                for (int k = 0; k < 3; k++) d[k] = (double)v[q][k] - B.c[k];
+                const double hv = ftd::dot3d(n, d);
+                hlo = fmin(hlo, hv);
+                hhi = fmax(hhi, hv);
+            }
+        }
+        const double mid = 0.5 * (hlo + hhi);
+        for (int k = 0; k < 3; k++) B.c[k] = (float)(B.c[k] + mid * n[k]);
+        double tau = 0.0, rho = 0.0;
         for (uint32_t s = b; s < e; s++) {
             float v[3][3];
             tri_vertices(tris, ifp, pidx[s], v[0], v[1], v[2]);
@@ -864,21 +881,34 @@ __device__ inline Bound merge_bounds(const Bound& A, const Bound& Bb)
     } else {
         for (int k = 0; k < 3; k++) n[k] = A.n[k];
     }
-    float tau = 0.0f, rho = 0.0f;
+    // The centre may be any point (thickness and radius below are taken
+    // about the rounded one); it is moved along n to centre the slab on the
+    // children's extents along n.
     const Bound* ch[2] = {&A, &Bb};
+    float cs[2], sn[2], hlo = INFINITY, hhi = -INFINITY;
+    for (int i = 0; i < 2; i++) {
+        const Bound& C = *ch[i];
+        cs[i] = fminf(1.0f, fabsf(n[0] * C.n[0] + n[1] * C.n[1] + n[2] * C.n[2]));
+        const float sx = n[1] * C.n[2] - n[2] * C.n[1], sy = n[2] * C.n[0] - n[0] * C.n[2],
+                    sz = n[0] * C.n[1] - n[1] * C.n[0];
+        sn[i] = __builtin_amdgcn_sqrtf(sx * sx + sy * sy + sz * sz);
+        const float hc = n[0] * (C.c[0] - P.c[0]) + n[1] * (C.c[1] - P.c[1]) + n[2] * (C.c[2] - P.c[2]);
+        const float ec = C.tau * cs[i] + C.rho * sn[i];
+        hlo = fminf(hlo, hc - ec);
+        hhi = fmaxf(hhi, hc + ec);
+    }
+    const float mid = 0.5f * (hlo + hhi);
+    for (int k = 0; k < 3; k++) P.c[k] += mid * n[k];
+    float tau = 0.0f, rho = 0.0f;
     for (int i = 0; i < 2; i++) {
         const Bound& C = *ch[i];
         const float dc[3] = {C.c[0] - P.c[0], C.c[1] - P.c[1], C.c[2] - P.c[2]};
-        const float cs = fminf(1.0f, fabsf(n[0] * C.n[0] + n[1] * C.n[1] + n[2] * C.n[2]));
-        const float sx = n[1] * C.n[2] - n[2] * C.n[1], sy = n[2] * C.n[0] - n[0] * C.n[2],
-                    sz = n[0] * C.n[1] - n[1] * C.n[0];
-        const float sn = __builtin_amdgcn_sqrtf(sx * sx + sy * sy + sz * sz);
         const float ax = fabsf(n[0] * dc[0] + n[1] * dc[1] + n[2] * dc[2]);
         const float rx = n[1] * dc[2] - n[2] * dc[1], ry = n[2] * dc[0] - n[0] * dc[2],
                     rz = n[0] * dc[1] - n[1] * dc[0];
         const float rad = __builtin_amdgcn_sqrtf(rx * rx + ry * ry + rz * rz);
-        tau = fmaxf(tau, ax + C.tau * cs + C.rho * sn);
-        rho = fmaxf(rho, rad + C.rho + C.tau * sn);
+        tau = fmaxf(tau, ax + C.tau * cs[i] + C.rho * sn[i]);
+        rho = fmaxf(rho, rad + C.rho + C.tau * sn[i]);
     }
     for (int k = 0; k < 3; k++) P.n[k] = n[k];
     P.tau = tau * (1.0f + 0x1.0p-12f) + 0x1.0p-18f * ext;
