@@ -1018,7 +1018,7 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned 
 // an absolute floor for near-zero distances.
 __device__ __forceinline__ float prune_bound(float best2)
 {
-    return best2 * (1.0f + 0x1.0p-16f) + 1.0e-30f;
+    return __fmaf_rn(best2, 1.0f + 0x1.0p-16f, 1.0e-30f);
 }
 
 template <int MODE>
@@ -1044,25 +1044,26 @@ struct Searcher {
 
     // Mode 0 keeps the nearest point (ties: lowest id, the reference's strict
     // < over ascending ids).  Mode 1 keeps the best two triangles by FP32
-    // distance, so that the epilogue can settle near-ties in FP64.
-    __device__ __forceinline__ void consider(float d, uint32_t s)
+    // distance, so that the epilogue can settle near-ties in FP64.  j is the
+    // primitive's original id (wave-uniform).  Written with selects, not
+    // branches: a lane-divergent branch costs exec-mask bookkeeping on every
+    // call, which is most of the work here.
+    __device__ __forceinline__ void consider(float d, int32_t j)
     {
-        if (MODE == 0) {
-            if (d < best || (d == best && d < INFINITY)) {
-                const int32_t j = pidx[s];
-                if (d < best || bj < 0 || j < bj) { set_best(d); bj = j; }
-            }
-        } else if (d < sec || (d == sec && d < INFINITY)) {
-            const int32_t j = pidx[s];
-            if (d < best || (d == best && (bj < 0 || j < bj))) {
-                if (bj >= 0 && j != bj) { sec = best; sj = bj; }
-                set_best(d);
-                bj = j;
-            } else if (j != bj && (d < sec || sj < 0 || j < sj)) {
-                sec = d;
-                sj = j;
-            }
+        // (bitwise & and |: short-circuit operators compile to branches)
+        const bool fin = d < INFINITY;
+        const bool nb = (d < best) | ((d == best) & fin & ((bj < 0) | (j < bj)));
+        if (MODE == 1) {
+            const bool demote = nb & (bj >= 0) & (j != bj);        // old best becomes second
+            const bool ns = !nb & (j != bj) & ((d < sec) | ((d == sec) & fin & ((sj < 0) | (j < sj))));
+            const float s2 = demote ? best : (ns ? d : sec);
+            const int32_t j2 = demote ? bj : (ns ? j : sj);
+            sec = s2;
+            sj = j2;
         }
+        best = nb ? d : best;
+        bj = nb ? j : bj;
+        bnd = nb ? prune_bound(MODE == 0 ? d * d : d) : bnd;
     }
 
     uint32_t n_leaves = 0, n_prims = 0, n_pairs = 0;   // statistics (FT_QUERY_STATS)
@@ -1071,11 +1072,11 @@ struct Searcher {
 
     // One triangle for all lanes: a wave-uniform early out when the disk
     // bound rules it out for every lane, else the exact FP32 distance.
-    __device__ __forceinline__ void test_tri(const TriRec& r, uint32_t s)
+    __device__ __forceinline__ void test_tri(const TriRec& r, int32_t j)
     {
         const ftd::TriProj pj = ftd::tri_project(x, y, z, r);
         if (__ballot(ftd::tri_disk_lb2(pj, r) <= bnd) == 0ull) return;
-        consider(ftd::tri_dist2_from(pj, r), s);
+        consider(ftd::tri_dist2_from(pj, r), j);
     }
 
     // Leaf test at a wave-uniform leaf: records read through scalar loads.
@@ -1088,9 +1089,10 @@ struct Searcher {
         if (MODE == 0) {
             for (uint32_t s = b; s < e; s++) {
                 const float4 p = ftd::cload(prec, s);
+                const int32_t j = ftd::cload(pidx, s);
                 const float dx = p.x - x, dy = p.y - y, dz = p.z - z;
                 const float d = sqrtf(dx * dx + dy * dy + dz * dz);   // EuclideanDistance.cu:20-28
-                consider(d, s);
+                consider(d, j);
             }
         } else {
             // two records per step: their scalar loads are in flight together
@@ -1100,10 +1102,11 @@ struct Searcher {
             for (; s + 1 < e; s += 2) {
                 const TriRec r0 = ftd::cload(trec, s);
                 const TriRec r1 = ftd::cload(trec, s + 1);
-                test_tri(r0, s);
-                test_tri(r1, s + 1);
+                const int32_t j0 = ftd::cload(pidx, s), j1 = ftd::cload(pidx, s + 1);
+                test_tri(r0, j0);
+                test_tri(r1, j1);
             }
-            if (s < e) test_tri(ftd::cload(trec, s), s);
+            if (s < e) test_tri(ftd::cload(trec, s), ftd::cload(pidx, s));
         }
     }
 
