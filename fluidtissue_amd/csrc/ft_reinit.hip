@@ -533,9 +533,15 @@ __global__ __launch_bounds__(kBS) void k_point_area(const uint32_t* off, const u
 // ============================================================================
 // Spatial index: uniform bins by counting sort, binary radix tree on bins
 // ============================================================================
+// Leaves per block of the local refit pass: a node whose leaf range stays
+// inside one block is fitted by that block's workgroup; the rest ("top"
+// nodes) by the single-workgroup pass after it.
+constexpr int kRefitBlock = 256;
+
 struct __align__(16) BNode {  // child bounds, children, parent
     Bound l, r;
-    int32_t left, right, parent, pad;
+    int32_t left, right, parent;
+    int32_t top;                 // 1: the leaf range spans refit blocks
 };
 
 __device__ __forceinline__ void tri_vertices(const int4* tris, const float* ifp, int32_t f,
@@ -819,30 +825,9 @@ __global__ __launch_bounds__(kBS) void k_karras(const uint32_t* keys, const uint
     if (i == 0) nodes[0].parent = -1;
     // the node whose range is exactly one field's leaves is that field's root
     const int lo = min(i, j), hi = max(i, j);
+    nodes[i].top = (lo / kRefitBlock) != (hi / kRefitBlock);
     const int f = (int)(keys[lo] >> code_bits);
     if (leaf_off[f] == lo && leaf_off[f + 1] == hi + 1) roots[f] = i;
-}
-
-// Child bounds of a node (16-byte aligned in BNode), 4 x 128-bit.  The
-// agent-scope fences around the arrival counter in k_refit order them: the
-// writer's fence (release) completes its stores before its increment, the
-// reader's fence (acquire) invalidates its vector L1 before it loads.
-__device__ __forceinline__ void st_bound(Bound* dst, const Bound& v)
-{
-    const unsigned* src = reinterpret_cast<const unsigned*>(&v);
-    unsigned* d = reinterpret_cast<unsigned*>(dst);
-    for (int k = 0; k < 16; k++)
-        __hip_atomic_store(&d[k], src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ Bound ld_bound(const Bound* srcp)
-{
-    Bound v;
-    const unsigned* s = reinterpret_cast<const unsigned*>(srcp);
-    unsigned* d = reinterpret_cast<unsigned*>(&v);
-    for (int k = 0; k < 16; k++)
-        d[k] = __hip_atomic_load(&s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return v;
 }
 
 // Parent bound of two children: box union and a cylinder that contains both
@@ -927,30 +912,84 @@ __device__ __forceinline__ void drop_loose_cylinder(Bound& b, float alpha)
     if (vc >= alpha * vb) { b.rho = 0.0f; b.tau = -1.0f; }
 }
 
-__global__ __launch_bounds__(kBS) void k_refit(const uint64_t* sc, const int32_t* leaf_parent,
-                                               const Bound* leaf_b, BNode* nodes, unsigned* flags,
-                                               float alpha)
+// Bottom-up fitting in two passes, with no device-wide fence: with one L2 per
+// XCD, an agent-scope release or acquire writes back or invalidates that L2,
+// and a fence per tree level made the one-pass form the index's slowest
+// kernel.  Karras node i spans leaves [min(i,j), max(i,j)] with i one end, so
+// a node inside refit block b has its index in that block.
+//
+// Pass 1, one workgroup per block of kRefitBlock leaves: a thread per leaf
+// climbs while the parent is inside the block.  Arrivals are counted in LDS;
+// the second child to arrive merges both (workgroup-scope fences order the
+// bound stores, which the workgroup's waves see through their CU).  At a top
+// node it stores its bound and counts the arrival in flags[p]; the second
+// arrival there appends p to the ready list (both children done).
+__global__ __launch_bounds__(kRefitBlock) void k_refit_local(const uint64_t* sc, const int32_t* leaf_parent,
+                                                             const Bound* leaf_b, BNode* nodes,
+                                                             unsigned* flags, unsigned* ready,
+                                                             float alpha)
 {
+    __shared__ unsigned arrived[kRefitBlock];
     if (halted(sc)) return;
     const int L = (int)cnt_leaves(sc);
-    const int l = blockIdx.x * kBS + threadIdx.x;
-    if (L < 2 || l >= L) return;
+    const int b0 = blockIdx.x * kRefitBlock;
+    if (L < 2 || b0 >= L) return;                 // uniform over the block
+    arrived[threadIdx.x] = 0u;
+    __syncthreads();
+    const int l = b0 + threadIdx.x;
+    if (l >= L) return;
     Bound cur_b = leaf_b[l];
     int cur = ~l;
     int p = leaf_parent[l];
-    for (int depth = 0; p >= 0 && depth < 64; depth++) {   // depth <= 24 for 24-bit keys
+    for (int depth = 0; p >= 0 && depth < 64; depth++) {   // depth <= 32 for 32-bit keys
         BNode* nd = &nodes[p];
         const bool is_left = (nd->left == cur);
-        st_bound(is_left ? &nd->l : &nd->r, cur_b);
-        __threadfence();
-        const unsigned old = atomicAdd(&flags[p], 1u);
-        if (old == 0) return;                       // sibling not done yet
-        __threadfence();
-        const Bound sib = ld_bound(is_left ? &nd->r : &nd->l);
+        if (is_left) nd->l = cur_b; else nd->r = cur_b;
+        if (nd->top) {
+            if (atomicAdd(&flags[p], 1u) == 1u) ready[1 + atomicAdd(&ready[0], 1u)] = (unsigned)p;
+            return;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&arrived[p - b0], 1u) == 0u) return;   // sibling not done yet
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const Bound sib = is_left ? nd->r : nd->l;
         cur_b = is_left ? merge_bounds(cur_b, sib) : merge_bounds(sib, cur_b);
         drop_loose_cylinder(cur_b, alpha);
         cur = p;
-        p = __hip_atomic_load(&nd->parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p = nd->parent;
+    }
+}
+
+// Pass 2, one workgroup: from each ready top node up.  flags[p] already counts
+// the children that pass 1 finished; the thread whose arrival completes a node
+// carries on with it.  All threads share one workgroup (one CU), so the
+// bound stores need only workgroup-scope fences; pass 1's stores are visible
+// across the kernel boundary.
+constexpr int kRefitTopThreads = 1024;
+__global__ __launch_bounds__(kRefitTopThreads) void k_refit_top(const uint64_t* sc, BNode* nodes,
+                                                                unsigned* flags, const unsigned* ready,
+                                                                float alpha)
+{
+    if (halted(sc)) return;
+    const unsigned nr = ready[0];
+    for (unsigned k = threadIdx.x; k < nr; k += kRefitTopThreads) {
+        int cur = (int)ready[1 + k];
+        Bound cur_b = merge_bounds(nodes[cur].l, nodes[cur].r);
+        drop_loose_cylinder(cur_b, alpha);
+        int p = nodes[cur].parent;
+        for (int depth = 0; p >= 0 && depth < 64; depth++) {
+            BNode* nd = &nodes[p];
+            const bool is_left = (nd->left == cur);
+            if (is_left) nd->l = cur_b; else nd->r = cur_b;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (atomicAdd(&flags[p], 1u) == 0u) break;   // the other child is still on its way
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const Bound sib = is_left ? nd->r : nd->l;
+            cur_b = is_left ? merge_bounds(cur_b, sib) : merge_bounds(sib, cur_b);
+            drop_loose_cylinder(cur_b, alpha);
+            cur = p;
+            p = nd->parent;
+        }
     }
 }
 
@@ -2035,7 +2074,8 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
         FT_TRY(ensure(c->leaf_b, sizeof(Bound) * at_least1(L)));
         FT_TRY(ensure(c->leaf_parent, sizeof(int32_t) * at_least1(L)));
         FT_TRY(ensure(c->bnodes, sizeof(BNode) * at_least1(L - 1)));
-        FT_TRY(ensure(c->bflags, sizeof(unsigned) * at_least1(L - 1)));
+        // arrival counters (L - 1), then the ready count and list (1 + L - 1)
+        FT_TRY(ensure(c->bflags, sizeof(unsigned) * at_least1(2 * L)));
         // scan scratch of the largest scan of a reinit
         const int64_t nb = std::max({scan_blocks(nblk), scan_blocks(nw_edges), scan_blocks(cp.P),
                                      scan_blocks(G)});
@@ -2238,14 +2278,19 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
                            c->pidx.as<int32_t>(), ifp, tris, c->leaf_b.as<Bound>());
     }
     if (L_cap > 1) {
-        FT_TRY(fill<unsigned>(s, c->bflags.as<unsigned>(), L_cap - 1, 0u));
+        // arrival counters and the ready count
+        FT_TRY(fill<unsigned>(s, c->bflags.as<unsigned>(), L_cap, 0u));
         hipLaunchKernelGGL(k_karras, dim3(grid1d(L_cap - 1)), dim3(kBS), 0, s,
                            c->leaf_key.as<uint32_t>(), sc, c->bnodes.as<BNode>(),
                            c->leaf_parent.as<int32_t>(), c->idx_code_bits,
                            c->leaf_off.as<int32_t>(), c->roots.as<int32_t>());
-        hipLaunchKernelGGL(k_refit, dim3(grid1d(L_cap)), dim3(kBS), 0, s, sc,
-                           c->leaf_parent.as<int32_t>(), c->leaf_b.as<Bound>(),
-                           c->bnodes.as<BNode>(), c->bflags.as<unsigned>(), cyl_alpha());
+        unsigned* ready = c->bflags.as<unsigned>() + (L_cap - 1);
+        hipLaunchKernelGGL(k_refit_local, dim3((unsigned)((L_cap + kRefitBlock - 1) / kRefitBlock)),
+                           dim3(kRefitBlock), 0, s, sc, c->leaf_parent.as<int32_t>(),
+                           c->leaf_b.as<Bound>(), c->bnodes.as<BNode>(), c->bflags.as<unsigned>(),
+                           ready, cyl_alpha());
+        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(kRefitTopThreads), 0, s, sc,
+                           c->bnodes.as<BNode>(), c->bflags.as<unsigned>(), ready, cyl_alpha());
     }
     FT_TRY(check_launch());
 
