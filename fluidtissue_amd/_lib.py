@@ -110,6 +110,8 @@ _SIGS = {
     "ft_graph_stats": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ft_get_normals": (C.c_int, [_H, _P]),
     "ft_interface_curvature": (C.c_int, [_H, _P, _P]),
+    "ft_surface_tension": (C.c_int, [_H, C.c_float, C.c_int32, _P, _P, _P]),
+    "ft_surface_tension_device": (C.c_int, [_H, C.c_float, C.c_int32, _P, _P, _P, _P]),
     "ft_get_counts": (C.c_int, [_H, _P]),
     "ft_write_interface_obj": (C.c_int, [_H, C.c_char_p, C.c_int32]),
     "ft_normals_device": (C.c_int, [_H, C.POINTER(C.c_void_p)]),

@@ -1478,6 +1478,70 @@ __global__ __launch_bounds__(kBS) void k_curv_points(const int32_t* ifp2e, int64
     k_if[i] = (float)((1.0 - (double)t) * (double)k[e.x] + (double)tk1);
 }
 
+// ---- Flow::surfaceTension_ (src/growth/Flow.cpp:307-341) -------------------
+// Crossing point i (on edge e = (e0, e1), t = phi[e0] / (phi[e0] - phi[e1])
+// as float) adds to node e0 the force (1 - t) st (-k_i) area_i rho n0 and to
+// e1 t st (-k_i) area_i rho n1, with n the node's normal over its norm.  The
+// reference's loop is an OpenMP loop whose += on a node shared by two points
+// races; the result defined here is the serial one, terms added in ascending
+// i.  Each node gathers its terms (ids 2i + side, sorted), so the order is
+// exact with no atomics on the forces.
+__global__ __launch_bounds__(kBS) void k_st_count(const int32_t* ifp2e, int64_t P, const int2* edges,
+                                                  uint32_t* cnt)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= P) return;
+    const int2 e = edges[ifp2e[i]];
+    atomicAdd(&cnt[e.x], 1u);
+    atomicAdd(&cnt[e.y], 1u);
+}
+
+__global__ __launch_bounds__(kBS) void k_st_fill(const int32_t* ifp2e, int64_t P, const int2* edges,
+                                                 const uint64_t* off, uint32_t* cursor, int32_t* ids)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= P) return;
+    const int2 e = edges[ifp2e[i]];
+    ids[off[e.x] + atomicAdd(&cursor[e.x], 1u)] = (int32_t)(2 * i);
+    ids[off[e.y] + atomicAdd(&cursor[e.y], 1u)] = (int32_t)(2 * i + 1);
+}
+
+// Types as the reference evaluates them: (1.0 - t) makes the e0 factor a
+// double product, cast to float when it scales the float normal; t * st
+// keeps the e1 factor in float; fx is double, each term added as a float.
+__global__ __launch_bounds__(kBS) void k_st_apply(const uint64_t* off, const int32_t* ids, int64_t N,
+                                                  const int32_t* ifp2e, const int2* edges,
+                                                  const double* phi, const float* k, const float* area,
+                                                  const float* normals, float st, int32_t rho,
+                                                  double* fx, double* fy, double* fz)
+{
+    const int64_t v = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (v >= N) return;
+    const uint64_t b = off[v], e = off[v + 1];
+    if (b == e) return;
+    const float x = normals[3 * v], y = normals[3 * v + 1], z = normals[3 * v + 2];
+    const float nrm = sqrtf(x * x + (y * y + z * z));
+    const float nx = x / nrm, ny = y / nrm, nz = z / nrm;
+    double ax = fx[v], ay = fy[v], az = fz[v];
+    for (uint64_t q = b; q < e; q++) {
+        const int32_t id = ids[q];
+        const int32_t i = id >> 1;
+        const int2 ed = edges[ifp2e[i]];
+        const float t = (float)(phi[ed.x] / (phi[ed.x] - phi[ed.y]));
+        float sc;
+        if ((id & 1) == 0)
+            sc = (float)((((1.0 - (double)t) * (double)st) * (double)(-k[i])) * (double)area[i] * (double)rho);
+        else
+            sc = (((t * st) * (-k[i])) * area[i]) * (float)rho;
+        ax += (double)(sc * nx);
+        ay += (double)(sc * ny);
+        az += (double)(sc * nz);
+    }
+    fx[v] = ax;
+    fy[v] = ay;
+    fz[v] = az;
+}
+
 // ============================================================================
 // Per-field interface counts (batches): ranks at the field boundaries
 // ============================================================================
@@ -1703,6 +1767,8 @@ struct ft_ctx {
     // normals: node -> tets lists (per field layout), tet gradients, node normals
     DevBuf n2t_off, n2t_ids, grad_t, normals, phi_ifn_int;
     DevBuf gsplit, kmesh, k_if;      // curvature: unit-normal components, node and point curvature
+    DevBuf st_cnt, st_off, st_ids;   // surface tension: node -> crossing-point terms
+    DevBuf st_f;                     // surface tension: device copy of host force arrays
     const double* last_phi = nullptr;   // device phi of the last reinit (blended)
     uint64_t n2t_layout = ~0ull;
     bool have_normals = false;
@@ -1750,7 +1816,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
             &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
             &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
-            &h->gsplit, &h->kmesh, &h->k_if};
+            &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f};
 }
 
 std::vector<void*> buffer_addresses(ft_ctx* c)
@@ -3115,13 +3181,12 @@ int ft_normals_device(ft_handle h, const float** normals)
     return FT_OK;
 }
 
-int ft_interface_curvature(ft_handle h, float* k_if, void* stream)
+// LevelSet::getInterfaceCurvature (LevelSet.cpp:215-253) of the last reinit
+// into the handle's k_if, on stream s.
+int enqueue_curvature(ft_ctx* h, hipStream_t s)
 {
-    FT_TRY(check_handle(h));
-    if (!k_if) return ft::set_error(FT_ERR_ARG, "null output");
     if (!h->have_normals || !h->have_interface || !h->last_phi)
         return ft::set_error(FT_ERR_STATE, "curvature needs a reinit with want_normals first");
-    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     const int64_t N = h->N, T = h->T, P = h->last.n_ifp;
     FT_TRY(ensure(h->gsplit, 24 * at_least1(N)));
     FT_TRY(ensure(h->kmesh, 4 * at_least1(N)));
@@ -3144,9 +3209,75 @@ int ft_interface_curvature(ft_handle h, float* k_if, void* stream)
                                h->edges.as<int2>(), h->last_phi, h->kmesh.as<float>(), h->k_if.as<float>());
         FT_TRY(check_launch());
     }
+    return FT_OK;
+}
+
+int ft_interface_curvature(ft_handle h, float* k_if, void* stream)
+{
+    FT_TRY(check_handle(h));
+    if (!k_if) return ft::set_error(FT_ERR_ARG, "null output");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    FT_TRY(enqueue_curvature(h, s));
+    const int64_t P = h->last.n_ifp;
     if (P > 0) FT_HIP(hipMemcpyAsync(k_if, h->k_if.p, 4 * P, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
     return FT_OK;
+}
+
+// Flow::surfaceTension_ (src/growth/Flow.cpp:307-341) on device arrays.
+int surface_tension(ft_ctx* h, float st, int32_t rho, double* fx, double* fy, double* fz, hipStream_t s)
+{
+    FT_TRY(enqueue_curvature(h, s));
+    const int64_t N = h->N, P = h->last.n_ifp;
+    if (N == 0 || P == 0) return FT_OK;
+    FT_TRY(ensure(h->st_cnt, 4 * N));
+    FT_TRY(ensure(h->st_off, 8 * (N + 1)));
+    FT_TRY(ensure(h->st_ids, 4 * 2 * P));
+    uint32_t* cnt = h->st_cnt.as<uint32_t>();
+    uint64_t* off = h->st_off.as<uint64_t>();
+    int32_t* ids = h->st_ids.as<int32_t>();
+    const int32_t* ifp2e = h->ifp2e.as<int32_t>();
+    const int2* edges = h->edges.as<int2>();
+    FT_TRY(fill<uint32_t>(s, cnt, N, 0u));
+    hipLaunchKernelGGL(k_st_count, dim3(grid1d(P)), dim3(kBS), 0, s, ifp2e, P, edges, cnt);
+    FT_TRY(scan<uint32_t, uint64_t>(h, s, cnt, off, N, off + N, LoadSame{}));
+    FT_TRY(fill<uint32_t>(s, cnt, N, 0u));
+    hipLaunchKernelGGL(k_st_fill, dim3(grid1d(P)), dim3(kBS), 0, s, ifp2e, P, edges, off, cnt, ids);
+    hipLaunchKernelGGL(k_n2t_sort, dim3(grid1d(N)), dim3(kBS), 0, s, off, N, ids);
+    hipLaunchKernelGGL(k_st_apply, dim3(grid1d(N)), dim3(kBS), 0, s, off, ids, N, ifp2e, edges,
+                       h->last_phi, h->k_if.as<float>(), h->ifp_area.as<float>(), h->normals.as<float>(),
+                       st, rho, fx, fy, fz);
+    FT_TRY(check_launch());
+    return FT_OK;
+}
+
+int ft_surface_tension_device(ft_handle h, float st, int32_t rho, double* fx, double* fy, double* fz,
+                              void* stream)
+{
+    FT_TRY(check_handle(h));
+    if (h->N > 0 && (!fx || !fy || !fz)) return ft::set_error(FT_ERR_ARG, "null force array");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    FT_TRY(surface_tension(h, st, rho, fx, fy, fz, s));
+    FT_HIP(hipStreamSynchronize(s));
+    return FT_OK;
+}
+
+int ft_surface_tension(ft_handle h, float st, int32_t rho, double* fx, double* fy, double* fz)
+{
+    FT_TRY(check_handle(h));
+    const int64_t N = h->N;
+    if (N > 0 && (!fx || !fy || !fz)) return ft::set_error(FT_ERR_ARG, "null force array");
+    hipStream_t s = h->stream;
+    FT_TRY(ensure(h->st_f, 24 * at_least1(N)));
+    double* d = h->st_f.as<double>();
+    double* host[3] = {fx, fy, fz};
+    for (int c = 0; c < 3 && N > 0; c++)
+        FT_HIP(hipMemcpyAsync(d + c * N, host[c], 8 * N, hipMemcpyHostToDevice, s));
+    int rc = surface_tension(h, st, rho, d, d + N, d + 2 * N, s);
+    for (int c = 0; c < 3 && N > 0 && rc == FT_OK; c++)
+        FT_HIP(hipMemcpyAsync(host[c], d + c * N, 8 * N, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
+    return rc;
 }
 
 int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches)

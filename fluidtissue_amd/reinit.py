@@ -333,6 +333,21 @@ class Reinit(_Handle):
         check(lib.ft_interface_curvature(self._h, _ptr(out), C.c_void_p(stream) if stream else None))
         return out
 
+    def surface_tension(self, st, rho, fx=None, fy=None, fz=None):
+        """Flow::surfaceTension_ after a reinit with want_normals: adds the
+        surface-tension force into fx, fy, fz (float64, one per node; zeros
+        when not given) and returns them."""
+        f = []
+        for a in (fx, fy, fz):
+            if a is None:
+                a = np.zeros(self.n, np.float64)
+            elif a.dtype != np.float64 or a.shape != (self.n,) or not a.flags.c_contiguous:
+                raise ValueError("force arrays: contiguous float64 of one entry per node")
+            f.append(a)
+        check(lib.ft_surface_tension(self._h, C.c_float(st), C.c_int32(rho), _ptr(f[0]), _ptr(f[1]),
+                                     _ptr(f[2])))
+        return tuple(f)
+
     def write_obj(self, fname, unit_normals=True):
         """fname.obj / fname.mtl of the last reinit's interface (cfish::Write_Wavefront_obj)."""
         check(lib.ft_write_interface_obj(self._h, str(fname).encode(), 1 if unit_normals else 0))

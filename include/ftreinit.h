@@ -175,6 +175,20 @@ int ft_normals_device(ft_handle h, const float** normals);
  * device copy; device: the caller's pointer) must still hold that phi. */
 int ft_interface_curvature(ft_handle h, float* k_if, void* stream);
 
+/* Flow::surfaceTension_ (src/growth/Flow.cpp:307-341) after a reinit with
+ * want_normals = 1: adds to fx, fy, fz (n_nodes doubles each, read and
+ * written) the surface-tension force of each crossing point i on edge
+ * (e0, e1), t = phi[e0] / (phi[e0] - phi[e1]) from the reinit's phi:
+ *   e0 += (1 - t) st (-k_i) ifpArea_i rho n(e0)/|n(e0)|
+ *   e1 +=  t      st (-k_i) ifpArea_i rho n(e1)/|n(e1)|
+ * with k_i the curvature of ft_interface_curvature.  Terms are added in
+ * increasing i, the reference's serial order (its OpenMP loop races where
+ * two points share a node).  Host arrays; the _device form takes device
+ * arrays and a stream (null: the handle's). */
+int ft_surface_tension(ft_handle h, float st, int32_t rho, double* fx, double* fy, double* fz);
+int ft_surface_tension_device(ft_handle h, float st, int32_t rho, double* fx, double* fy,
+                              double* fz, void* stream);
+
 /* Totals of the last successful reinit on h (sizes for ft_get_interface). */
 int ft_get_counts(ft_handle h, ft_counts* total);
 

@@ -499,3 +499,41 @@ void ora_interface_curvature(int64_t n_nodes, const float* p, int64_t n_tets,
     free(gn);
     free(k);
 }
+
+
+/* ---------------------------------------------------------------------- */
+/* Flow::surfaceTension_  (src/growth/Flow.cpp:307-341)                    */
+/* ---------------------------------------------------------------------- */
+static void unit_normal(const float* g, int32_t v, float* n)
+{
+    const float x = g[3 * (int64_t)v], y = g[3 * (int64_t)v + 1], z = g[3 * (int64_t)v + 2];
+    const float nrm = sqrtf(x * x + (y * y + z * z));              /* :327, Vec3f::norm */
+    n[0] = x / nrm;
+    n[1] = y / nrm;
+    n[2] = z / nrm;
+}
+
+void ora_surface_tension(const float* g, const double* phi, int64_t n_ifp,
+                         const int32_t* ifp2e, const int32_t* edges, const float* k_if,
+                         const float* area, float st, int32_t rho, double* fx, double* fy,
+                         double* fz)
+{
+    for (int64_t i = 0; i < n_ifp; i++) {                          /* :313 */
+        const int32_t e0 = edges[2 * (int64_t)ifp2e[i]], e1 = edges[2 * (int64_t)ifp2e[i] + 1];
+        const float t = (float)(phi[e0] / (phi[e0] - phi[e1]));    /* :320 */
+        float n[3];
+        unit_normal(g, e0, n);                                     /* :327 */
+        const double s0 = (1.0 - (double)t) * (double)st;          /* :328, left to right */
+        const double s1 = s0 * (double)(-k_if[i]);
+        const double s2 = s1 * (double)area[i];
+        const float f0 = (float)(s2 * (double)rho);
+        fx[e0] += (double)(f0 * n[0]);                             /* :329-331 */
+        fy[e0] += (double)(f0 * n[1]);
+        fz[e0] += (double)(f0 * n[2]);
+        unit_normal(g, e1, n);                                     /* :333 */
+        const float f1 = (((t * st) * (-k_if[i])) * area[i]) * (float)rho;   /* :334, float */
+        fx[e1] += (double)(f1 * n[0]);                             /* :335-337 */
+        fy[e1] += (double)(f1 * n[1]);
+        fz[e1] += (double)(f1 * n[2]);
+    }
+}

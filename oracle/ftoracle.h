@@ -137,6 +137,18 @@ void ora_interface_curvature(int64_t n_nodes, const float* p, int64_t n_tets,
                              const int32_t* ifp2e, const int32_t* edges,
                              float* k_if);
 
+/* Flow::surfaceTension_ (src/growth/Flow.cpp:307-341), serially in
+ * increasing crossing point i (the reference's OpenMP loop races on nodes
+ * that two points share; serial order is the defined result): on edge
+ * (e0, e1), t = float(phi0 / (phi0 - phi1)), unit normal n = g / |g| in
+ * float, then fx[e0] += float(((((1.0 - t) st) (-k_i)) area_i) rho) n.x
+ * (a double chain, cast to float to scale the float normal) and fx[e1] +=
+ * ((((t st) (-k_i)) area_i) rho) n.x in float; likewise y and z. */
+void ora_surface_tension(const float* g, const double* phi, int64_t n_ifp,
+                         const int32_t* ifp2e, const int32_t* edges, const float* k_if,
+                         const float* area, float st, int32_t rho, double* fx, double* fy,
+                         double* fz);
+
 #ifdef __cplusplus
 }
 #endif

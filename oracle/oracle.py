@@ -43,6 +43,8 @@ _o.ora_grad_nodes.argtypes = [_i64, _P, _P, _P, _P]
 
 _o.ora_interface_curvature.restype = None
 _o.ora_interface_curvature.argtypes = [_i64, _P, _i64, _P, _P, _P, _P, _P, _i64, _P, _P, _P]
+_o.ora_surface_tension.restype = None
+_o.ora_surface_tension.argtypes = [_P, _P, _i64, _P, _P, _P, _P, C.c_float, C.c_int32, _P, _P, _P]
 
 NTHREADS = max(1, min(16, os.cpu_count() or 1))
 
@@ -211,3 +213,18 @@ def interface_curvature(tets, xyz, normals, phi, ifp2e, edges):
     _o.ora_interface_curvature(n, _p(xyz), len(tets), _p(tets), _p(off), _p(ids), _p(g), _p(phi),
                                len(ifp2e), _p(ifp2e), _p(edges), _p(out))
     return out
+
+
+def surface_tension(normals, phi, ifp2e, edges, k_if, area, st, rho, fx=None, fy=None, fz=None):
+    """Flow::surfaceTension_ in serial crossing-point order; adds into fx, fy,
+    fz (float64 per node, zeros when not given) and returns them."""
+    g = _c(normals, np.float32)
+    n = len(g)
+    out = []
+    for a in (fx, fy, fz):
+        out.append(np.zeros(n, np.float64) if a is None else np.array(a, np.float64, copy=True))
+    phi, ifp2e, edges = _c(phi, np.float64), _c(ifp2e, np.int32), _c(edges, np.int32)
+    k_if, area = _c(k_if, np.float32), _c(area, np.float32)
+    _o.ora_surface_tension(_p(g), _p(phi), len(ifp2e), _p(ifp2e), _p(edges), _p(k_if), _p(area),
+                           float(st), int(rho), _p(out[0]), _p(out[1]), _p(out[2]))
+    return tuple(out)

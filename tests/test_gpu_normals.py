@@ -109,6 +109,33 @@ def test_interface_curvature_sphere_on_device():
     assert abs(np.median(k) - 2 / R) < 0.05 * 2 / R
 
 
+@pytest.mark.parametrize("name,mode,blend", [("cfg1", "points", "update"), ("ellipsoids", "triangles", "init"),
+                                             ("cfg2s", "triangles", "update")])
+def test_surface_tension_matches_oracle(name, mode, blend):
+    """Flow::surfaceTension_ (src/growth/Flow.cpp:307-341) on the device, added
+    into given force arrays, bit for bit against the oracle's serial loop; the
+    host and device entry points agree."""
+    import ctypes as C
+    import torch
+    m, phi, h = workload(name)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, cnt = r.reset_phi(phi, mode=mode, blend=blend, want_normals=True)
+    g, k, itf = r.normals(), r.curvature(), r.interface()
+    rng = np.random.default_rng(3)
+    f0 = [rng.standard_normal(len(m.xyz)) for _ in range(3)]
+    got = r.surface_tension(0.7, 3, *[a.copy() for a in f0])
+    d = [torch.from_numpy(a.copy()).cuda() for a in f0]
+    fa._lib.check(fa._lib.lib.ft_surface_tension_device(r._h, C.c_float(0.7), C.c_int32(3),
+                                                        *[C.c_void_p(t.data_ptr()) for t in d], None))
+    r.close()
+    ref = oracle.surface_tension(g, out, itf.ifp2e, m.edges, k, itf.ifp_area, 0.7, 3, *f0)
+    for a in range(3):
+        assert np.array_equal(got[a].view(np.uint64), ref[a].view(np.uint64)), a
+        assert np.array_equal(d[a].cpu().numpy().view(np.uint64), ref[a].view(np.uint64)), a
+    assert np.any(got[0] != f0[0])
+
+
 @pytest.mark.parametrize("unit", [True, False])
 def test_interface_obj_matches_reference_text(tmp_path, unit):
     """The OBJ export of the interface, byte for byte the text the reference's

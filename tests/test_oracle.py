@@ -374,3 +374,53 @@ def test_interface_curvature_plane_is_zero():
     k = oracle.interface_curvature(m.tets, m.xyz, oracle.grad_nodes(m.tets, m.xyz, phi), ref["phi"],
                                    ref["ifp2e"], m.edges)
     assert np.max(np.abs(k)) < 1e-4        # linear phi: constant unit normal, zero divergence
+
+
+def _sphere_tension(update=False, st=0.7, rho=3):
+    import fluidtissue_amd as fa
+    m = fa.kuhn_mesh(40)
+    R = 0.3
+    c = np.array([0.5, 0.49, 0.51])
+    phi = np.linalg.norm(m.xyz.astype(np.float64) - c, axis=1) - R
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=1 if update else 0)
+    g = oracle.grad_nodes(m.tets, m.xyz, ref["phi_ifn"])
+    k = oracle.interface_curvature(m.tets, m.xyz, g, ref["phi"], ref["ifp2e"], m.edges)
+    f = oracle.surface_tension(g, ref["phi"], ref["ifp2e"], m.edges, k, ref["ifp_area"], st, rho)
+    return m, ref, k, np.stack(f, 1), R, c, st, rho
+
+
+def test_surface_tension_sphere_closed_forms():
+    """Flow::surfaceTension_ on a sphere: each crossing point's force
+    st (-k) area rho along the unit normal, split (1-t, t) over its edge.
+    - every touched node is pushed inward (k = 2/R > 0, normals outward);
+    - sum of |f| = st rho sum_i k_i area_i, with sum_i area_i = 3 x 4 pi R^2
+      (each triangle's full area goes to each of its vertices) and k ~ 2/R;
+    - the net force of a closed surface vanishes;
+    - the split puts each point's force at phi = 0 along its edge:
+      (1-t) phi0 + t phi1 = 0, so sum_v |f_v| phi_v = 0 up to rounding."""
+    m, ref, k, f, R, c, st, rho = _sphere_tension()
+    touched = np.nonzero(np.linalg.norm(f, axis=1) > 0)[0]
+    x = m.xyz[touched].astype(np.float64) - c
+    assert np.all(np.sum(f[touched] * x, axis=1) < 0)
+    mag = np.linalg.norm(f, axis=1)
+    total = st * rho * np.sum(k.astype(np.float64) * ref["ifp_area"])
+    assert abs(mag.sum() - total) < 1e-5 * total
+    assert abs(np.sum(ref["ifp_area"]) - 3 * 4 * np.pi * R * R) < 0.02 * 3 * 4 * np.pi * R * R
+    assert abs(total - st * rho * (2 / R) * 3 * 4 * np.pi * R * R) < 0.05 * total
+    assert np.linalg.norm(f.sum(0)) < 0.01 * mag.sum()
+    phi = ref["phi"]
+    assert abs(np.sum(mag * phi)) < 1e-5 * np.sum(mag * np.abs(phi))
+
+
+def test_surface_tension_adds_into_given_forces():
+    """The force is added to the caller's arrays (+=), in the reference's
+    serial order; with the update blend t comes from the blended phi."""
+    m, ref, k, f, R, c, st, rho = _sphere_tension(update=True)
+    g = oracle.grad_nodes(m.tets, m.xyz, ref["phi_ifn"])
+    rng = np.random.default_rng(7)
+    f0 = [rng.standard_normal(len(m.xyz)) for _ in range(3)]
+    f1 = oracle.surface_tension(g, ref["phi"], ref["ifp2e"], m.edges, k, ref["ifp_area"], st, rho, *f0)
+    for a in range(3):
+        assert np.allclose(f1[a] - f0[a], f[:, a], rtol=1e-6, atol=1e-6 * np.abs(f[:, a]).max())
+    mag = np.linalg.norm(f, axis=1)
+    assert abs(np.sum(mag * ref["phi"])) < 1e-5 * np.sum(mag * np.abs(ref["phi"]))
