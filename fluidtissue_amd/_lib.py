@@ -18,6 +18,7 @@ FT_ERR_T2E = -5
 FT_ERR_DEGENERATE = -6
 FT_ERR_STATE = -7
 FT_ERR_BUDGET = -8
+FT_ERR_SOLVER = -9
 
 FT_DIST_POINTS = 0
 FT_DIST_TRIANGLES = 1
@@ -52,6 +53,16 @@ class FtCounts(C.Structure):
         ("n_if_elems", C.c_int64),
         ("n_clamped", C.c_int64),
         ("bad_tet", C.c_int64),
+    ]
+
+
+class FtAdvectInfo(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("restarts", C.c_int32),
+        ("error", C.c_double),
+        ("converged", C.c_int32),
+        ("skipped", C.c_int32),
     ]
 
 
@@ -111,6 +122,10 @@ _SIGS = {
     "ft_get_normals": (C.c_int, [_H, _P]),
     "ft_interface_curvature": (C.c_int, [_H, _P, _P]),
     "ft_surface_tension": (C.c_int, [_H, C.c_float, C.c_int32, _P, _P, _P]),
+    "ft_advect": (C.c_int, [_H, _P, _P, C.c_double, C.c_double, C.c_int32, C.POINTER(FtAdvectInfo)]),
+    "ft_advect_device": (C.c_int, [_H, _P, _P, C.c_double, C.c_double, C.c_int32, C.POINTER(FtAdvectInfo),
+                                   _P]),
+    "ft_fem_assemble": (C.c_int, [_H, C.c_int32, _P, _P, _P, _P]),
     "ft_surface_tension_device": (C.c_int, [_H, C.c_float, C.c_int32, _P, _P, _P, _P]),
     "ft_get_counts": (C.c_int, [_H, _P]),
     "ft_write_interface_obj": (C.c_int, [_H, C.c_char_p, C.c_int32]),

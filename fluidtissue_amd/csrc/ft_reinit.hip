@@ -1543,6 +1543,435 @@ __global__ __launch_bounds__(kBS) void k_st_apply(const uint64_t* off, const int
 }
 
 // ============================================================================
+// Advection (LevelSet::update, LevelSet.cpp:89-141): Crank-Nicolson on the P1
+// Galerkin form, (M + s A) phi_n = (M - s A) phi with s = 0.5 dt, solved by
+// BiCGSTAB with a Jacobi preconditioner.
+//
+// Matrices are stored in sliced ELL: slices of kFemSlice rows, each slice
+// column-major over its width (the longest row in it), so that a wave reads
+// k-th entries of 64 rows in one coalesced access.  Rows keep the reference's
+// column order (increasing; Assembler::bilin_with_p2t fills a std::map), and
+// every row sum runs over it in that order.
+// ============================================================================
+constexpr int kFemSlice = 64;          // rows per ELL slice (one wave)
+constexpr int kFemMaxCols = 48;        // columns per row (the node and its tet neighbours)
+
+struct FemQuad {
+    double phi[16];                    // phi_j at quadrature point q: [4 j + q]
+    double w;                          // weight (0.25 / 6)
+};
+
+struct FemEll {
+    const int32_t* len;                // row lengths
+    const uint64_t* soff;              // slice offsets in units of kFemSlice entries
+    const int32_t* cols;               // ELL columns (padding: the row itself)
+};
+__device__ __forceinline__ int64_t ell_at(const FemEll& e, int64_t i, int k)
+{
+    return ((int64_t)e.soff[i / kFemSlice] + k) * kFemSlice + (i % kFemSlice);
+}
+
+// getElementInfo_ij_ (Assembler.hpp:429-459) and the mapped basis gradients
+// G[j] = invBt grad(phi_j) (the constant gradients (-1,-1,-1), e1, e2, e3,
+// Tetrahedron.hpp:36-50; sums left to right, as in k_tet_grad).  False for a
+// singular map, whose tet the reference leaves with an unset inverse.
+__device__ __forceinline__ bool fem_frame(const float* xyz, const int qi[4], double G[4][3], double& detB)
+{
+    double x[4][3];
+    for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 3; k++) x[j][k] = (double)xyz[3 * (int64_t)qi[j] + k];
+    double B[3][3], C[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) B[r][c] = x[c + 1][r] - x[0][r];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            C[r][c] = B[r1][c1] * B[r2][c2] - B[r1][c2] * B[r2][c1];
+        }
+    const double det = C[0][0] * B[0][0] + (C[1][0] * B[1][0] + C[2][0] * B[2][0]);
+    if (det == 0.0) return false;
+    const double inv = 1.0 / det;
+    for (int r = 0; r < 3; r++) {
+        const double b0 = C[r][0] * inv, b1 = C[r][1] * inv, b2 = C[r][2] * inv;
+        G[0][r] = (b0 * -1.0 + b1 * -1.0) + b2 * -1.0;
+        G[1][r] = b0;
+        G[2][r] = b1;
+        G[3][r] = b2;
+    }
+    detB = fabs(det);
+    return true;
+}
+
+// Sorted unique columns of row i (the node and every node of its tets) into
+// lst; returns the count, or -1 past kFemMaxCols.
+__device__ int fem_row_cols(const uint64_t* n2t_off, const int32_t* n2t_ids, const int4* tets, int64_t i,
+                            int32_t* lst)
+{
+    int cnt = 0;
+    for (uint64_t a = n2t_off[i]; a < n2t_off[i + 1]; a++) {
+        const int4 q = tets[n2t_ids[a]];
+        const int v4[4] = {q.x, q.y, q.z, q.w};
+        for (int j = 0; j < 4; j++) {
+            const int32_t v = v4[j];
+            int pos = 0;
+            while (pos < cnt && lst[pos] < v) pos++;
+            if (pos < cnt && lst[pos] == v) continue;
+            if (cnt == kFemMaxCols) return -1;
+            for (int k = cnt; k > pos; k--) lst[k] = lst[k - 1];
+            lst[pos] = v;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+// Pass 1 of the pattern: row lengths and slice widths.  One wave per slice.
+__global__ __launch_bounds__(kFemSlice) void k_fem_rowlen(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                          const int4* tets, int64_t N, int32_t* len,
+                                                          uint32_t* width, unsigned* overflow)
+{
+    __shared__ int32_t lst[kFemSlice][kFemMaxCols + 1];
+    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
+    int cnt = 0;
+    if (i < N) {
+        cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, lst[threadIdx.x]);
+        if (cnt < 0) { atomicOr(overflow, 1u); cnt = 0; }
+        len[i] = cnt;
+    }
+    int wmax = cnt;
+    for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+    if (threadIdx.x == 0) width[blockIdx.x] = (uint32_t)wmax;
+}
+
+// Pass 2: the ELL columns; padding entries name the row itself.
+__global__ __launch_bounds__(kFemSlice) void k_fem_rowfill(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                           const int4* tets, int64_t N, FemEll e,
+                                                           const uint32_t* width, int32_t* cols)
+{
+    __shared__ int32_t lst[kFemSlice][kFemMaxCols + 1];
+    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
+    if (i >= N) return;
+    const int cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, lst[threadIdx.x]);
+    const int W = (int)width[blockIdx.x];
+    for (int k = 0; k < W; k++) cols[ell_at(e, i, k)] = (k < cnt) ? lst[threadIdx.x][k] : (int32_t)i;
+}
+
+// Row i of the mass (FORM 0) or advection (FORM 1) form of
+// Assembler::bilin_with_p2t (Assembler.hpp:131-226) into acc[0..len): over
+// the tets of i in increasing id and their local nodes j, the integral
+// ((f0 w + f2 w) + (f1 w + f3 w)) |det B| of f(u = phi_j, v = phi_i) added to
+// column t(j):  mass  f = u v;  advection  f = V1 (ux v) + V2 (uy v) + V3 (uz v)
+// with V_c(q) = sum_j V_c[t(j)] phi_j(q) accumulated from 0 in j order.
+template <int FORM>
+__device__ void fem_row(const uint64_t* n2t_off, const int32_t* n2t_ids, const int4* tets,
+                        const float* xyz, const double* vel, int64_t N, const FemQuad& qd,
+                        const FemEll& e, int64_t i, int len, double* acc)
+{
+    for (int k = 0; k < len; k++) acc[k] = 0.0;
+    for (uint64_t a = n2t_off[i]; a < n2t_off[i + 1]; a++) {
+        const int4 q = tets[n2t_ids[a]];
+        const int qi[4] = {q.x, q.y, q.z, q.w};
+        double G[4][3], detB;
+        if (!fem_frame(xyz, qi, G, detB)) continue;
+        const int li = qi[0] == (int)i ? 0 : qi[1] == (int)i ? 1 : qi[2] == (int)i ? 2 : 3;
+        double V[3][4];
+        if (FORM == 1) {
+            for (int c = 0; c < 3; c++)
+                for (int q4 = 0; q4 < 4; q4++) {
+                    double acc_v = 0.0;
+                    for (int j = 0; j < 4; j++) acc_v = acc_v + vel[c * N + qi[j]] * qd.phi[4 * j + q4];
+                    V[c][q4] = acc_v;
+                }
+        }
+        for (int j = 0; j < 4; j++) {
+            double f[4];
+            for (int q4 = 0; q4 < 4; q4++) {
+                const double v = qd.phi[4 * li + q4];
+                if (FORM == 0)
+                    f[q4] = qd.phi[4 * j + q4] * v;
+                else
+                    f[q4] = (V[0][q4] * (G[j][0] * v) + V[1][q4] * (G[j][1] * v)) + V[2][q4] * (G[j][2] * v);
+            }
+            const double w = qd.w;
+            const double qint = ((f[0] * w + f[2] * w) + (f[1] * w + f[3] * w)) * detB;
+            for (int k = 0; k < len; k++)
+                if (e.cols[ell_at(e, i, k)] == qi[j]) { acc[k] += qint; break; }
+        }
+    }
+}
+
+// The mass matrix, once per mesh (LevelSet::init, LevelSet.cpp:35-39).
+__global__ __launch_bounds__(kFemSlice) void k_fem_mass(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                        const int4* tets, const float* xyz, int64_t N,
+                                                        FemQuad qd, FemEll e, double* mvals)
+{
+    __shared__ double acc[kFemSlice][kFemMaxCols + 1];
+    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
+    if (i >= N) return;
+    const int len = e.len[i];
+    fem_row<0>(n2t_off, n2t_ids, tets, xyz, nullptr, N, qd, e, i, len, acc[threadIdx.x]);
+    for (int k = 0; k < len; k++) mvals[ell_at(e, i, k)] = acc[threadIdx.x][k];
+}
+
+// The advection matrix alone (ft_fem_assemble), into vals in ELL layout.
+__global__ __launch_bounds__(kFemSlice) void k_fem_adv(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                       const int4* tets, const float* xyz, const double* vel,
+                                                       int64_t N, FemQuad qd, FemEll e, double* vals)
+{
+    __shared__ double acc[kFemSlice][kFemMaxCols + 1];
+    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
+    if (i >= N) return;
+    const int len = e.len[i];
+    fem_row<1>(n2t_off, n2t_ids, tets, xyz, vel, N, qd, e, i, len, acc[threadIdx.x]);
+    for (int k = 0; k < len; k++) vals[ell_at(e, i, k)] = acc[threadIdx.x][k];
+}
+
+// ELL rows to CSR at the row offsets off (a scan of the lengths).
+__global__ __launch_bounds__(kBS) void k_ell_to_csr(int64_t N, FemEll e, const double* vals, const uint64_t* off,
+                                                    int32_t* cols, double* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= N) return;
+    const int len = e.len[i];
+    for (int k = 0; k < len; k++) {
+        const int64_t at = ell_at(e, i, k);
+        cols[off[i] + k] = e.cols[at];
+        out[off[i] + k] = vals[at];
+    }
+}
+
+// Deterministic block sum of up to two values per thread into part[2 b], part[2 b + 1].
+template <int BS>
+__device__ __forceinline__ void block_sum2(double a, double b, double* part)
+{
+    __shared__ double red[2][BS / 64];
+    for (int o = 32; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = a; red[1][threadIdx.x >> 6] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0;
+        for (int w = 0; w < BS / 64; w++) { sa += red[0][w]; sb += red[1][w]; }
+        part[2 * blockIdx.x] = sa;
+        part[2 * blockIdx.x + 1] = sb;
+    }
+}
+
+// Per call: the advection row, then L = M + s A and R = M - s A entrywise
+// (LevelSet.cpp:126-127), invdiag = 1 / L_ii (1 where 0: Eigen's
+// DiagonalPreconditioner), b = R phi and r = b - L phi (BiCGSTAB's first
+// residual from the guess phi), each summed in column order; r0 = r, x = phi,
+// p = v = 0.  Block partials of r.r and b.b.
+__global__ __launch_bounds__(kFemSlice) void k_fem_cn(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                      const int4* tets, const float* xyz, const double* vel,
+                                                      int64_t N, FemQuad qd, FemEll e, const double* mvals,
+                                                      double s, const double* phi, double* lvals,
+                                                      double* invdiag, double* b, double* x, double* r,
+                                                      double* r0, double* p, double* v, double* part)
+{
+    __shared__ double acc[kFemSlice][kFemMaxCols + 1];
+    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
+    double rr = 0.0, bb = 0.0;
+    if (i < N) {
+        const int len = e.len[i];
+        double* A = acc[threadIdx.x];
+        fem_row<1>(n2t_off, n2t_ids, tets, xyz, vel, N, qd, e, i, len, A);
+        double bi = 0.0, lx = 0.0, id = 1.0;
+        for (int k = 0; k < len; k++) {
+            const int64_t at = ell_at(e, i, k);
+            const int32_t c = e.cols[at];
+            const double m = mvals[at], sa = s * A[k];
+            const double L = m + sa, R = m - sa;
+            lvals[at] = L;
+            bi += R * phi[c];
+            lx += L * phi[c];
+            if (c == (int32_t)i && L != 0.0) id = 1.0 / L;
+        }
+        const double ri = bi - lx;
+        invdiag[i] = id;
+        b[i] = bi;
+        x[i] = phi[i];
+        r[i] = ri;
+        r0[i] = ri;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        rr = ri * ri;
+        bb = bi * bi;
+    }
+    block_sum2<kFemSlice>(rr, bb, part);
+}
+
+// ---- BiCGSTAB (Eigen's bicgstab; H. A. van der Vorst 1992) ----------------
+// Scalars live on the device; each finishing kernel reduces the block
+// partials (in block order) and updates them.
+struct BcgScalars {
+    double rho, rho_old, alpha, w;
+    double rr, bb, r0sq;
+    double r0v, tt, ts;
+};
+
+constexpr int kBcgBS = 256;
+
+// p = r + beta (p - w v), beta = (rho / rho_old) (alpha / w).
+__global__ __launch_bounds__(kBcgBS) void k_bcg_p(int64_t N, const BcgScalars* sc, const double* r,
+                                                  const double* v, double* p)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    if (i >= N) return;
+    const double beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->w);
+    p[i] = r[i] + beta * (p[i] - sc->w * v[i]);
+}
+
+// v = L y with y = invdiag p (y = P p, then v = A y); partial r0.v.
+__global__ __launch_bounds__(kBcgBS) void k_bcg_v(int64_t N, FemEll e, const double* lvals,
+                                                  const double* invdiag, const double* p, const double* r0,
+                                                  double* v, double* part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    double d = 0.0;
+    if (i < N) {
+        const int len = e.len[i];
+        double t = 0.0;
+        for (int k = 0; k < len; k++) {
+            const int64_t at = ell_at(e, i, k);
+            const int32_t c = e.cols[at];
+            t += lvals[at] * (invdiag[c] * p[c]);
+        }
+        v[i] = t;
+        d = r0[i] * t;
+    }
+    block_sum2<kBcgBS>(d, 0.0, part);
+}
+
+// t = L z with z = invdiag s, s = r - alpha v (formed where read); partials
+// t.t and t.s.
+__global__ __launch_bounds__(kBcgBS) void k_bcg_t(int64_t N, FemEll e, const double* lvals,
+                                                  const double* invdiag, const double* r, const double* v,
+                                                  const BcgScalars* sc, double* t, double* part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    double tt = 0.0, ts = 0.0;
+    if (i < N) {
+        const double alpha = sc->alpha;
+        const int len = e.len[i];
+        double acc = 0.0;
+        for (int k = 0; k < len; k++) {
+            const int64_t at = ell_at(e, i, k);
+            const int32_t c = e.cols[at];
+            acc += lvals[at] * (invdiag[c] * (r[c] - alpha * v[c]));
+        }
+        t[i] = acc;
+        tt = acc * acc;
+        ts = acc * (r[i] - alpha * v[i]);
+    }
+    block_sum2<kBcgBS>(tt, ts, part);
+}
+
+// x += alpha y + w z;  r = s - w t;  partials r.r and r0.r.
+__global__ __launch_bounds__(kBcgBS) void k_bcg_x(int64_t N, const double* invdiag, const double* p,
+                                                  const double* v, const double* t, const double* r0,
+                                                  const BcgScalars* sc, double* x, double* r, double* part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    double rr = 0.0, r0r = 0.0;
+    if (i < N) {
+        const double alpha = sc->alpha, w = sc->w;
+        const double y = invdiag[i] * p[i];
+        const double si = r[i] - alpha * v[i];
+        const double z = invdiag[i] * si;
+        x[i] = x[i] + (alpha * y + w * z);
+        const double ri = si - w * t[i];
+        r[i] = ri;
+        rr = ri * ri;
+        r0r = r0[i] * ri;
+    }
+    block_sum2<kBcgBS>(rr, r0r, part);
+}
+
+// Restart (a residual too orthogonal to r0): r = b - L x, r0 = r; partial r.r.
+__global__ __launch_bounds__(kBcgBS) void k_bcg_restart(int64_t N, FemEll e, const double* lvals,
+                                                        const double* b, const double* x, double* r,
+                                                        double* r0, double* part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    double rr = 0.0;
+    if (i < N) {
+        const int len = e.len[i];
+        double t = 0.0;
+        for (int k = 0; k < len; k++) {
+            const int64_t at = ell_at(e, i, k);
+            t += lvals[at] * x[e.cols[at]];
+        }
+        const double ri = b[i] - t;
+        r[i] = ri;
+        r0[i] = ri;
+        rr = ri * ri;
+    }
+    block_sum2<kBcgBS>(rr, 0.0, part);
+}
+
+enum BcgStep : int { BCG_INIT = 0, BCG_ALPHA = 1, BCG_W = 2, BCG_END = 3, BCG_RESTART = 4, BCG_SUMS = 5 };
+
+// Sums nb block partial pairs in block order (one workgroup: thread k sums a
+// strided subset in order, then the 256 results in order) and updates the
+// scalars for the step.
+__global__ __launch_bounds__(256) void k_bcg_fin(const double* part, int64_t nb, int step, BcgScalars* sc)
+{
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (int64_t k = threadIdx.x; k < nb; k += 256) { a += part[2 * k]; b += part[2 * k + 1]; }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double A = 0.0, B = 0.0;
+    for (int k = 0; k < 256; k++) { A += red[0][k]; B += red[1][k]; }
+    switch (step) {
+    case BCG_INIT:                       // r.r (= r0.r0 = r0.r), b.b; rho = alpha = w = 1
+        sc->rr = A; sc->r0sq = A; sc->bb = B;
+        sc->alpha = 1.0; sc->w = 1.0;
+        sc->rho_old = 1.0; sc->rho = A;  // the first iteration's rho_old = rho, rho = r0.r
+        break;
+    case BCG_ALPHA:
+        sc->r0v = A;
+        sc->alpha = sc->rho / A;
+        break;
+    case BCG_W:
+        sc->tt = A; sc->ts = B;
+        sc->w = A > 0.0 ? B / A : 0.0;
+        break;
+    case BCG_END:                        // r.r, and the next iteration's rho_old, rho
+        sc->rr = A;
+        sc->rho_old = sc->rho;
+        sc->rho = B;
+        break;
+    case BCG_RESTART:                    // rho = |r0|^2 = |r|^2 (rho_old kept)
+        sc->rr = A; sc->r0sq = A; sc->rho = A;
+        break;
+    case BCG_SUMS:                       // the two sums as they are
+        sc->tt = A; sc->ts = B;
+        break;
+    }
+}
+
+// |flow|^2 over 4 N entries and whether the velocity (first 3 N) has a NaN.
+__global__ __launch_bounds__(kBcgBS) void k_flow_check(const double* flow, int64_t N, double* part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    double sq = 0.0, nan = 0.0;
+    if (i < N) {
+        for (int c = 0; c < 4; c++) {
+            const double f = flow[c * N + i];
+            sq += f * f;
+            if (c < 3 && f != f) nan = 1.0;
+        }
+    }
+    block_sum2<kBcgBS>(sq, nan, part);
+}
+
+// ============================================================================
 // Per-field interface counts (batches): ranks at the field boundaries
 // ============================================================================
 __device__ __forceinline__ uint64_t low_mask(int b) { return b ? ((1ull << b) - 1ull) : 0ull; }
@@ -1769,6 +2198,13 @@ struct ft_ctx {
     DevBuf gsplit, kmesh, k_if;      // curvature: unit-normal components, node and point curvature
     DevBuf st_cnt, st_off, st_ids;   // surface tension: node -> crossing-point terms
     DevBuf st_f;                     // surface tension: device copy of host force arrays
+    // advection: sliced-ELL pattern and mass matrix (per layout), the
+    // Crank-Nicolson matrix, BiCGSTAB vectors, block partials and scalars
+    DevBuf fem_len, fem_w, fem_soff, fem_cols, fem_m, fem_l, fem_flag;
+    DevBuf bcg_vec, bcg_part, bcg_sc, adv_phi, adv_flow, fem_csr_off;
+    uint64_t fem_layout = ~0ull;
+    int64_t fem_nnz = 0;             // ELL entries with padding
+    void* h_bcg = nullptr;           // pinned mirror of the solver scalars
     const double* last_phi = nullptr;   // device phi of the last reinit (blended)
     uint64_t n2t_layout = ~0ull;
     bool have_normals = false;
@@ -1816,7 +2252,9 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
             &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
             &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
-            &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f};
+            &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f,
+            &h->fem_len, &h->fem_w, &h->fem_soff, &h->fem_cols, &h->fem_m, &h->fem_l, &h->fem_flag,
+            &h->bcg_vec, &h->bcg_part, &h->bcg_sc, &h->adv_phi, &h->adv_flow, &h->fem_csr_off};
 }
 
 std::vector<void*> buffer_addresses(ft_ctx* c)
@@ -2787,6 +3225,7 @@ int ft_destroy(ft_handle h)
     drop_graph(h);
     for (DevBuf* b : all_buffers(h)) release(*b);
     if (h->h_scalars) (void)hipHostFree(h->h_scalars);
+    if (h->h_bcg) (void)hipHostFree(h->h_bcg);
     for (int k = 0; k < 5; k++)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     (void)hipStreamDestroy(h->stream);
@@ -3278,6 +3717,241 @@ int ft_surface_tension(ft_handle h, float st, int32_t rho, double* fx, double* f
         FT_HIP(hipMemcpyAsync(host[c], d + c * N, 8 * N, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
     return rc;
+}
+
+// ---- Advection (LevelSet::update, LevelSet.cpp:89-141) ---------------------
+// Quadrature(2, P1) (Quadrature.hpp:23-36) at the float shape functions
+// (Tetrahedron.hpp:30-34): the point coordinates go in as float and phi_0 =
+// 1.0 - x - y - z comes back rounded to float.
+FemQuad fem_quad()
+{
+    FemQuad q;
+    const double a = (5.0 + 3.0 * std::sqrt(5.0)) / 20.0, b = (5.0 - std::sqrt(5.0)) / 20.0;
+    const double qp[3][4] = {{a, b, b, b}, {b, a, b, b}, {b, b, a, b}};
+    for (int k = 0; k < 4; k++) {
+        const float x = (float)qp[0][k], y = (float)qp[1][k], z = (float)qp[2][k];
+        q.phi[0 * 4 + k] = (double)(float)(((1.0 - (double)x) - (double)y) - (double)z);
+        q.phi[1 * 4 + k] = (double)x;
+        q.phi[2 * 4 + k] = (double)y;
+        q.phi[3 * 4 + k] = (double)z;
+    }
+    q.w = 0.25 / 6.0;
+    return q;
+}
+
+FemEll fem_ell(ft_ctx* h)
+{
+    return FemEll{h->fem_len.as<int32_t>(), h->fem_soff.as<uint64_t>(), h->fem_cols.as<int32_t>()};
+}
+
+// The sliced-ELL pattern and the mass matrix (LevelSet::init,
+// LevelSet.cpp:35-39), once per mesh layout.
+int fem_setup(ft_ctx* h, hipStream_t s)
+{
+    if (h->fem_layout == h->layout_gen) return FT_OK;
+    if (!h->has_conn) return ft::set_error(FT_ERR_STATE, "advection needs a mesh (ft_set_mesh)");
+    FT_TRY(build_node_tets(h, s));
+    const int64_t N = h->N, ns = (N + kFemSlice - 1) / kFemSlice;
+    FT_TRY(ensure(h->fem_len, 4 * at_least1(N)));
+    FT_TRY(ensure(h->fem_w, 4 * at_least1(ns)));
+    FT_TRY(ensure(h->fem_soff, 8 * (ns + 1)));
+    FT_TRY(ensure(h->fem_flag, 16));
+    uint32_t* w = h->fem_w.as<uint32_t>();
+    uint64_t* soff = h->fem_soff.as<uint64_t>();
+    FT_TRY(fill<unsigned>(s, h->fem_flag.as<unsigned>(), 1, 0u));
+    FT_TRY(fill<uint64_t>(s, soff + ns, 1, 0ull));
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_fem_rowlen, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
+                           h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), N, h->fem_len.as<int32_t>(), w,
+                           h->fem_flag.as<unsigned>());
+        FT_TRY(scan<uint32_t, uint64_t>(h, s, w, soff, ns, soff + ns, LoadSame{}));
+    }
+    uint64_t total = 0;
+    unsigned flag = 0;
+    FT_HIP(hipMemcpyAsync(&total, soff + ns, 8, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipMemcpyAsync(&flag, h->fem_flag.p, 4, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
+    if (flag)
+        return ft::set_error(FT_ERR_ARG, "a node shares tets with more than %d nodes", kFemMaxCols - 1);
+    const int64_t nnz = (int64_t)total * kFemSlice;
+    FT_TRY(ensure(h->fem_cols, 4 * at_least1(nnz)));
+    FT_TRY(ensure(h->fem_m, 8 * at_least1(nnz)));
+    FT_TRY(ensure(h->fem_l, 8 * at_least1(nnz)));
+    const FemEll e = fem_ell(h);
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_fem_rowfill, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
+                           h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), N, e, w, h->fem_cols.as<int32_t>());
+        hipLaunchKernelGGL(k_fem_mass, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
+                           h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), h->xyz.as<float>(), N, fem_quad(), e,
+                           h->fem_m.as<double>());
+    }
+    FT_TRY(check_launch());
+    FT_HIP(hipStreamSynchronize(s));
+    h->fem_nnz = nnz;
+    h->fem_layout = h->layout_gen;
+    return FT_OK;
+}
+
+int bcg_fin(ft_ctx* h, hipStream_t s, int64_t nb, int step, bool read)
+{
+    hipLaunchKernelGGL(k_bcg_fin, dim3(1), dim3(256), 0, s, h->bcg_part.as<double>(), nb, step,
+                       h->bcg_sc.as<BcgScalars>());
+    if (read) {
+        FT_HIP(hipMemcpyAsync(h->h_bcg, h->bcg_sc.p, sizeof(BcgScalars), hipMemcpyDeviceToHost, s));
+        FT_HIP(hipStreamSynchronize(s));
+    }
+    return FT_OK;
+}
+
+// Crank-Nicolson step of LevelSet::update on device phi (in/out) and flow
+// (4 N: velocity x, y, z, pressure), BiCGSTAB to tol in at most max_iters.
+int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol, int32_t max_iters,
+           ft_advect_info* info, hipStream_t s)
+{
+    *info = ft_advect_info{};
+    if (h->nf != 1) return ft::set_error(FT_ERR_STATE, "advection needs a single field");
+    if (!h->has_conn) return ft::set_error(FT_ERR_STATE, "advection needs a mesh (ft_set_mesh)");
+    const int64_t N = h->N;
+    if (dt < FLT_MIN || N == 0) { info->skipped = 1; info->converged = 1; return FT_OK; }   // :92-93
+    const int64_t nb = (N + kBcgBS - 1) / kBcgBS, ns = (N + kFemSlice - 1) / kFemSlice;
+    FT_TRY(ensure(h->bcg_part, 16 * (size_t)std::max(nb, ns)));
+    FT_TRY(ensure(h->bcg_sc, sizeof(BcgScalars)));
+    if (!h->h_bcg) FT_HIP(hipHostMalloc(&h->h_bcg, sizeof(BcgScalars), hipHostMallocDefault));
+    BcgScalars* hs = static_cast<BcgScalars*>(h->h_bcg);
+    double* part = h->bcg_part.as<double>();
+    BcgScalars* sc = h->bcg_sc.as<BcgScalars>();
+    // flow.norm() < FLT_MIN: nothing to do (:93); NaN velocity: error (:112-115)
+    hipLaunchKernelGGL(k_flow_check, dim3((unsigned)nb), dim3(kBcgBS), 0, s, d_flow, N, part);
+    FT_TRY(bcg_fin(h, s, nb, BCG_SUMS, true));
+    if (std::sqrt(hs->tt) < FLT_MIN) { info->skipped = 1; info->converged = 1; return FT_OK; }
+    if (hs->ts > 0.0) return ft::set_error(FT_ERR_ARG, "the velocity has NaN (LevelSet.cpp:112-115)");
+    FT_TRY(fem_setup(h, s));
+    FT_TRY(ensure(h->bcg_vec, 8 * 8 * (size_t)N));
+    double* vec = h->bcg_vec.as<double>();
+    double *x = vec, *r = vec + N, *r0 = vec + 2 * N, *p = vec + 3 * N, *v = vec + 4 * N, *t = vec + 5 * N,
+           *invdiag = vec + 6 * N, *b = vec + 7 * N;
+    const FemEll e = fem_ell(h);
+    double* lv = h->fem_l.as<double>();
+    hipLaunchKernelGGL(k_fem_cn, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
+                       h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), h->xyz.as<float>(), d_flow, N, fem_quad(), e,
+                       h->fem_m.as<double>(), 0.5 * dt, d_phi, lv, invdiag, b, x, r, r0, p, v, part);
+    FT_TRY(check_launch());
+    FT_TRY(bcg_fin(h, s, ns, BCG_INIT, true));
+    const double tol_ = tol > 0.0 ? tol : DBL_EPSILON;                 // Eigen's defaults
+    const int64_t max_it = max_iters > 0 ? max_iters : 2 * N;
+    if (hs->bb == 0.0) {                                               // zero right-hand side: x = 0
+        FT_TRY(fill<double>(s, d_phi, N, 0.0));
+        FT_HIP(hipStreamSynchronize(s));
+        info->iterations = (int32_t)std::min<int64_t>(max_it, INT32_MAX);
+        info->error = tol_;
+        info->converged = 1;
+        return FT_OK;
+    }
+    const double tol2 = tol_ * tol_ * hs->bb, eps2 = DBL_EPSILON * DBL_EPSILON;
+    int64_t it = 0, restarts = 0;
+    const dim3 g((unsigned)nb), bs(kBcgBS);
+    while (hs->rr > tol2 && it < max_it) {
+        if (std::fabs(hs->rho) < eps2 * hs->r0sq) {
+            hipLaunchKernelGGL(k_bcg_restart, g, bs, 0, s, N, e, lv, b, x, r, r0, part);
+            FT_TRY(bcg_fin(h, s, nb, BCG_RESTART, false));
+            if (restarts++ == 0) it = 0;
+        }
+        hipLaunchKernelGGL(k_bcg_p, g, bs, 0, s, N, sc, r, v, p);
+        hipLaunchKernelGGL(k_bcg_v, g, bs, 0, s, N, e, lv, invdiag, p, r0, v, part);
+        FT_TRY(bcg_fin(h, s, nb, BCG_ALPHA, false));
+        hipLaunchKernelGGL(k_bcg_t, g, bs, 0, s, N, e, lv, invdiag, r, v, sc, t, part);
+        FT_TRY(bcg_fin(h, s, nb, BCG_W, false));
+        hipLaunchKernelGGL(k_bcg_x, g, bs, 0, s, N, invdiag, p, v, t, r0, sc, x, r, part);
+        FT_TRY(bcg_fin(h, s, nb, BCG_END, true));
+        it++;
+    }
+    FT_TRY(check_launch());
+    info->iterations = (int32_t)std::min<int64_t>(it, INT32_MAX);
+    info->restarts = (int32_t)restarts;
+    info->error = std::sqrt(hs->rr / hs->bb);
+    info->converged = info->error <= tol_;
+    if (!info->converged)                                              // :135-139, phi kept
+        return ft::set_error(FT_ERR_SOLVER, "BiCGSTAB stopped at |r|/|b| = %g > %g after %lld iterations",
+                             info->error, tol_, (long long)it);
+    FT_HIP(hipMemcpyAsync(d_phi, x, 8 * N, hipMemcpyDeviceToDevice, s));
+    FT_HIP(hipStreamSynchronize(s));
+    return FT_OK;
+}
+
+int ft_advect_device(ft_handle h, double* d_phi, const double* d_flow, double dt, double tol,
+                     int32_t max_iters, ft_advect_info* info, void* stream)
+{
+    FT_TRY(check_handle(h));
+    if (h->N > 0 && (!d_phi || !d_flow)) return ft::set_error(FT_ERR_ARG, "null phi or flow");
+    ft_advect_info tmp;
+    return advect(h, d_phi, d_flow, dt, tol, max_iters, info ? info : &tmp,
+                  stream ? (hipStream_t)stream : h->stream);
+}
+
+int ft_advect(ft_handle h, double* phi, const double* flow, double dt, double tol, int32_t max_iters,
+              ft_advect_info* info)
+{
+    FT_TRY(check_handle(h));
+    const int64_t N = h->N;
+    if (N > 0 && (!phi || !flow)) return ft::set_error(FT_ERR_ARG, "null phi or flow");
+    ft_advect_info tmp;
+    if (!info) info = &tmp;
+    hipStream_t s = h->stream;
+    FT_TRY(ensure(h->adv_phi, 8 * at_least1(N)));
+    FT_TRY(ensure(h->adv_flow, 32 * at_least1(N)));
+    if (N > 0) {
+        FT_HIP(hipMemcpyAsync(h->adv_phi.p, phi, 8 * N, hipMemcpyHostToDevice, s));
+        FT_HIP(hipMemcpyAsync(h->adv_flow.p, flow, 32 * N, hipMemcpyHostToDevice, s));
+    }
+    FT_TRY(advect(h, h->adv_phi.as<double>(), h->adv_flow.as<double>(), dt, tol, max_iters, info, s));
+    if (N > 0 && !info->skipped) {
+        FT_HIP(hipMemcpyAsync(phi, h->adv_phi.p, 8 * N, hipMemcpyDeviceToHost, s));
+        FT_HIP(hipStreamSynchronize(s));
+    }
+    return FT_OK;
+}
+
+int ft_fem_assemble(ft_handle h, int32_t form, const double* vel, int64_t* row_off, int32_t* cols,
+                    double* vals)
+{
+    FT_TRY(check_handle(h));
+    if ((form != 0 && form != 1) || !row_off || (cols && !vals) || (form == 1 && cols && !vel))
+        return ft::set_error(FT_ERR_ARG, "ft_fem_assemble: bad arguments");
+    if (h->nf != 1) return ft::set_error(FT_ERR_STATE, "needs a single field");
+    hipStream_t s = h->stream;
+    FT_TRY(fem_setup(h, s));
+    const int64_t N = h->N;
+    FT_TRY(ensure(h->fem_csr_off, 8 * (N + 1)));
+    uint64_t* off = h->fem_csr_off.as<uint64_t>();
+    FT_TRY(fill<uint64_t>(s, off + N, 1, 0ull));
+    if (N > 0) FT_TRY(scan<int32_t, uint64_t>(h, s, h->fem_len.as<int32_t>(), off, N, off + N, LoadSame{}));
+    FT_HIP(hipMemcpyAsync(row_off, off, 8 * (N + 1), hipMemcpyDeviceToHost, s));
+    FT_HIP(hipStreamSynchronize(s));
+    if (!cols) return FT_OK;                                           // sizes only
+    const int64_t nnz = row_off[N];
+    const FemEll e = fem_ell(h);
+    const double* src = h->fem_m.as<double>();
+    if (form == 1) {
+        FT_TRY(ensure(h->adv_flow, 32 * at_least1(N)));
+        FT_HIP(hipMemcpyAsync(h->adv_flow.p, vel, 24 * N, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_fem_adv, dim3((unsigned)((N + kFemSlice - 1) / kFemSlice)), dim3(kFemSlice), 0, s,
+                           h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->tets.as<int4>(),
+                           h->xyz.as<float>(), h->adv_flow.as<double>(), N, fem_quad(), e, h->fem_l.as<double>());
+        src = h->fem_l.as<double>();
+    }
+    const int64_t cbytes = (4 * at_least1(nnz) + 255) & ~(int64_t)255;   // CSR staging: cols, vals
+    FT_TRY(ensure(h->bcg_vec, cbytes + 8 * at_least1(nnz)));
+    int32_t* dc = h->bcg_vec.as<int32_t>();
+    double* dv = reinterpret_cast<double*>(h->bcg_vec.as<char>() + cbytes);
+    if (N > 0)
+        hipLaunchKernelGGL(k_ell_to_csr, dim3(grid1d(N)), dim3(kBS), 0, s, N, e, src, off, dc, dv);
+    FT_TRY(check_launch());
+    if (nnz > 0) {
+        FT_HIP(hipMemcpyAsync(cols, dc, 4 * nnz, hipMemcpyDeviceToHost, s));
+        FT_HIP(hipMemcpyAsync(vals, dv, 8 * nnz, hipMemcpyDeviceToHost, s));
+    }
+    FT_HIP(hipStreamSynchronize(s));
+    return FT_OK;
 }
 
 int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches)

@@ -333,6 +333,46 @@ class Reinit(_Handle):
         check(lib.ft_interface_curvature(self._h, _ptr(out), C.c_void_p(stream) if stream else None))
         return out
 
+    def fem_matrix(self, form, vel=None):
+        """Assembled mass (form 0) or advection (form 1, velocity vel: 3n
+        floats, x then y then z) matrix in CSR: (row_off, cols, vals)."""
+        row_off = np.empty(self.n + 1, np.int64)
+        check(lib.ft_fem_assemble(self._h, int(form), None, _ptr(row_off), None, None))
+        nnz = int(row_off[-1])
+        cols = np.empty(max(nnz, 1), np.int32)
+        vals = np.empty(max(nnz, 1), np.float64)
+        v = None if vel is None else _c(vel, np.float64)
+        if v is not None and v.shape[0] < 3 * self.n:
+            raise ValueError("velocity: 3 values per node")
+        check(lib.ft_fem_assemble(self._h, int(form), _ptr(v), _ptr(row_off), _ptr(cols), _ptr(vals)))
+        return row_off, cols[:nnz], vals[:nnz]
+
+    def advect(self, phi, flow, dt, tol=1e-10, max_iters=1000):
+        """The advection step of LevelSet::update: Crank-Nicolson with
+        BiCGSTAB.  flow: 4n floats (velocity x, y, z, then pressure).
+        Returns (phi_n, info dict); FtError(FT_ERR_SOLVER) if the solve stops
+        above tol."""
+        phi_n = np.array(phi, np.float64, copy=True, order="C")
+        flow = _c(flow, np.float64)
+        if phi_n.shape != (self.n,) or flow.shape != (4 * self.n,):
+            raise ValueError("phi: one value per node; flow: four per node")
+        info = _lib.FtAdvectInfo()
+        check(lib.ft_advect(self._h, _ptr(phi_n), _ptr(flow), float(dt), float(tol), int(max_iters),
+                            C.byref(info)))
+        return phi_n, dict(iterations=info.iterations, restarts=info.restarts, error=info.error,
+                           converged=bool(info.converged), skipped=bool(info.skipped))
+
+    def update(self, phi, flow, dt, tol=1e-10, max_iters=1000, mode="points", **kw):
+        """LevelSet::update (LevelSet.cpp:89-152): the advection, then the
+        reinit with the update blend.  A skipped advection (dt or |flow| below
+        FLT_MIN) does nothing, as the reference returns before the reinit.
+        Returns (phi_out, phi_ifn, counts, advection info)."""
+        phi_a, info = self.advect(phi, flow, dt, tol, max_iters)
+        if info["skipped"]:
+            return phi_a, None, None, info
+        out, ifn, cnt = self.reset_phi(phi_a, mode=mode, blend="update", **kw)
+        return out, ifn, cnt, info
+
     def surface_tension(self, st, rho, fx=None, fy=None, fz=None):
         """Flow::surfaceTension_ after a reinit with want_normals: adds the
         surface-tension force into fx, fy, fz (float64, one per node; zeros

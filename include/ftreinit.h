@@ -50,6 +50,7 @@ extern "C" {
                                   the tet there (LevelSet.cpp:338-341)        */
 #define FT_ERR_STATE       -7  /* call order (e.g. reinit before set_mesh)    */
 #define FT_ERR_BUDGET      -8  /* request exceeds the device memory budget    */
+#define FT_ERR_SOLVER      -9  /* the advection solve did not converge        */
 
 /* ---- options ----------------------------------------------------------- */
 #define FT_DIST_POINTS     0   /* reference semantics: nearest crossing point
@@ -188,6 +189,41 @@ int ft_interface_curvature(ft_handle h, float* k_if, void* stream);
 int ft_surface_tension(ft_handle h, float st, int32_t rho, double* fx, double* fy, double* fz);
 int ft_surface_tension_device(ft_handle h, float st, int32_t rho, double* fx, double* fy,
                               double* fz, void* stream);
+
+/* ---- Advection: the step before the reinit in LevelSet::update ----------
+ * (LevelSet.cpp:89-141.)  Crank-Nicolson on the P1 Galerkin form of
+ * phi_t + V.grad(phi) = 0 over the handle's mesh (one field):
+ *   (M + s A) phi_n = (M - s A) phi,  s = 0.5 dt,
+ * M the mass matrix (assembled once per mesh, LevelSet.cpp:35-39), A the
+ * advection matrix of the velocity (Assembler::assemble_bilin with
+ * bilin_with_p2t, Assembler.hpp:131-226, :107-117), solved by BiCGSTAB with a
+ * Jacobi preconditioner from the guess phi (Eigen::BiCGSTAB, LevelSet.hpp:145).
+ * The matrices are the reference's to the bit (row order, sums in column
+ * order); the solver's dot products are reduced in a fixed tree, so phi_n
+ * agrees with a CPU solve to the solver tolerance.
+ * flow: 4 n_nodes doubles, velocity x, y, z then pressure (Flow's layout).
+ * tol <= 0 and max_iters <= 0 take Eigen's defaults (machine epsilon, 2 n).
+ * dt < FLT_MIN or |flow| < FLT_MIN: nothing is done (skipped = 1).  A NaN in
+ * the velocity is FT_ERR_ARG.  If the solve stops above tol, FT_ERR_SOLVER
+ * and phi is left as it was (the reference returns -1 before its swap). */
+typedef struct ft_advect_info {
+    int32_t iterations;   /* solver_.iterations()                          */
+    int32_t restarts;     /* BiCGSTAB restarts (r0 too orthogonal to r)     */
+    double  error;        /* solver_.error(): |b - L phi_n| / |b|           */
+    int32_t converged;    /* error <= tol                                   */
+    int32_t skipped;      /* dt or |flow| below FLT_MIN: phi untouched      */
+} ft_advect_info;
+int ft_advect(ft_handle h, double* phi, const double* flow, double dt, double tol,
+              int32_t max_iters, ft_advect_info* info);
+/* The same on device arrays, on stream (null: the handle's). */
+int ft_advect_device(ft_handle h, double* d_phi, const double* d_flow, double dt, double tol,
+                     int32_t max_iters, ft_advect_info* info, void* stream);
+/* The assembled mass (form 0) or advection (form 1, vel = 3 n_nodes doubles,
+ * host) matrix in CSR, rows in node order, columns increasing: row_off
+ * (n_nodes + 1) is always written; with cols == NULL only the sizes are
+ * (nnz = row_off[n_nodes]).  Assembler::assemble_bilin of LevelSet's forms. */
+int ft_fem_assemble(ft_handle h, int32_t form, const double* vel, int64_t* row_off, int32_t* cols,
+                    double* vals);
 
 /* Totals of the last successful reinit on h (sizes for ft_get_interface). */
 int ft_get_counts(ft_handle h, ft_counts* total);

@@ -537,3 +537,172 @@ void ora_surface_tension(const float* g, const double* phi, int64_t n_ifp,
         fz[e1] += (double)(f1 * n[2]);
     }
 }
+
+
+/* ---------------------------------------------------------------------- */
+/* LevelSet::update, advection (LevelSet.cpp:89-141; Assembler.hpp:131-226) */
+/* ---------------------------------------------------------------------- */
+void ora_fem_quad(double* phi, double* w)
+{
+    const double a = (5.0 + 3.0 * sqrt(5.0)) / 20.0, b = (5.0 - sqrt(5.0)) / 20.0;
+    const double qp[3][4] = {{a, b, b, b}, {b, a, b, b}, {b, b, a, b}};
+    for (int q = 0; q < 4; q++) {
+        const float x = (float)qp[0][q], y = (float)qp[1][q], z = (float)qp[2][q];
+        phi[0 * 4 + q] = (double)(float)(((1.0 - (double)x) - (double)y) - (double)z);
+        phi[1 * 4 + q] = (double)x;
+        phi[2 * 4 + q] = (double)y;
+        phi[3 * 4 + q] = (double)z;
+    }
+    *w = 0.25 / 6.0;
+}
+
+/* getElementInfo_ij_ (Assembler.hpp:429-459) as in ora_tet_grad, and the
+ * mapped basis gradients G[j]; returns 0 for a singular map. */
+static int fem_frame(const float* p, const int32_t* q, double G[4][3], double* detB)
+{
+    double x[4][3];
+    for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 3; k++) x[j][k] = (double)p[3 * (int64_t)q[j] + k];
+    double B[3][3], C[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) B[r][c] = x[c + 1][r] - x[0][r];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            C[r][c] = B[r1][c1] * B[r2][c2] - B[r1][c2] * B[r2][c1];
+        }
+    const double det = C[0][0] * B[0][0] + (C[1][0] * B[1][0] + C[2][0] * B[2][0]);
+    if (det == 0.0) return 0;
+    const double inv = 1.0 / det;
+    for (int r = 0; r < 3; r++) {
+        const double b0 = C[r][0] * inv, b1 = C[r][1] * inv, b2 = C[r][2] * inv;
+        G[0][r] = (b0 * -1.0 + b1 * -1.0) + b2 * -1.0;
+        G[1][r] = b0;
+        G[2][r] = b1;
+        G[3][r] = b2;
+    }
+    *detB = fabs(det);
+    return 1;
+}
+
+int64_t ora_fem_assemble(int form, int64_t n, const float* p, const int32_t* tets,
+                         const int64_t* n2t_off, const int32_t* n2t_ids,
+                         const int64_t* row_off, const int32_t* cols, const double* vel,
+                         double* vals)
+{
+    double phi[16], w;
+    ora_fem_quad(phi, &w);
+    int64_t bad = -1;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t rb = row_off[i], re = row_off[i + 1];
+        for (int64_t c = rb; c < re; c++) vals[c] = 0.0;
+        for (int64_t a = n2t_off[i]; a < n2t_off[i + 1]; a++) {
+            const int32_t k = n2t_ids[a];
+            const int32_t* q = tets + 4 * (int64_t)k;
+            double G[4][3], detB;
+            if (!fem_frame(p, q, G, &detB)) {
+                if (bad < 0 || k < bad) bad = k;
+                continue;
+            }
+            int li = 0;
+            while (li < 3 && q[li] != (int32_t)i) li++;
+            double V[3][4] = {{0.0}};
+            if (form == 1)
+                for (int c = 0; c < 3; c++)
+                    for (int q4 = 0; q4 < 4; q4++)
+                        for (int j = 0; j < 4; j++)
+                            V[c][q4] = V[c][q4] + vel[c * n + q[j]] * phi[4 * j + q4];
+            for (int j = 0; j < 4; j++) {
+                double f[4];
+                for (int q4 = 0; q4 < 4; q4++) {
+                    const double v = phi[4 * li + q4];
+                    if (form == 0)
+                        f[q4] = phi[4 * j + q4] * v;
+                    else
+                        f[q4] = (V[0][q4] * (G[j][0] * v) + V[1][q4] * (G[j][1] * v)) +
+                                V[2][q4] * (G[j][2] * v);
+                }
+                const double qint = ((f[0] * w + f[2] * w) + (f[1] * w + f[3] * w)) * detB;
+                int64_t c = rb;
+                while (c < re && cols[c] != q[j]) c++;
+                if (c < re) vals[c] += qint;
+            }
+        }
+    }
+    return bad;
+}
+
+void ora_spmv(int64_t n, const int64_t* row_off, const int32_t* cols, const double* vals,
+              const double* x, double* y)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double t = 0.0;
+        for (int64_t c = row_off[i]; c < row_off[i + 1]; c++) t += vals[c] * x[cols[c]];
+        y[i] = t;
+    }
+}
+
+static double dotv(int64_t n, const double* a, const double* b)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+int ora_bicgstab(int64_t n, const int64_t* row_off, const int32_t* cols, const double* vals,
+                 const double* b, double* x, double tol, int64_t max_iters, int64_t* iters,
+                 double* error, int64_t* restarts)
+{
+    const size_t sz = sizeof(double) * (size_t)(n > 0 ? n : 1);
+    double* invd = (double*)malloc(sz);
+    double *r = (double*)malloc(sz), *r0 = (double*)malloc(sz), *p = (double*)calloc(1, sz);
+    double *v = (double*)calloc(1, sz), *y = (double*)malloc(sz), *z = (double*)malloc(sz);
+    double *s = (double*)malloc(sz), *t = (double*)malloc(sz);
+    for (int64_t i = 0; i < n; i++) {
+        invd[i] = 1.0;
+        for (int64_t c = row_off[i]; c < row_off[i + 1]; c++)
+            if (cols[c] == i) { if (vals[c] != 0.0) invd[i] = 1.0 / vals[c]; break; }
+    }
+    ora_spmv(n, row_off, cols, vals, x, t);
+    for (int64_t i = 0; i < n; i++) r[i] = r0[i] = b[i] - t[i];
+    double r0_sq = dotv(n, r0, r0);
+    const double b_sq = dotv(n, b, b);
+    int64_t it = 0, rs = 0;
+    int ok = 1;
+    if (b_sq == 0.0) {
+        for (int64_t i = 0; i < n; i++) x[i] = 0.0;
+        *error = 0.0;
+    } else {
+        double rho = 1.0, alpha = 1.0, w = 1.0;
+        const double tol2 = tol * tol * b_sq, eps2 = DBL_EPSILON * DBL_EPSILON;
+        while (dotv(n, r, r) > tol2 && it < max_iters) {
+            const double rho_old = rho;
+            rho = dotv(n, r0, r);
+            if (fabs(rho) < eps2 * r0_sq) {
+                ora_spmv(n, row_off, cols, vals, x, t);
+                for (int64_t i = 0; i < n; i++) r[i] = r0[i] = b[i] - t[i];
+                rho = r0_sq = dotv(n, r, r);
+                if (rs++ == 0) it = 0;
+            }
+            const double beta = (rho / rho_old) * (alpha / w);
+            for (int64_t i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - w * v[i]);
+            for (int64_t i = 0; i < n; i++) y[i] = invd[i] * p[i];
+            ora_spmv(n, row_off, cols, vals, y, v);
+            alpha = rho / dotv(n, r0, v);
+            for (int64_t i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
+            for (int64_t i = 0; i < n; i++) z[i] = invd[i] * s[i];
+            ora_spmv(n, row_off, cols, vals, z, t);
+            const double tt = dotv(n, t, t);
+            w = tt > 0.0 ? dotv(n, t, s) / tt : 0.0;
+            for (int64_t i = 0; i < n; i++) x[i] += alpha * y[i] + w * z[i];
+            for (int64_t i = 0; i < n; i++) r[i] = s[i] - w * t[i];
+            it++;
+        }
+        *error = sqrt(dotv(n, r, r) / b_sq);
+        ok = *error <= tol;
+    }
+    *iters = it;
+    *restarts = rs;
+    free(invd); free(r); free(r0); free(p); free(v); free(y); free(z); free(s); free(t);
+    return ok;
+}

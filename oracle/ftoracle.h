@@ -149,6 +149,54 @@ void ora_surface_tension(const float* g, const double* phi, int64_t n_ifp,
                          const float* area, float st, int32_t rho, double* fx, double* fy,
                          double* fz);
 
+/* ---- The advection step of LevelSet::update (LevelSet.cpp:89-141) ---------
+ * Crank-Nicolson on the P1 Galerkin form: (M + s A) phi_n = (M - s A) phi,
+ * s = 0.5 dt, solved by BiCGSTAB with a Jacobi preconditioner from the guess
+ * phi (Eigen::BiCGSTAB, LevelSet.hpp:145).
+ *
+ * Quadrature (Quadrature.hpp:23-36, Tetrahedron.hpp:30-50): points (a,b,b),
+ * (b,a,b), (b,b,a), (b,b,b), a = (5 + 3 sqrt5)/20, b = (5 - sqrt5)/20, given
+ * to the float shape functions (phi_0 = 1 - x - y - z, phi_1..3 = x, y, z,
+ * rounded to float); weights 0.25 / 6.  phi[4 j + q]. */
+void ora_fem_quad(double* phi, double* w);
+
+/* Assembler::bilin_with_p2t (Assembler.hpp:131-226): row i sums, over the
+ * tets of i in increasing id (Mesh3D::p2t) and their local nodes j, the
+ * integral int f(u = phi_j, v = phi_i) into column t(j):
+ *   form 0 (mass, LevelSet.cpp:37):       f = u v
+ *   form 1 (advection, :107-110, data V): f = V1 (ux v) + V2 (uy v) + V3 (uz v),
+ *     V_c(q) = sum_j V[c n + t(j)] phi_j(q) accumulated from 0 in j order;
+ * each integral as ((f0 w + f2 w) + (f1 w + f3 w)) |det B|, basis gradients
+ * from invBt as in ora_tet_grad.  A tet with det B = 0 contributes nothing
+ * (the reference leaves its inverse unset).  The pattern row_off/cols is the
+ * node and its tet neighbours in increasing order (the std::map of a row).
+ * Returns the first such tet or -1. */
+int64_t ora_fem_assemble(int form, int64_t n, const float* p, const int32_t* tets,
+                         const int64_t* n2t_off, const int32_t* n2t_ids,
+                         const int64_t* row_off, const int32_t* cols, const double* vel,
+                         double* vals);
+
+/* y = A x, each row summed in column order (Eigen's row-major sparse times
+ * dense vector). */
+void ora_spmv(int64_t n, const int64_t* row_off, const int32_t* cols, const double* vals,
+              const double* x, double* y);
+
+/* Eigen's BiCGSTAB (H. A. van der Vorst, SIAM J. Sci. Stat. Comput. 13,
+ * 1992; Eigen/src/IterativeLinearSolvers/BiCGSTAB.h, Eigen >= 3.3, not
+ * vendored) with its DiagonalPreconditioner (1 / A_ii, 1 where A_ii = 0),
+ * from the guess in x: r = b - A x, r0 = r, rho = alpha = w = 1, v = p = 0;
+ * while |r|^2 > tol^2 |b|^2 and i < max_iters: rho_old = rho, rho = r0.r;
+ * if |rho| < eps^2 |r0|^2 restart (r = b - A x, r0 = r, rho = |r0|^2 = |r|^2,
+ * i = 0 on the first restart); beta = (rho/rho_old)(alpha/w);
+ * p = r + beta (p - w v); y = P p; v = A y; alpha = rho / r0.v; s = r - alpha v;
+ * z = P s; t = A z; w = t.s / t.t (0 if t.t = 0); x += alpha y + w z;
+ * r = s - w t.  |b| = 0 gives x = 0.  Dot products here run left to right
+ * (Eigen's order is its SIMD width's: parity with it is to the tolerance).
+ * error = |r| / |b|.  Returns 1 if error <= tol. */
+int ora_bicgstab(int64_t n, const int64_t* row_off, const int32_t* cols, const double* vals,
+                 const double* b, double* x, double tol, int64_t max_iters, int64_t* iters,
+                 double* error, int64_t* restarts);
+
 #ifdef __cplusplus
 }
 #endif

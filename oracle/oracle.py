@@ -43,6 +43,14 @@ _o.ora_grad_nodes.argtypes = [_i64, _P, _P, _P, _P]
 
 _o.ora_interface_curvature.restype = None
 _o.ora_interface_curvature.argtypes = [_i64, _P, _i64, _P, _P, _P, _P, _P, _i64, _P, _P, _P]
+_o.ora_fem_quad.restype = None
+_o.ora_fem_quad.argtypes = [_P, _P]
+_o.ora_fem_assemble.restype = _i64
+_o.ora_fem_assemble.argtypes = [C.c_int, _i64, _P, _P, _P, _P, _P, _P, _P, _P]
+_o.ora_spmv.restype = None
+_o.ora_spmv.argtypes = [_i64, _P, _P, _P, _P, _P]
+_o.ora_bicgstab.restype = C.c_int
+_o.ora_bicgstab.argtypes = [_i64, _P, _P, _P, _P, _P, C.c_double, _i64, _P, _P, _P]
 _o.ora_surface_tension.restype = None
 _o.ora_surface_tension.argtypes = [_P, _P, _i64, _P, _P, _P, _P, C.c_float, C.c_int32, _P, _P, _P]
 
@@ -228,3 +236,89 @@ def surface_tension(normals, phi, ifp2e, edges, k_if, area, st, rho, fx=None, fy
     _o.ora_surface_tension(_p(g), _p(phi), len(ifp2e), _p(ifp2e), _p(edges), _p(k_if), _p(area),
                            float(st), int(rho), _p(out[0]), _p(out[1]), _p(out[2]))
     return tuple(out)
+
+
+FLT_MIN = float(np.finfo(np.float32).tiny)
+
+
+def fem_quad():
+    """P1 basis values at the 4 quadrature points (float-rounded) and the weight."""
+    phi = np.empty(16, np.float64)
+    w = np.empty(1, np.float64)
+    _o.ora_fem_quad(_p(phi), _p(w))
+    return phi.reshape(4, 4), float(w[0])
+
+
+def fem_pattern(n_nodes, tets):
+    """Sparsity of Assembler::bilin_with_p2t: row i holds i and every node
+    that shares a tet with it, in increasing order (CSR: row_off, cols)."""
+    t = np.asarray(tets, np.int64)
+    r = np.repeat(t, 4, axis=1).ravel()
+    c = np.tile(t, (1, 4)).ravel()
+    key = np.unique(r * n_nodes + c)
+    rows = key // n_nodes
+    row_off = np.zeros(n_nodes + 1, np.int64)
+    row_off[1:] = np.cumsum(np.bincount(rows, minlength=n_nodes))
+    return row_off, (key % n_nodes).astype(np.int32)
+
+
+def fem_assemble(form, xyz, tets, row_off, cols, vel=None):
+    """Assembler::assemble_bilin of the mass (form 0) or advection (form 1,
+    velocity vel = [vx, vy, vz] per node, 3n) bilinear form on the pattern."""
+    xyz, tets = _c(xyz, np.float32), _c(tets, np.int32)
+    n = len(xyz)
+    off, ids = node_tets(n, tets)
+    vals = np.empty(len(cols), np.float64)
+    v = None if vel is None else _c(vel, np.float64)
+    bad = _o.ora_fem_assemble(int(form), n, _p(xyz), _p(tets), _p(off), _p(ids), _p(_c(row_off, np.int64)),
+                              _p(_c(cols, np.int32)), _p(v), _p(vals))
+    return vals, int(bad)
+
+
+def spmv(row_off, cols, vals, x):
+    x = _c(x, np.float64)
+    y = np.empty(len(row_off) - 1, np.float64)
+    _o.ora_spmv(len(y), _p(_c(row_off, np.int64)), _p(_c(cols, np.int32)), _p(_c(vals, np.float64)),
+                _p(x), _p(y))
+    return y
+
+
+def bicgstab(row_off, cols, vals, b, x0, tol, max_iters):
+    """Eigen's BiCGSTAB with the Jacobi preconditioner from guess x0:
+    (x, converged, iterations, error, restarts)."""
+    x = np.array(x0, np.float64, copy=True)
+    it, err, rs = np.zeros(1, np.int64), np.zeros(1, np.float64), np.zeros(1, np.int64)
+    ok = _o.ora_bicgstab(len(x), _p(_c(row_off, np.int64)), _p(_c(cols, np.int32)),
+                         _p(_c(vals, np.float64)), _p(_c(b, np.float64)), _p(x), float(tol),
+                         int(max_iters), _p(it), _p(err), _p(rs))
+    return x, bool(ok), int(it[0]), float(err[0]), int(rs[0])
+
+
+def advect_system(xyz, tets, phi, flow, dt):
+    """The Crank-Nicolson system of LevelSet::update (LevelSet.cpp:99-129):
+    (row_off, cols, L = M + s A, b = (M - s A) phi), s = 0.5 dt."""
+    n = len(xyz)
+    row_off, cols = fem_pattern(n, tets)
+    M, _ = fem_assemble(0, xyz, tets, row_off, cols)
+    A, _ = fem_assemble(1, xyz, tets, row_off, cols, np.asarray(flow, np.float64)[:3 * n])
+    s = 0.5 * dt
+    L = M + s * A
+    R = M - s * A
+    return row_off, cols, L, spmv(row_off, cols, R, phi)
+
+
+def advect(xyz, tets, phi, flow, dt, tol=0.0, max_iters=0):
+    """The advection of LevelSet::update (LevelSet.cpp:89-141): phi_n and
+    dict(iterations, error, restarts, converged, skipped)."""
+    n = len(xyz)
+    flow = np.asarray(flow, np.float64)
+    if dt < FLT_MIN or np.sqrt(np.sum(flow * flow)) < FLT_MIN:
+        return np.array(phi, np.float64, copy=True), dict(skipped=True, converged=True, iterations=0,
+                                                         error=0.0, restarts=0)
+    if np.isnan(flow[:3 * n]).any():
+        raise ValueError("velocity has NaN")
+    row_off, cols, L, b = advect_system(xyz, tets, phi, flow, dt)
+    tol = tol if tol > 0 else float(np.finfo(np.float64).eps)
+    max_iters = max_iters if max_iters > 0 else 2 * n
+    x, ok, it, err, rs = bicgstab(row_off, cols, L, b, phi, tol, max_iters)
+    return x, dict(skipped=False, converged=ok, iterations=it, error=err, restarts=rs)

@@ -1,0 +1,111 @@
+"""GPU: the advection step of LevelSet::update (LevelSet.cpp:89-141) on the
+device.  The assembled matrices are the oracle's bit for bit; the solve agrees
+with the oracle's BiCGSTAB to the solver tolerance (dot products are reduced
+in a different order), and with closed forms."""
+import numpy as np
+import pytest
+
+import fluidtissue_amd as fa
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _flow(m, kind, n=None):
+    x = m.xyz.astype(np.float64)
+    n = len(x)
+    if kind == "rotation":
+        v = np.concatenate([-(x[:, 1] - 0.5), x[:, 0] - 0.5, 0.3 * np.sin(3 * x[:, 0])])
+    elif kind == "random":
+        v = np.random.default_rng(4).standard_normal(3 * n)
+    else:
+        v = np.concatenate([np.full(n, 0.8), np.zeros(n), np.zeros(n)])
+    return np.concatenate([v, np.linspace(0, 1, n)])
+
+
+@pytest.mark.parametrize("n,jitter", [(12, 0.0), (17, 0.2), ((9, 14, 11), 0.1)])
+def test_fem_matrices_match_oracle(n, jitter):
+    m = fa.kuhn_mesh(n, jitter=jitter, seed=6)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    off, cols, M = r.fem_matrix(0)
+    flow = _flow(m, "random")
+    off1, cols1, A = r.fem_matrix(1, flow[:3 * len(m.xyz)])
+    r.close()
+    ro, co = oracle.fem_pattern(len(m.xyz), m.tets)
+    assert np.array_equal(off, ro) and np.array_equal(cols, co)
+    assert np.array_equal(off1, ro) and np.array_equal(cols1, co)
+    Mr, _ = oracle.fem_assemble(0, m.xyz, m.tets, ro, co)
+    Ar, _ = oracle.fem_assemble(1, m.xyz, m.tets, ro, co, flow[:3 * len(m.xyz)])
+    assert np.array_equal(M.view(np.uint64), Mr.view(np.uint64))
+    assert np.array_equal(A.view(np.uint64), Ar.view(np.uint64))
+
+
+@pytest.mark.parametrize("kind", ["rotation", "random"])
+def test_advect_matches_oracle_to_tolerance(kind):
+    """Both solves stop at |b - L x| <= tol |b|; their difference is bounded
+    by the conditioning of L (mass dominated for this dt), here 1e-8 of |phi|."""
+    m = fa.kuhn_mesh(20, jitter=0.2, seed=2)
+    phi = fa.phi_field("spheres", m.xyz, seed=1)
+    flow = _flow(m, kind)
+    dt, tol = 0.01, 1e-11
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    got, info = r.advect(phi, flow, dt, tol=tol, max_iters=500)
+    r.close()
+    ref, rinfo = oracle.advect(m.xyz, m.tets, phi, flow, dt, tol=tol, max_iters=500)
+    assert info["converged"] and rinfo["converged"] and not info["skipped"]
+    assert info["error"] <= tol and abs(info["iterations"] - rinfo["iterations"]) <= 2
+    assert np.max(np.abs(got - ref)) <= 1e-8 * np.max(np.abs(phi))
+    ro, co, L, b = oracle.advect_system(m.xyz, m.tets, phi, flow, dt)
+    res = b - oracle.spmv(ro, co, L, got)
+    assert np.linalg.norm(res) <= 1.01 * tol * np.linalg.norm(b)    # the device's own stop rule, in FP64
+
+
+def test_advect_translates_a_plane_exactly():
+    m = fa.kuhn_mesh(24, jitter=0.15, seed=9)
+    phi = m.xyz[:, 0].astype(np.float64) - 0.41
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    got, info = r.advect(phi, _flow(m, "const"), 0.02, tol=1e-13, max_iters=500)
+    r.close()
+    assert info["converged"]
+    assert np.max(np.abs(got - (phi - 0.8 * 0.02))) < 1e-10
+
+
+def test_advect_skip_nan_and_no_convergence():
+    m = fa.kuhn_mesh(10)
+    n = len(m.xyz)
+    phi = m.xyz[:, 0].astype(np.float64) - 0.5
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    got, info = r.advect(phi, np.zeros(4 * n), 0.1)
+    assert info["skipped"] and np.array_equal(got, phi)
+    got, info = r.advect(phi, _flow(m, "rotation"), 0.0)
+    assert info["skipped"] and np.array_equal(got, phi)
+    bad = _flow(m, "rotation")
+    bad[7] = np.nan
+    with pytest.raises(fa._lib.FtError) as e:
+        r.advect(phi, bad, 0.1)
+    assert e.value.code == fa._lib.FT_ERR_ARG
+    with pytest.raises(fa._lib.FtError) as e:            # one iteration cannot reach 1e-14
+        r.advect(phi, _flow(m, "random"), 0.5, tol=1e-14, max_iters=1)
+    assert e.value.code == fa._lib.FT_ERR_SOLVER
+    r.close()
+
+
+def test_update_is_advect_then_reinit():
+    """LevelSet::update: advection, then the reinit with the update blend,
+    against the oracle's advection and reinit (crossing points move with the
+    solve's rounding, so to a tolerance)."""
+    m = fa.kuhn_mesh(24, jitter=0.1, seed=3)
+    phi = fa.phi_field("ellipsoids", m.xyz, seed=2)
+    flow = _flow(m, "rotation")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, cnt, info = r.update(phi, flow, 0.01, tol=1e-12, mode="points")
+    r.close()
+    adv, _ = oracle.advect(m.xyz, m.tets, phi, flow, 0.01, tol=1e-12, max_iters=1000)
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, adv, blend=1)
+    assert cnt.n_ifp == len(ref["ifp"])
+    assert np.max(np.abs(out - ref["phi"])) < 1e-6
