@@ -72,6 +72,89 @@ def grid_xyz(n):
     return xyz.reshape(-1, 3)
 
 
+def advect_flow(xyz):
+    """Velocity of the advection workload: a rotation about the vertical axis
+    through the domain centre plus a vertical shear, |V| <= 0.9; pressure 0."""
+    x = xyz.astype(np.float64)
+    n = len(x)
+    v = np.concatenate([-(x[:, 1] - 0.5), x[:, 0] - 0.5, 0.2 * np.sin(2 * np.pi * x[:, 0])])
+    return np.concatenate([v, np.zeros(n)])
+
+
+def run_advect(args):
+    """One step = the advection of LevelSet::update (LevelSet.cpp:89-141) on
+    the 256^3 mesh: assembly of the advection matrix, the Crank-Nicolson
+    system and BiCGSTAB to |r| <= 1e-10 |b|, phi restored from a device copy
+    before each step (timed)."""
+    import torch
+    import fluidtissue_amd as fa
+    from fluidtissue_amd import _lib
+    import ctypes as C
+    n = args.advect_n
+    m = fa.kuhn_mesh(n, jitter=0.2, seed=1)
+    phi = fa.phi_field("spheres", m.xyz, seed=0)
+    flow = advect_flow(m.xyz)
+    h = 1.0 / (n - 1)
+    dt = 0.5 * h / 0.9                           # CFL 0.5
+    tol, max_iters = 1e-10, 500
+    dev = torch.device("cuda:0")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    d_phi0 = torch.from_numpy(phi).to(dev)
+    d_phi = torch.empty_like(d_phi0)
+    d_flow = torch.from_numpy(flow).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    info = _lib.FtAdvectInfo()
+
+    def step():
+        d_phi.copy_(d_phi0)
+        _lib.check(_lib.lib.ft_advect_device(r._h, C.c_void_p(d_phi.data_ptr()), C.c_void_p(d_flow.data_ptr()),
+                                             dt, tol, max_iters, C.byref(info), C.c_void_p(stream)))
+        return info.iterations
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    N = m.n_nodes
+    row_off = np.empty(N + 1, np.int64)
+    _lib.check(_lib.lib.ft_fem_assemble(r._h, 0, None, row_off.ctypes.data_as(C.c_void_p), None, None))
+    nnz = int(row_off[-1])
+    log(f"advect {n}^3: {ms:.3f} ms per step, {np.mean(its):.1f} BiCGSTAB iterations, "
+        f"error {info.error:.3g}, nnz {nnz}")
+    # per iteration: two SpMVs (8 B value + 4 B column per entry) and the
+    # vector passes (p 32, v 32, t 32, x/r 72 bytes per row)
+    it_bytes = 24 * nnz + 168 * N
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        nc = 64
+        mc = fa.kuhn_mesh(nc, jitter=0.2, seed=1)
+        pc = fa.phi_field("spheres", mc.xyz, seed=0)
+        t1 = time.perf_counter()
+        _, ci = oracle.advect(mc.xyz, mc.tets, pc, advect_flow(mc.xyz), 0.5 / (nc - 1) / 0.9, tol, max_iters)
+        tc = time.perf_counter() - t1
+        cpu = dict(value=mc.n_nodes / tc / 1e6, unit="Mnodes/s", cores=1, kind="port",
+                   sample=f"oracle advection of the {nc}^3 mesh (same field, velocity and CFL): "
+                          f"{tc:.2f} s, {ci['iterations']} iterations, 1 thread")
+    out = {
+        "metric": f"Mnodes/s advection step (Crank-Nicolson + BiCGSTAB to 1e-10) at {n}^3",
+        "value": N / (ms * 1e-3) / 1e6, "unit": "Mnodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"advect: {n}^3 Kuhn (jittered 0.2h), 64 spheres phi, rotation + shear "
+                               f"velocity, CFL 0.5", "nodes": N, "nnz": nnz, "dt": dt,
+                   "iterations": float(np.mean(its)), "error": info.error},
+        "iteration_bytes": it_bytes,
+        "cpu_baseline": cpu,
+    }
+    r.close()
+    return out
+
+
 def run_batched(args):
     """512 fields, N ~ U{24..96}, each a union of 1-16 ellipsoids: one batched
     reinit (autotuned bin size and workgroup size) against 512 serial plain
@@ -355,7 +438,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS) + ["batched", "serving"])
+    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS) + ["batched", "serving", "advect"])
+    ap.add_argument("--advect-n", type=int, default=256)
     ap.add_argument("--fields", type=int, default=512)
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 500.0])
@@ -378,6 +462,9 @@ def main():
         return
     if args.workload == "serving":
         print(json.dumps(run_serving(args)), flush=True)
+        return
+    if args.workload == "advect":
+        print(json.dumps(run_advect(args)), flush=True)
         return
     wl = WORKLOADS[args.workload]
     t0 = time.perf_counter()

@@ -53,7 +53,6 @@ namespace {
 constexpr int kBS = 256;                 // threads per block of the streaming kernels
 constexpr int kScanTile = 2048;          // elements per scan block (256 x 8)
 constexpr int kStack = 24;               // traversal stack depth (tree depth <= 24)
-constexpr int kQueryBS = 256;
 constexpr int kMaxBinsLog2PerAxis = 8;   // <= 256 bins per axis, 2^24 Morton cells
 
 #define FT_HIP(call)                                                                   \
@@ -1628,102 +1627,160 @@ __device__ int fem_row_cols(const uint64_t* n2t_off, const int32_t* n2t_ids, con
 // Pass 1 of the pattern: row lengths and slice widths.  One wave per slice.
 __global__ __launch_bounds__(kFemSlice) void k_fem_rowlen(const uint64_t* n2t_off, const int32_t* n2t_ids,
                                                           const int4* tets, int64_t N, int32_t* len,
-                                                          uint32_t* width, unsigned* overflow)
+                                                          uint32_t* width, unsigned* flags)
 {
     __shared__ int32_t lst[kFemSlice][kFemMaxCols + 1];
     const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
     int cnt = 0;
     if (i < N) {
         cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, lst[threadIdx.x]);
-        if (cnt < 0) { atomicOr(overflow, 1u); cnt = 0; }
+        if (cnt < 0) { atomicOr(&flags[0], 1u); cnt = 0; }
         len[i] = cnt;
+        atomicMax(&flags[2], (unsigned)(n2t_off[i + 1] - n2t_off[i]));   // most tets at a node
     }
     int wmax = cnt;
     for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
-    if (threadIdx.x == 0) width[blockIdx.x] = (uint32_t)wmax;
+    if (threadIdx.x == 0) {
+        width[blockIdx.x] = (uint32_t)wmax;
+        atomicMax(&flags[1], (unsigned)wmax);                // longest row of the mesh
+    }
 }
 
-// Pass 2: the ELL columns; padding entries name the row itself.
+// Pass 2: the ELL columns (padding entries name the row itself) and the
+// scatter map: for the a-th tet of row i (n2t order), the row positions of
+// its four nodes, one byte each, in fmap[n2t_off[i] + a].
 __global__ __launch_bounds__(kFemSlice) void k_fem_rowfill(const uint64_t* n2t_off, const int32_t* n2t_ids,
                                                            const int4* tets, int64_t N, FemEll e,
-                                                           const uint32_t* width, int32_t* cols)
+                                                           const uint32_t* width, int32_t* cols,
+                                                           uint32_t* fmap)
 {
     __shared__ int32_t lst[kFemSlice][kFemMaxCols + 1];
     const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
     if (i >= N) return;
-    const int cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, lst[threadIdx.x]);
+    int32_t* l = lst[threadIdx.x];
+    const int cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, l);
     const int W = (int)width[blockIdx.x];
-    for (int k = 0; k < W; k++) cols[ell_at(e, i, k)] = (k < cnt) ? lst[threadIdx.x][k] : (int32_t)i;
-}
-
-// Row i of the mass (FORM 0) or advection (FORM 1) form of
-// Assembler::bilin_with_p2t (Assembler.hpp:131-226) into acc[0..len): over
-// the tets of i in increasing id and their local nodes j, the integral
-// ((f0 w + f2 w) + (f1 w + f3 w)) |det B| of f(u = phi_j, v = phi_i) added to
-// column t(j):  mass  f = u v;  advection  f = V1 (ux v) + V2 (uy v) + V3 (uz v)
-// with V_c(q) = sum_j V_c[t(j)] phi_j(q) accumulated from 0 in j order.
-template <int FORM>
-__device__ void fem_row(const uint64_t* n2t_off, const int32_t* n2t_ids, const int4* tets,
-                        const float* xyz, const double* vel, int64_t N, const FemQuad& qd,
-                        const FemEll& e, int64_t i, int len, double* acc)
-{
-    for (int k = 0; k < len; k++) acc[k] = 0.0;
+    for (int k = 0; k < W; k++) cols[ell_at(e, i, k)] = (k < cnt) ? l[k] : (int32_t)i;
     for (uint64_t a = n2t_off[i]; a < n2t_off[i + 1]; a++) {
         const int4 q = tets[n2t_ids[a]];
-        const int qi[4] = {q.x, q.y, q.z, q.w};
-        double G[4][3], detB;
-        if (!fem_frame(xyz, qi, G, detB)) continue;
-        const int li = qi[0] == (int)i ? 0 : qi[1] == (int)i ? 1 : qi[2] == (int)i ? 2 : 3;
-        double V[3][4];
-        if (FORM == 1) {
-            for (int c = 0; c < 3; c++)
-                for (int q4 = 0; q4 < 4; q4++) {
-                    double acc_v = 0.0;
-                    for (int j = 0; j < 4; j++) acc_v = acc_v + vel[c * N + qi[j]] * qd.phi[4 * j + q4];
-                    V[c][q4] = acc_v;
-                }
-        }
+        const int v4[4] = {q.x, q.y, q.z, q.w};
+        uint32_t m = 0;
         for (int j = 0; j < 4; j++) {
-            double f[4];
-            for (int q4 = 0; q4 < 4; q4++) {
-                const double v = qd.phi[4 * li + q4];
-                if (FORM == 0)
-                    f[q4] = qd.phi[4 * j + q4] * v;
-                else
-                    f[q4] = (V[0][q4] * (G[j][0] * v) + V[1][q4] * (G[j][1] * v)) + V[2][q4] * (G[j][2] * v);
+            int k = 0;
+            while (k < cnt - 1 && l[k] != v4[j]) k++;
+            m |= (uint32_t)k << (8 * j);
+        }
+        fmap[a] = m;
+    }
+}
+
+// Rows of the mass (FORM 0) or advection (FORM 1) form of
+// Assembler::bilin_with_p2t (Assembler.hpp:131-226): over the tets of row i
+// in increasing id and their local nodes j, the integral
+// ((f0 w + f2 w) + (f1 w + f3 w)) |det B| of f(u = phi_j, v = phi_i) is
+// added to column t(j):  mass  f = u v;  advection  f = V1 (ux v) + V2 (uy v)
+// + V3 (uz v) with V_c(q) = sum_j V_c[t(j)] phi_j(q) accumulated from 0 in
+// j order.
+//
+// kFemLanes lanes share a row: they compute the contributions of disjoint
+// tets into LDS (qb, four per tet), then the row's first lane adds them into
+// the row accumulator in tet order, so every entry is summed exactly as the
+// reference's std::map does.  A workgroup is one wave of kFemRowsPerWave rows.
+constexpr int kFemLanes = 4;
+constexpr int kFemRowsPerWave = 64 / kFemLanes;
+
+// LDS: kFemRowsPerWave accumulators of `stride` doubles, then as many
+// buffers of 4 tmax doubles.
+__host__ __device__ inline size_t fem_lds_bytes(int stride, int tmax)
+{
+    return sizeof(double) * (size_t)kFemRowsPerWave * ((size_t)stride + 4 * (size_t)tmax);
+}
+
+template <int FORM>
+__device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const uint32_t* fmap,
+                         const int4* tets, const float* xyz, const double* vel, int64_t N, const FemQuad& qd,
+                         const FemEll& e, int stride, int tmax, double* lds, int64_t& row, double*& acc,
+                         int& len)
+{
+    const int lane = threadIdx.x % kFemLanes, rl = threadIdx.x / kFemLanes;
+    const int64_t i = (int64_t)blockIdx.x * kFemRowsPerWave + rl;
+    double* qb = lds + (size_t)kFemRowsPerWave * stride + (size_t)rl * 4 * tmax;
+    acc = lds + (size_t)rl * stride;
+    row = i;
+    len = 0;
+    uint64_t a0 = 0, na = 0;
+    if (i < N) {
+        a0 = n2t_off[i];
+        na = n2t_off[i + 1] - a0;
+        for (uint64_t a = lane; a < na; a += kFemLanes) {
+            const int4 q = tets[n2t_ids[a0 + a]];
+            const int qi[4] = {q.x, q.y, q.z, q.w};
+            double G[4][3], detB;
+            if (!fem_frame(xyz, qi, G, detB)) {
+                for (int j = 0; j < 4; j++) qb[4 * a + j] = 0.0;   // no contribution
+                continue;
             }
-            const double w = qd.w;
-            const double qint = ((f[0] * w + f[2] * w) + (f[1] * w + f[3] * w)) * detB;
-            for (int k = 0; k < len; k++)
-                if (e.cols[ell_at(e, i, k)] == qi[j]) { acc[k] += qint; break; }
+            const int li = qi[0] == (int)i ? 0 : qi[1] == (int)i ? 1 : qi[2] == (int)i ? 2 : 3;
+            double V[3][4];
+            if (FORM == 1) {
+                for (int c = 0; c < 3; c++)
+                    for (int q4 = 0; q4 < 4; q4++) {
+                        double acc_v = 0.0;
+                        for (int j = 0; j < 4; j++) acc_v = acc_v + vel[c * N + qi[j]] * qd.phi[4 * j + q4];
+                        V[c][q4] = acc_v;
+                    }
+            }
+            for (int j = 0; j < 4; j++) {
+                double f[4];
+                for (int q4 = 0; q4 < 4; q4++) {
+                    const double v = qd.phi[4 * li + q4];
+                    if (FORM == 0)
+                        f[q4] = qd.phi[4 * j + q4] * v;
+                    else
+                        f[q4] = (V[0][q4] * (G[j][0] * v) + V[1][q4] * (G[j][1] * v)) + V[2][q4] * (G[j][2] * v);
+                }
+                const double w = qd.w;
+                qb[4 * a + j] = ((f[0] * w + f[2] * w) + (f[1] * w + f[3] * w)) * detB;
+            }
+        }
+    }
+    __syncthreads();
+    if (i < N && lane == 0) {
+        len = e.len[i];
+        for (int k = 0; k < len; k++) acc[k] = 0.0;
+        for (uint64_t a = 0; a < na; a++) {
+            const uint32_t pos = fmap[a0 + a];
+            for (int j = 0; j < 4; j++) acc[(pos >> (8 * j)) & 0xffu] += qb[4 * a + j];
         }
     }
 }
 
 // The mass matrix, once per mesh (LevelSet::init, LevelSet.cpp:35-39).
-__global__ __launch_bounds__(kFemSlice) void k_fem_mass(const uint64_t* n2t_off, const int32_t* n2t_ids,
-                                                        const int4* tets, const float* xyz, int64_t N,
-                                                        FemQuad qd, FemEll e, double* mvals)
+__global__ __launch_bounds__(64) void k_fem_mass(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                 const uint32_t* fmap, const int4* tets, const float* xyz,
+                                                 int64_t N, FemQuad qd, FemEll e, int stride, int tmax,
+                                                 double* mvals)
 {
-    __shared__ double acc[kFemSlice][kFemMaxCols + 1];
-    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
-    if (i >= N) return;
-    const int len = e.len[i];
-    fem_row<0>(n2t_off, n2t_ids, tets, xyz, nullptr, N, qd, e, i, len, acc[threadIdx.x]);
-    for (int k = 0; k < len; k++) mvals[ell_at(e, i, k)] = acc[threadIdx.x][k];
+    extern __shared__ double fem_lds[];
+    int64_t i;
+    double* acc;
+    int len;
+    fem_rows<0>(n2t_off, n2t_ids, fmap, tets, xyz, nullptr, N, qd, e, stride, tmax, fem_lds, i, acc, len);
+    for (int k = 0; k < len; k++) mvals[ell_at(e, i, k)] = acc[k];
 }
 
 // The advection matrix alone (ft_fem_assemble), into vals in ELL layout.
-__global__ __launch_bounds__(kFemSlice) void k_fem_adv(const uint64_t* n2t_off, const int32_t* n2t_ids,
-                                                       const int4* tets, const float* xyz, const double* vel,
-                                                       int64_t N, FemQuad qd, FemEll e, double* vals)
+__global__ __launch_bounds__(64) void k_fem_adv(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                                const uint32_t* fmap, const int4* tets, const float* xyz,
+                                                const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
+                                                int tmax, double* vals)
 {
-    __shared__ double acc[kFemSlice][kFemMaxCols + 1];
-    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
-    if (i >= N) return;
-    const int len = e.len[i];
-    fem_row<1>(n2t_off, n2t_ids, tets, xyz, vel, N, qd, e, i, len, acc[threadIdx.x]);
-    for (int k = 0; k < len; k++) vals[ell_at(e, i, k)] = acc[threadIdx.x][k];
+    extern __shared__ double fem_lds[];
+    int64_t i;
+    double* acc;
+    int len;
+    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, tmax, fem_lds, i, acc, len);
+    for (int k = 0; k < len; k++) vals[ell_at(e, i, k)] = acc[k];
 }
 
 // ELL rows to CSR at the row offsets off (a scan of the lengths).
@@ -1763,21 +1820,20 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* part)
 // (LevelSet.cpp:126-127), invdiag = 1 / L_ii (1 where 0: Eigen's
 // DiagonalPreconditioner), b = R phi and r = b - L phi (BiCGSTAB's first
 // residual from the guess phi), each summed in column order; r0 = r, x = phi,
-// p = v = 0.  Block partials of r.r and b.b.
-__global__ __launch_bounds__(kFemSlice) void k_fem_cn(const uint64_t* n2t_off, const int32_t* n2t_ids,
-                                                      const int4* tets, const float* xyz, const double* vel,
-                                                      int64_t N, FemQuad qd, FemEll e, const double* mvals,
-                                                      double s, const double* phi, double* lvals,
-                                                      double* invdiag, double* b, double* x, double* r,
-                                                      double* r0, double* p, double* v, double* part)
+// p = v = 0.
+__global__ __launch_bounds__(64) void k_fem_cn(const uint64_t* n2t_off, const int32_t* n2t_ids,
+                                               const uint32_t* fmap, const int4* tets, const float* xyz,
+                                               const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
+                                               int tmax, const double* mvals, double s, const double* phi,
+                                               double* lvals, double* invdiag, double* b, double* x, double* r,
+                                               double* r0, double* p, double* v)
 {
-    __shared__ double acc[kFemSlice][kFemMaxCols + 1];
-    const int64_t i = (int64_t)blockIdx.x * kFemSlice + threadIdx.x;
-    double rr = 0.0, bb = 0.0;
-    if (i < N) {
-        const int len = e.len[i];
-        double* A = acc[threadIdx.x];
-        fem_row<1>(n2t_off, n2t_ids, tets, xyz, vel, N, qd, e, i, len, A);
+    extern __shared__ double fem_lds[];
+    int64_t i;
+    double* A;
+    int len;
+    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, tmax, fem_lds, i, A, len);
+    if (i < N && threadIdx.x % kFemLanes == 0) {
         double bi = 0.0, lx = 0.0, id = 1.0;
         for (int k = 0; k < len; k++) {
             const int64_t at = ell_at(e, i, k);
@@ -1797,10 +1853,7 @@ __global__ __launch_bounds__(kFemSlice) void k_fem_cn(const uint64_t* n2t_off, c
         r0[i] = ri;
         p[i] = 0.0;
         v[i] = 0.0;
-        rr = ri * ri;
-        bb = bi * bi;
     }
-    block_sum2<kFemSlice>(rr, bb, part);
 }
 
 // ---- BiCGSTAB (Eigen's bicgstab; H. A. van der Vorst 1992) ----------------
@@ -1813,15 +1866,20 @@ struct BcgScalars {
 };
 
 constexpr int kBcgBS = 256;
+constexpr int kBcgGrid = 2048;         // workgroups of the solver kernels (block partials)
+
+// Grid-stride loops: row i is handled by workgroup (i / kBcgBS) % gridDim.x;
+// each thread sums its rows in order, so the partials are fixed by N and the
+// grid size alone.
+#define BCG_ROWS(i) for (int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x; i < N; \
+                         i += (int64_t)gridDim.x * kBcgBS)
 
 // p = r + beta (p - w v), beta = (rho / rho_old) (alpha / w).
 __global__ __launch_bounds__(kBcgBS) void k_bcg_p(int64_t N, const BcgScalars* sc, const double* r,
                                                   const double* v, double* p)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
-    if (i >= N) return;
-    const double beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->w);
-    p[i] = r[i] + beta * (p[i] - sc->w * v[i]);
+    const double beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->w), w = sc->w;
+    BCG_ROWS(i) p[i] = r[i] + beta * (p[i] - w * v[i]);
 }
 
 // v = L y with y = invdiag p (y = P p, then v = A y); partial r0.v.
@@ -1829,9 +1887,8 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_v(int64_t N, FemEll e, const dou
                                                   const double* invdiag, const double* p, const double* r0,
                                                   double* v, double* part)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
     double d = 0.0;
-    if (i < N) {
+    BCG_ROWS(i) {
         const int len = e.len[i];
         double t = 0.0;
         for (int k = 0; k < len; k++) {
@@ -1840,7 +1897,7 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_v(int64_t N, FemEll e, const dou
             t += lvals[at] * (invdiag[c] * p[c]);
         }
         v[i] = t;
-        d = r0[i] * t;
+        d += r0[i] * t;
     }
     block_sum2<kBcgBS>(d, 0.0, part);
 }
@@ -1851,10 +1908,9 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_t(int64_t N, FemEll e, const dou
                                                   const double* invdiag, const double* r, const double* v,
                                                   const BcgScalars* sc, double* t, double* part)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    const double alpha = sc->alpha;
     double tt = 0.0, ts = 0.0;
-    if (i < N) {
-        const double alpha = sc->alpha;
+    BCG_ROWS(i) {
         const int len = e.len[i];
         double acc = 0.0;
         for (int k = 0; k < len; k++) {
@@ -1863,8 +1919,8 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_t(int64_t N, FemEll e, const dou
             acc += lvals[at] * (invdiag[c] * (r[c] - alpha * v[c]));
         }
         t[i] = acc;
-        tt = acc * acc;
-        ts = acc * (r[i] - alpha * v[i]);
+        tt += acc * acc;
+        ts += acc * (r[i] - alpha * v[i]);
     }
     block_sum2<kBcgBS>(tt, ts, part);
 }
@@ -1874,18 +1930,17 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_x(int64_t N, const double* invdi
                                                   const double* v, const double* t, const double* r0,
                                                   const BcgScalars* sc, double* x, double* r, double* part)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
+    const double alpha = sc->alpha, w = sc->w;
     double rr = 0.0, r0r = 0.0;
-    if (i < N) {
-        const double alpha = sc->alpha, w = sc->w;
+    BCG_ROWS(i) {
         const double y = invdiag[i] * p[i];
         const double si = r[i] - alpha * v[i];
         const double z = invdiag[i] * si;
         x[i] = x[i] + (alpha * y + w * z);
         const double ri = si - w * t[i];
         r[i] = ri;
-        rr = ri * ri;
-        r0r = r0[i] * ri;
+        rr += ri * ri;
+        r0r += r0[i] * ri;
     }
     block_sum2<kBcgBS>(rr, r0r, part);
 }
@@ -1895,9 +1950,8 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_restart(int64_t N, FemEll e, con
                                                         const double* b, const double* x, double* r,
                                                         double* r0, double* part)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
     double rr = 0.0;
-    if (i < N) {
+    BCG_ROWS(i) {
         const int len = e.len[i];
         double t = 0.0;
         for (int k = 0; k < len; k++) {
@@ -1907,9 +1961,21 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_restart(int64_t N, FemEll e, con
         const double ri = b[i] - t;
         r[i] = ri;
         r0[i] = ri;
-        rr = ri * ri;
+        rr += ri * ri;
     }
     block_sum2<kBcgBS>(rr, 0.0, part);
+}
+
+// Partials of a.b and c.d.
+__global__ __launch_bounds__(kBcgBS) void k_dots2(int64_t N, const double* a, const double* b,
+                                                  const double* c, const double* d, double* part)
+{
+    double ab = 0.0, cd = 0.0;
+    BCG_ROWS(i) {
+        ab += a[i] * b[i];
+        cd += c[i] * d[i];
+    }
+    block_sum2<kBcgBS>(ab, cd, part);
 }
 
 enum BcgStep : int { BCG_INIT = 0, BCG_ALPHA = 1, BCG_W = 2, BCG_END = 3, BCG_RESTART = 4, BCG_SUMS = 5 };
@@ -1959,9 +2025,8 @@ __global__ __launch_bounds__(256) void k_bcg_fin(const double* part, int64_t nb,
 // |flow|^2 over 4 N entries and whether the velocity (first 3 N) has a NaN.
 __global__ __launch_bounds__(kBcgBS) void k_flow_check(const double* flow, int64_t N, double* part)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x;
     double sq = 0.0, nan = 0.0;
-    if (i < N) {
+    BCG_ROWS(i) {
         for (int c = 0; c < 4; c++) {
             const double f = flow[c * N + i];
             sq += f * f;
@@ -2200,10 +2265,12 @@ struct ft_ctx {
     DevBuf st_f;                     // surface tension: device copy of host force arrays
     // advection: sliced-ELL pattern and mass matrix (per layout), the
     // Crank-Nicolson matrix, BiCGSTAB vectors, block partials and scalars
-    DevBuf fem_len, fem_w, fem_soff, fem_cols, fem_m, fem_l, fem_flag;
+    DevBuf fem_len, fem_w, fem_soff, fem_cols, fem_m, fem_l, fem_flag, fem_map;
     DevBuf bcg_vec, bcg_part, bcg_sc, adv_phi, adv_flow, fem_csr_off;
     uint64_t fem_layout = ~0ull;
     int64_t fem_nnz = 0;             // ELL entries with padding
+    int fem_stride = 1;              // LDS doubles per row accumulator (longest row, odd)
+    int fem_tmax = 0;                // most tets at a node
     void* h_bcg = nullptr;           // pinned mirror of the solver scalars
     const double* last_phi = nullptr;   // device phi of the last reinit (blended)
     uint64_t n2t_layout = ~0ull;
@@ -2253,7 +2320,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
             &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
             &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f,
-            &h->fem_len, &h->fem_w, &h->fem_soff, &h->fem_cols, &h->fem_m, &h->fem_l, &h->fem_flag,
+            &h->fem_len, &h->fem_w, &h->fem_soff, &h->fem_cols, &h->fem_m, &h->fem_l, &h->fem_flag, &h->fem_map,
             &h->bcg_vec, &h->bcg_part, &h->bcg_sc, &h->adv_phi, &h->adv_flow, &h->fem_csr_off};
 }
 
@@ -2354,11 +2421,6 @@ int read_scalars(ft_ctx* c, hipStream_t s, int count)
     FT_HIP(hipMemcpyAsync(c->h_scalars, c->scalars.p, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
     return FT_OK;
-}
-
-void mark(ft_ctx* c, int k, hipStream_t s)
-{
-    if (c->timing) (void)hipEventRecord(c->ev[k], s);
 }
 
 int check_handle(ft_handle h)
@@ -3739,6 +3801,9 @@ FemQuad fem_quad()
     return q;
 }
 
+size_t fem_lds(const ft_ctx* h) { return fem_lds_bytes(h->fem_stride, h->fem_tmax); }
+unsigned fem_grid(int64_t N) { return (unsigned)((N + kFemRowsPerWave - 1) / kFemRowsPerWave); }
+
 FemEll fem_ell(ft_ctx* h)
 {
     return FemEll{h->fem_len.as<int32_t>(), h->fem_soff.as<uint64_t>(), h->fem_cols.as<int32_t>()};
@@ -3758,7 +3823,7 @@ int fem_setup(ft_ctx* h, hipStream_t s)
     FT_TRY(ensure(h->fem_flag, 16));
     uint32_t* w = h->fem_w.as<uint32_t>();
     uint64_t* soff = h->fem_soff.as<uint64_t>();
-    FT_TRY(fill<unsigned>(s, h->fem_flag.as<unsigned>(), 1, 0u));
+    FT_TRY(fill<unsigned>(s, h->fem_flag.as<unsigned>(), 3, 0u));   // overflow, longest row, most tets
     FT_TRY(fill<uint64_t>(s, soff + ns, 1, 0ull));
     if (ns > 0) {
         hipLaunchKernelGGL(k_fem_rowlen, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
@@ -3767,23 +3832,30 @@ int fem_setup(ft_ctx* h, hipStream_t s)
         FT_TRY(scan<uint32_t, uint64_t>(h, s, w, soff, ns, soff + ns, LoadSame{}));
     }
     uint64_t total = 0;
-    unsigned flag = 0;
+    unsigned flag[3] = {0, 0, 0};
     FT_HIP(hipMemcpyAsync(&total, soff + ns, 8, hipMemcpyDeviceToHost, s));
-    FT_HIP(hipMemcpyAsync(&flag, h->fem_flag.p, 4, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipMemcpyAsync(flag, h->fem_flag.p, 12, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
-    if (flag)
+    if (flag[0])
         return ft::set_error(FT_ERR_ARG, "a node shares tets with more than %d nodes", kFemMaxCols - 1);
     const int64_t nnz = (int64_t)total * kFemSlice;
     FT_TRY(ensure(h->fem_cols, 4 * at_least1(nnz)));
     FT_TRY(ensure(h->fem_m, 8 * at_least1(nnz)));
     FT_TRY(ensure(h->fem_l, 8 * at_least1(nnz)));
+    FT_TRY(ensure(h->fem_map, 4 * at_least1(4 * h->T)));             // one word per n2t entry
+    h->fem_stride = (int)flag[1] | 1;
+    h->fem_tmax = (int)flag[2];
+    if (fem_lds(h) > 64 * 1024)
+        return ft::set_error(FT_ERR_ARG, "a node has %u tets: beyond the assembly's workgroup memory", flag[2]);
     const FemEll e = fem_ell(h);
     if (ns > 0) {
         hipLaunchKernelGGL(k_fem_rowfill, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
-                           h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), N, e, w, h->fem_cols.as<int32_t>());
-        hipLaunchKernelGGL(k_fem_mass, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
-                           h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), h->xyz.as<float>(), N, fem_quad(), e,
-                           h->fem_m.as<double>());
+                           h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), N, e, w, h->fem_cols.as<int32_t>(),
+                           h->fem_map.as<uint32_t>());
+        hipLaunchKernelGGL(k_fem_mass, dim3(fem_grid(N)), dim3(64), fem_lds(h), s,
+                           h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(),
+                           h->tets.as<int4>(), h->xyz.as<float>(), N, fem_quad(), e, h->fem_stride,
+                           h->fem_tmax, h->fem_m.as<double>());
     }
     FT_TRY(check_launch());
     FT_HIP(hipStreamSynchronize(s));
@@ -3813,8 +3885,8 @@ int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol
     if (!h->has_conn) return ft::set_error(FT_ERR_STATE, "advection needs a mesh (ft_set_mesh)");
     const int64_t N = h->N;
     if (dt < FLT_MIN || N == 0) { info->skipped = 1; info->converged = 1; return FT_OK; }   // :92-93
-    const int64_t nb = (N + kBcgBS - 1) / kBcgBS, ns = (N + kFemSlice - 1) / kFemSlice;
-    FT_TRY(ensure(h->bcg_part, 16 * (size_t)std::max(nb, ns)));
+    const int64_t nb = std::min<int64_t>((N + kBcgBS - 1) / kBcgBS, kBcgGrid);
+    FT_TRY(ensure(h->bcg_part, 16 * (size_t)nb));
     FT_TRY(ensure(h->bcg_sc, sizeof(BcgScalars)));
     if (!h->h_bcg) FT_HIP(hipHostMalloc(&h->h_bcg, sizeof(BcgScalars), hipHostMallocDefault));
     BcgScalars* hs = static_cast<BcgScalars*>(h->h_bcg);
@@ -3832,11 +3904,13 @@ int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol
            *invdiag = vec + 6 * N, *b = vec + 7 * N;
     const FemEll e = fem_ell(h);
     double* lv = h->fem_l.as<double>();
-    hipLaunchKernelGGL(k_fem_cn, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
-                       h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), h->xyz.as<float>(), d_flow, N, fem_quad(), e,
-                       h->fem_m.as<double>(), 0.5 * dt, d_phi, lv, invdiag, b, x, r, r0, p, v, part);
+    hipLaunchKernelGGL(k_fem_cn, dim3(fem_grid(N)), dim3(64), fem_lds(h), s, h->n2t_off.as<uint64_t>(),
+                       h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(), h->tets.as<int4>(),
+                       h->xyz.as<float>(), d_flow, N, fem_quad(), e, h->fem_stride, h->fem_tmax,
+                       h->fem_m.as<double>(), 0.5 * dt, d_phi, lv, invdiag, b, x, r, r0, p, v);
+    hipLaunchKernelGGL(k_dots2, dim3((unsigned)nb), dim3(kBcgBS), 0, s, N, r, r, b, b, part);
     FT_TRY(check_launch());
-    FT_TRY(bcg_fin(h, s, ns, BCG_INIT, true));
+    FT_TRY(bcg_fin(h, s, nb, BCG_INIT, true));
     const double tol_ = tol > 0.0 ? tol : DBL_EPSILON;                 // Eigen's defaults
     const int64_t max_it = max_iters > 0 ? max_iters : 2 * N;
     if (hs->bb == 0.0) {                                               // zero right-hand side: x = 0
@@ -3934,9 +4008,10 @@ int ft_fem_assemble(ft_handle h, int32_t form, const double* vel, int64_t* row_o
     if (form == 1) {
         FT_TRY(ensure(h->adv_flow, 32 * at_least1(N)));
         FT_HIP(hipMemcpyAsync(h->adv_flow.p, vel, 24 * N, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_fem_adv, dim3((unsigned)((N + kFemSlice - 1) / kFemSlice)), dim3(kFemSlice), 0, s,
-                           h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->tets.as<int4>(),
-                           h->xyz.as<float>(), h->adv_flow.as<double>(), N, fem_quad(), e, h->fem_l.as<double>());
+        hipLaunchKernelGGL(k_fem_adv, dim3(fem_grid(N)), dim3(64), fem_lds(h), s, h->n2t_off.as<uint64_t>(),
+                           h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(), h->tets.as<int4>(),
+                           h->xyz.as<float>(), h->adv_flow.as<double>(), N, fem_quad(), e, h->fem_stride,
+                           h->fem_tmax, h->fem_l.as<double>());
         src = h->fem_l.as<double>();
     }
     const int64_t cbytes = (4 * at_least1(nnz) + 255) & ~(int64_t)255;   // CSR staging: cols, vals
