@@ -1636,7 +1636,6 @@ __global__ __launch_bounds__(kFemSlice) void k_fem_rowlen(const uint64_t* n2t_of
         cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, lst[threadIdx.x]);
         if (cnt < 0) { atomicOr(&flags[0], 1u); cnt = 0; }
         len[i] = cnt;
-        atomicMax(&flags[2], (unsigned)(n2t_off[i + 1] - n2t_off[i]));   // most tets at a node
     }
     int wmax = cnt;
     for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
@@ -1680,92 +1679,119 @@ __global__ __launch_bounds__(kFemSlice) void k_fem_rowfill(const uint64_t* n2t_o
 // ((f0 w + f2 w) + (f1 w + f3 w)) |det B| of f(u = phi_j, v = phi_i) is
 // added to column t(j):  mass  f = u v;  advection  f = V1 (ux v) + V2 (uy v)
 // + V3 (uz v) with V_c(q) = sum_j V_c[t(j)] phi_j(q) accumulated from 0 in
-// j order.
+// j order (vel: the velocity interleaved per node).
 //
-// kFemLanes lanes share a row: they compute the contributions of disjoint
-// tets into LDS (qb, four per tet), then the row's first lane adds them into
-// the row accumulator in tet order, so every entry is summed exactly as the
-// reference's std::map does.  A workgroup is one wave of kFemRowsPerWave rows.
+// kFemLanes lanes share a row and take its tets in rounds: in each round lane
+// L computes the four contributions of tet kFemLanes r + L into LDS, then the
+// row's first lane adds that round's tets into the row accumulator in tet
+// order, so every entry is summed exactly as the reference's std::map does.
+// A workgroup is one wave of kFemRowsPerWave rows.
 constexpr int kFemLanes = 4;
 constexpr int kFemRowsPerWave = 64 / kFemLanes;
 
-// LDS: kFemRowsPerWave accumulators of `stride` doubles, then as many
-// buffers of 4 tmax doubles.
-__host__ __device__ inline size_t fem_lds_bytes(int stride, int tmax)
+// LDS: kFemRowsPerWave accumulators of `stride` doubles, then as many round
+// buffers of 4 kFemLanes doubles.
+__host__ __device__ inline size_t fem_lds_bytes(int stride)
 {
-    return sizeof(double) * (size_t)kFemRowsPerWave * ((size_t)stride + 4 * (size_t)tmax);
+    return sizeof(double) * (size_t)kFemRowsPerWave * ((size_t)stride + 4 * kFemLanes);
 }
 
 template <int FORM>
 __device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const uint32_t* fmap,
                          const int4* tets, const float* xyz, const double* vel, int64_t N, const FemQuad& qd,
-                         const FemEll& e, int stride, int tmax, double* lds, int64_t& row, double*& acc,
-                         int& len)
+                         const FemEll& e, int stride, double* lds, int64_t& row, double*& acc, int& len)
 {
     const int lane = threadIdx.x % kFemLanes, rl = threadIdx.x / kFemLanes;
     const int64_t i = (int64_t)blockIdx.x * kFemRowsPerWave + rl;
-    double* qb = lds + (size_t)kFemRowsPerWave * stride + (size_t)rl * 4 * tmax;
+    double* qb = lds + (size_t)kFemRowsPerWave * stride + (size_t)rl * 4 * kFemLanes;
     acc = lds + (size_t)rl * stride;
     row = i;
     len = 0;
-    uint64_t a0 = 0, na = 0;
+    uint64_t a0 = 0;
+    int na = 0;
     if (i < N) {
         a0 = n2t_off[i];
-        na = n2t_off[i + 1] - a0;
-        for (uint64_t a = lane; a < na; a += kFemLanes) {
+        na = (int)(n2t_off[i + 1] - a0);
+        if (lane == 0) {
+            len = e.len[i];
+            for (int k = 0; k < len; k++) acc[k] = 0.0;
+        }
+    }
+    int na_max = na;                                   // rounds are uniform over the wave
+    for (int o = 32; o >= 1; o >>= 1) na_max = max(na_max, __shfl_xor(na_max, o));
+    for (int r0 = 0; r0 < na_max; r0 += kFemLanes) {
+        const int a = r0 + lane;
+        if (a < na) {
             const int4 q = tets[n2t_ids[a0 + a]];
             const int qi[4] = {q.x, q.y, q.z, q.w};
             double G[4][3], detB;
             if (!fem_frame(xyz, qi, G, detB)) {
-                for (int j = 0; j < 4; j++) qb[4 * a + j] = 0.0;   // no contribution
-                continue;
-            }
-            const int li = qi[0] == (int)i ? 0 : qi[1] == (int)i ? 1 : qi[2] == (int)i ? 2 : 3;
-            double V[3][4];
-            if (FORM == 1) {
-                for (int c = 0; c < 3; c++)
-                    for (int q4 = 0; q4 < 4; q4++) {
-                        double acc_v = 0.0;
-                        for (int j = 0; j < 4; j++) acc_v = acc_v + vel[c * N + qi[j]] * qd.phi[4 * j + q4];
-                        V[c][q4] = acc_v;
-                    }
-            }
-            for (int j = 0; j < 4; j++) {
-                double f[4];
-                for (int q4 = 0; q4 < 4; q4++) {
-                    const double v = qd.phi[4 * li + q4];
-                    if (FORM == 0)
-                        f[q4] = qd.phi[4 * j + q4] * v;
-                    else
-                        f[q4] = (V[0][q4] * (G[j][0] * v) + V[1][q4] * (G[j][1] * v)) + V[2][q4] * (G[j][2] * v);
-                }
+                for (int j = 0; j < 4; j++) qb[4 * lane + j] = 0.0;   // no contribution
+            } else {
+                const int li = qi[0] == (int)i ? 0 : qi[1] == (int)i ? 1 : qi[2] == (int)i ? 2 : 3;
+                // quadrature points in the order 0, 2, 1, 3: s02 = f0 w + f2 w and
+                // s13 = f1 w + f3 w per basis function, then (s02 + s13) |det B|
                 const double w = qd.w;
-                qb[4 * a + j] = ((f[0] * w + f[2] * w) + (f[1] * w + f[3] * w)) * detB;
+                double s02[4], s13[4];
+                for (int qq = 0; qq < 4; qq++) {
+                    const int q4 = qq == 1 ? 2 : qq == 2 ? 1 : qq;
+                    const double v = qd.phi[4 * li + q4];
+                    double V0 = 0.0, V1 = 0.0, V2 = 0.0;
+                    if (FORM == 1) {
+                        for (int j = 0; j < 4; j++) {
+                            const double* vj = vel + 3 * (int64_t)qi[j];
+                            const double pj = qd.phi[4 * j + q4];
+                            V0 = V0 + vj[0] * pj;
+                            V1 = V1 + vj[1] * pj;
+                            V2 = V2 + vj[2] * pj;
+                        }
+                    }
+                    for (int j = 0; j < 4; j++) {
+                        const double f = FORM == 0 ? qd.phi[4 * j + q4] * v
+                                                   : (V0 * (G[j][0] * v) + V1 * (G[j][1] * v)) + V2 * (G[j][2] * v);
+                        const double pw = f * w;
+                        if (qq == 0) s02[j] = pw;
+                        else if (qq == 1) s02[j] = s02[j] + pw;
+                        else if (qq == 2) s13[j] = pw;
+                        else s13[j] = s13[j] + pw;
+                    }
+                }
+                for (int j = 0; j < 4; j++) qb[4 * lane + j] = (s02[j] + s13[j]) * detB;
             }
         }
-    }
-    __syncthreads();
-    if (i < N && lane == 0) {
-        len = e.len[i];
-        for (int k = 0; k < len; k++) acc[k] = 0.0;
-        for (uint64_t a = 0; a < na; a++) {
-            const uint32_t pos = fmap[a0 + a];
-            for (int j = 0; j < 4; j++) acc[(pos >> (8 * j)) & 0xffu] += qb[4 * a + j];
+        __syncthreads();
+        if (lane == 0) {
+            const int nt = min(kFemLanes, na - r0);
+            for (int t = 0; t < nt; t++) {
+                const uint32_t pos = fmap[a0 + r0 + t];
+                for (int j = 0; j < 4; j++) acc[(pos >> (8 * j)) & 0xffu] += qb[4 * t + j];
+            }
         }
+        __syncthreads();
     }
+}
+
+// The velocity interleaved per node (vx, vy, vz), so that a tet's gathers
+// touch one 24-byte run per node instead of three arrays.
+__global__ __launch_bounds__(kBS) void k_vel_interleave(const double* flow, int64_t N, double* vel3)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (i >= N) return;
+    vel3[3 * i] = flow[i];
+    vel3[3 * i + 1] = flow[N + i];
+    vel3[3 * i + 2] = flow[2 * N + i];
 }
 
 // The mass matrix, once per mesh (LevelSet::init, LevelSet.cpp:35-39).
 __global__ __launch_bounds__(64) void k_fem_mass(const uint64_t* n2t_off, const int32_t* n2t_ids,
                                                  const uint32_t* fmap, const int4* tets, const float* xyz,
-                                                 int64_t N, FemQuad qd, FemEll e, int stride, int tmax,
-                                                 double* mvals)
+                                                 int64_t N, FemQuad qd, FemEll e, int stride, double* mvals)
 {
     extern __shared__ double fem_lds[];
     int64_t i;
     double* acc;
     int len;
-    fem_rows<0>(n2t_off, n2t_ids, fmap, tets, xyz, nullptr, N, qd, e, stride, tmax, fem_lds, i, acc, len);
+    fem_rows<0>(n2t_off, n2t_ids, fmap, tets, xyz, nullptr, N, qd, e, stride, fem_lds, i, acc, len);
     for (int k = 0; k < len; k++) mvals[ell_at(e, i, k)] = acc[k];
 }
 
@@ -1773,13 +1799,13 @@ __global__ __launch_bounds__(64) void k_fem_mass(const uint64_t* n2t_off, const 
 __global__ __launch_bounds__(64) void k_fem_adv(const uint64_t* n2t_off, const int32_t* n2t_ids,
                                                 const uint32_t* fmap, const int4* tets, const float* xyz,
                                                 const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
-                                                int tmax, double* vals)
+                                                double* vals)
 {
     extern __shared__ double fem_lds[];
     int64_t i;
     double* acc;
     int len;
-    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, tmax, fem_lds, i, acc, len);
+    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, fem_lds, i, acc, len);
     for (int k = 0; k < len; k++) vals[ell_at(e, i, k)] = acc[k];
 }
 
@@ -1821,10 +1847,11 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* part)
 // DiagonalPreconditioner), b = R phi and r = b - L phi (BiCGSTAB's first
 // residual from the guess phi), each summed in column order; r0 = r, x = phi,
 // p = v = 0.
-__global__ __launch_bounds__(64) void k_fem_cn(const uint64_t* n2t_off, const int32_t* n2t_ids,
+// At least 5 waves per SIMD: 94 VGPRs, no spills (measured faster than 118 at 4).
+__global__ __launch_bounds__(64, 5) void k_fem_cn(const uint64_t* n2t_off, const int32_t* n2t_ids,
                                                const uint32_t* fmap, const int4* tets, const float* xyz,
                                                const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
-                                               int tmax, const double* mvals, double s, const double* phi,
+                                               const double* mvals, double s, const double* phi,
                                                double* lvals, double* invdiag, double* b, double* x, double* r,
                                                double* r0, double* p, double* v)
 {
@@ -1832,7 +1859,7 @@ __global__ __launch_bounds__(64) void k_fem_cn(const uint64_t* n2t_off, const in
     int64_t i;
     double* A;
     int len;
-    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, tmax, fem_lds, i, A, len);
+    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, fem_lds, i, A, len);
     if (i < N && threadIdx.x % kFemLanes == 0) {
         double bi = 0.0, lx = 0.0, id = 1.0;
         for (int k = 0; k < len; k++) {
@@ -2266,11 +2293,10 @@ struct ft_ctx {
     // advection: sliced-ELL pattern and mass matrix (per layout), the
     // Crank-Nicolson matrix, BiCGSTAB vectors, block partials and scalars
     DevBuf fem_len, fem_w, fem_soff, fem_cols, fem_m, fem_l, fem_flag, fem_map;
-    DevBuf bcg_vec, bcg_part, bcg_sc, adv_phi, adv_flow, fem_csr_off;
+    DevBuf bcg_vec, bcg_part, bcg_sc, adv_phi, adv_flow, adv_vel3, fem_csr_off;
     uint64_t fem_layout = ~0ull;
     int64_t fem_nnz = 0;             // ELL entries with padding
     int fem_stride = 1;              // LDS doubles per row accumulator (longest row, odd)
-    int fem_tmax = 0;                // most tets at a node
     void* h_bcg = nullptr;           // pinned mirror of the solver scalars
     const double* last_phi = nullptr;   // device phi of the last reinit (blended)
     uint64_t n2t_layout = ~0ull;
@@ -2321,7 +2347,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
             &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f,
             &h->fem_len, &h->fem_w, &h->fem_soff, &h->fem_cols, &h->fem_m, &h->fem_l, &h->fem_flag, &h->fem_map,
-            &h->bcg_vec, &h->bcg_part, &h->bcg_sc, &h->adv_phi, &h->adv_flow, &h->fem_csr_off};
+            &h->bcg_vec, &h->bcg_part, &h->bcg_sc, &h->adv_phi, &h->adv_flow, &h->adv_vel3, &h->fem_csr_off};
 }
 
 std::vector<void*> buffer_addresses(ft_ctx* c)
@@ -3801,7 +3827,7 @@ FemQuad fem_quad()
     return q;
 }
 
-size_t fem_lds(const ft_ctx* h) { return fem_lds_bytes(h->fem_stride, h->fem_tmax); }
+size_t fem_lds(const ft_ctx* h) { return fem_lds_bytes(h->fem_stride); }
 unsigned fem_grid(int64_t N) { return (unsigned)((N + kFemRowsPerWave - 1) / kFemRowsPerWave); }
 
 FemEll fem_ell(ft_ctx* h)
@@ -3823,7 +3849,7 @@ int fem_setup(ft_ctx* h, hipStream_t s)
     FT_TRY(ensure(h->fem_flag, 16));
     uint32_t* w = h->fem_w.as<uint32_t>();
     uint64_t* soff = h->fem_soff.as<uint64_t>();
-    FT_TRY(fill<unsigned>(s, h->fem_flag.as<unsigned>(), 3, 0u));   // overflow, longest row, most tets
+    FT_TRY(fill<unsigned>(s, h->fem_flag.as<unsigned>(), 2, 0u));   // overflow, longest row
     FT_TRY(fill<uint64_t>(s, soff + ns, 1, 0ull));
     if (ns > 0) {
         hipLaunchKernelGGL(k_fem_rowlen, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
@@ -3832,9 +3858,9 @@ int fem_setup(ft_ctx* h, hipStream_t s)
         FT_TRY(scan<uint32_t, uint64_t>(h, s, w, soff, ns, soff + ns, LoadSame{}));
     }
     uint64_t total = 0;
-    unsigned flag[3] = {0, 0, 0};
+    unsigned flag[2] = {0, 0};
     FT_HIP(hipMemcpyAsync(&total, soff + ns, 8, hipMemcpyDeviceToHost, s));
-    FT_HIP(hipMemcpyAsync(flag, h->fem_flag.p, 12, hipMemcpyDeviceToHost, s));
+    FT_HIP(hipMemcpyAsync(flag, h->fem_flag.p, 8, hipMemcpyDeviceToHost, s));
     FT_HIP(hipStreamSynchronize(s));
     if (flag[0])
         return ft::set_error(FT_ERR_ARG, "a node shares tets with more than %d nodes", kFemMaxCols - 1);
@@ -3844,9 +3870,6 @@ int fem_setup(ft_ctx* h, hipStream_t s)
     FT_TRY(ensure(h->fem_l, 8 * at_least1(nnz)));
     FT_TRY(ensure(h->fem_map, 4 * at_least1(4 * h->T)));             // one word per n2t entry
     h->fem_stride = (int)flag[1] | 1;
-    h->fem_tmax = (int)flag[2];
-    if (fem_lds(h) > 64 * 1024)
-        return ft::set_error(FT_ERR_ARG, "a node has %u tets: beyond the assembly's workgroup memory", flag[2]);
     const FemEll e = fem_ell(h);
     if (ns > 0) {
         hipLaunchKernelGGL(k_fem_rowfill, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
@@ -3855,7 +3878,7 @@ int fem_setup(ft_ctx* h, hipStream_t s)
         hipLaunchKernelGGL(k_fem_mass, dim3(fem_grid(N)), dim3(64), fem_lds(h), s,
                            h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(),
                            h->tets.as<int4>(), h->xyz.as<float>(), N, fem_quad(), e, h->fem_stride,
-                           h->fem_tmax, h->fem_m.as<double>());
+                           h->fem_m.as<double>());
     }
     FT_TRY(check_launch());
     FT_HIP(hipStreamSynchronize(s));
@@ -3904,9 +3927,11 @@ int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol
            *invdiag = vec + 6 * N, *b = vec + 7 * N;
     const FemEll e = fem_ell(h);
     double* lv = h->fem_l.as<double>();
+    FT_TRY(ensure(h->adv_vel3, 24 * (size_t)N));
+    hipLaunchKernelGGL(k_vel_interleave, dim3(grid1d(N)), dim3(kBS), 0, s, d_flow, N, h->adv_vel3.as<double>());
     hipLaunchKernelGGL(k_fem_cn, dim3(fem_grid(N)), dim3(64), fem_lds(h), s, h->n2t_off.as<uint64_t>(),
                        h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(), h->tets.as<int4>(),
-                       h->xyz.as<float>(), d_flow, N, fem_quad(), e, h->fem_stride, h->fem_tmax,
+                       h->xyz.as<float>(), h->adv_vel3.as<double>(), N, fem_quad(), e, h->fem_stride,
                        h->fem_m.as<double>(), 0.5 * dt, d_phi, lv, invdiag, b, x, r, r0, p, v);
     hipLaunchKernelGGL(k_dots2, dim3((unsigned)nb), dim3(kBcgBS), 0, s, N, r, r, b, b, part);
     FT_TRY(check_launch());
@@ -4008,10 +4033,13 @@ int ft_fem_assemble(ft_handle h, int32_t form, const double* vel, int64_t* row_o
     if (form == 1) {
         FT_TRY(ensure(h->adv_flow, 32 * at_least1(N)));
         FT_HIP(hipMemcpyAsync(h->adv_flow.p, vel, 24 * N, hipMemcpyHostToDevice, s));
+        FT_TRY(ensure(h->adv_vel3, 24 * (size_t)N));
+        hipLaunchKernelGGL(k_vel_interleave, dim3(grid1d(N)), dim3(kBS), 0, s, h->adv_flow.as<double>(), N,
+                           h->adv_vel3.as<double>());
         hipLaunchKernelGGL(k_fem_adv, dim3(fem_grid(N)), dim3(64), fem_lds(h), s, h->n2t_off.as<uint64_t>(),
                            h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(), h->tets.as<int4>(),
-                           h->xyz.as<float>(), h->adv_flow.as<double>(), N, fem_quad(), e, h->fem_stride,
-                           h->fem_tmax, h->fem_l.as<double>());
+                           h->xyz.as<float>(), h->adv_vel3.as<double>(), N, fem_quad(), e, h->fem_stride,
+                           h->fem_l.as<double>());
         src = h->fem_l.as<double>();
     }
     const int64_t cbytes = (4 * at_least1(nnz) + 255) & ~(int64_t)255;   // CSR staging: cols, vals
