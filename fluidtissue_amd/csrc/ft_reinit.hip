@@ -1271,14 +1271,18 @@ __global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
         S.test_leaf_uniform(~root);
     } else if (root >= 0) {
         if (a.probe) {
-            // Greedy descent for the tile's middle slot to the leaf nearest to
-            // it by bound: a first exact candidate for every lane.
+            // Greedy descent for the tile's middle slot to a leaf near it: a
+            // first exact candidate for every lane.  The child whose bound
+            // centre is nearer is taken (cheaper than the bounds and, in a
+            // model of the traversal, a seed after which less is tested).
             const float cx = __shfl(px, 32), cy = __shfl(py, 32), cz = __shfl(pz, 32);
             int cur = root;
             for (int depth = 0; depth < kStack + 1 && cur >= 0; depth++) {
                 const BNode nd = ftd::cload(a.nodes, cur);
-                const float ll = ftd::bound_dist2(cx, cy, cz, nd.l);
-                const float lr = ftd::bound_dist2(cx, cy, cz, nd.r);
+                const float lx = cx - nd.l.c[0], ly = cy - nd.l.c[1], lz = cz - nd.l.c[2];
+                const float rx = cx - nd.r.c[0], ry = cy - nd.r.c[1], rz = cz - nd.r.c[2];
+                const float ll = __fmaf_rn(lz, lz, __fmaf_rn(ly, ly, lx * lx));
+                const float lr = __fmaf_rn(rz, rz, __fmaf_rn(ry, ry, rx * rx));
                 cur = __builtin_amdgcn_readfirstlane(ll <= lr ? nd.left : nd.right);
             }
             if (cur < 0) S.test_leaf_uniform(~cur);
