@@ -109,3 +109,24 @@ def test_update_is_advect_then_reinit():
     ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, adv, blend=1)
     assert cnt.n_ifp == len(ref["ifp"])
     assert np.max(np.abs(out - ref["phi"])) < 1e-6
+
+
+def test_advect_host_arrays_equal_device_arrays():
+    """ft_advect stages large host arrays through pinned chunks (the flow here
+    is 12 MB); the result is the device entry point's, bit for bit."""
+    import ctypes as C
+    import torch
+    m = fa.kuhn_mesh(72, jitter=0.1, seed=4)
+    phi = fa.phi_field("spheres", m.xyz, seed=3)
+    flow = _flow(m, "rotation")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    host, info = r.advect(phi, flow, 0.005, tol=1e-10)
+    d_phi = torch.from_numpy(phi.copy()).cuda()
+    d_flow = torch.from_numpy(flow).cuda()
+    di = fa._lib.FtAdvectInfo()
+    fa._lib.check(fa._lib.lib.ft_advect_device(r._h, C.c_void_p(d_phi.data_ptr()), C.c_void_p(d_flow.data_ptr()),
+                                               0.005, 1e-10, 1000, C.byref(di), None))
+    r.close()
+    assert info["iterations"] == di.iterations and info["error"] == di.error
+    assert np.array_equal(host.view(np.uint64), d_phi.cpu().numpy().view(np.uint64))

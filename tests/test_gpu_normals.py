@@ -136,6 +136,27 @@ def test_surface_tension_matches_oracle(name, mode, blend):
     assert np.any(got[0] != f0[0])
 
 
+def test_surface_tension_host_arrays_equal_device_arrays():
+    """At 128^3 the force arrays (16.8 MB each) go through the pinned staging
+    of the host entry point; the result is the device entry point's."""
+    import ctypes as C
+    import torch
+    m, phi, h = workload("cfg2")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    r.reset_phi(phi, mode="triangles", blend="update", want_normals=True)
+    rng = np.random.default_rng(8)
+    f0 = [rng.standard_normal(len(m.xyz)) for _ in range(3)]
+    got = r.surface_tension(1.3, 2, *[a.copy() for a in f0])
+    d = [torch.from_numpy(a.copy()).cuda() for a in f0]
+    fa._lib.check(fa._lib.lib.ft_surface_tension_device(r._h, C.c_float(1.3), C.c_int32(2),
+                                                        *[C.c_void_p(t.data_ptr()) for t in d], None))
+    r.close()
+    for a in range(3):
+        assert np.array_equal(got[a].view(np.uint64), d[a].cpu().numpy().view(np.uint64))
+    assert np.count_nonzero(got[0] != f0[0]) > 1000
+
+
 @pytest.mark.parametrize("unit", [True, False])
 def test_interface_obj_matches_reference_text(tmp_path, unit):
     """The OBJ export of the interface, byte for byte the text the reference's
