@@ -4178,8 +4178,8 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     hipStream_t s = h->stream;
     FT_TRY(ensure(h->phi_tmp, 8 * std::max<int64_t>(1, h->N)));
     FT_TRY(ensure(h->ifp, 12 * std::max<int64_t>(1, n_ifp)));
-    if (n_ifp > 0) FT_HIP(hipMemcpyAsync(h->ifp.p, ifp, 12 * n_ifp, hipMemcpyHostToDevice, s));
-    if (h->N > 0) FT_HIP(hipMemcpyAsync(h->phi_tmp.p, phi_n, 8 * h->N, hipMemcpyHostToDevice, s));
+    FT_TRY(copy_h2d(h, h->ifp.p, ifp, 12 * (size_t)n_ifp, s));
+    FT_TRY(copy_h2d(h, h->phi_tmp.p, phi_n, 8 * (size_t)h->N, s));
     h->have_interface = false;
     // the crossing points are given: their count goes where the edge scan
     // would have put it, and the search runs as in a reinit
@@ -4203,9 +4203,7 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     ft_counts cnt{};
     FT_TRY(collect(h, s, true, &cnt));
     h->have_interface = false;        // points only: no triangles, p2t or areas
-    if (h->N > 0) FT_HIP(hipMemcpyAsync(phi_n, h->phi_tmp.p, 8 * h->N, hipMemcpyDeviceToHost, s));
-    FT_HIP(hipStreamSynchronize(s));
-    return FT_OK;
+    return copy_d2h(h, {{phi_n, h->phi_tmp.p, 8 * (size_t)h->N}}, s);
 }
 
 int ft_reinit_device(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts* o,
