@@ -22,11 +22,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
-    "cfg1": dict(n=32, jitter=0.0, phi="sphere_wave", desc="32^3 Kuhn, sphere+wave"),
-    "cfg2": dict(n=128, jitter=0.0, phi="star", band_h=6.0, desc="128^3 Kuhn, star surface"),
-    "cfg3": dict(n=256, jitter=0.2, phi="spheres",
+    "sphere_wave_32": dict(n=32, jitter=0.0, phi="sphere_wave", desc="32^3 Kuhn, sphere+wave"),
+    "star_128": dict(n=128, jitter=0.0, phi="star", band_h=6.0, desc="128^3 Kuhn, star surface"),
+    "spheres_256": dict(n=256, jitter=0.2, phi="spheres",
                  desc="256^3 Kuhn, interior nodes jittered +-0.2h, 64 spheres"),
-    "cfg3_128": dict(n=128, jitter=0.2, phi="spheres",
+    "spheres_128": dict(n=128, jitter=0.2, phi="spheres",
                      desc="128^3 Kuhn, interior nodes jittered +-0.2h, 64 spheres (the 256^3 field, coarser)"),
 }
 
@@ -438,7 +438,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS) + ["batched", "serving", "advect"])
+    ap.add_argument("--workload", default="spheres_256", choices=sorted(WORKLOADS) + ["batched", "serving", "advect"])
     ap.add_argument("--advect-n", type=int, default=256)
     ap.add_argument("--fields", type=int, default=512)
     ap.add_argument("--jobs", type=int, default=2000)
@@ -451,7 +451,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one field on one GPU; --gpus must be 1 "
-              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+              "(sharding one field over several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -495,7 +495,7 @@ def main():
         def copy(dst, src, nb):
             d_phi.copy_(d_phi0)
         bs, qb, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
-                                      mode=args.mode, reps=1, sync=torch.cuda.synchronize, copy=copy,
+                                      mode=args.mode, reps=3, sync=torch.cuda.synchronize, copy=copy,
                                       bin_scales=(1.0, 1.25, 1.5, 2.0), query_blocks=(64, 256), band=band)
         args.bin_scale, args.query_block = bs, qb
         tuned = {"bin_scale": bs, "query_block": qb,

@@ -83,7 +83,7 @@ except ImportError:
     pass
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    raise ImportError(f"{LIB_PATH} is not built; run `python fluidtissue_amd/build.py`")
 
 lib = C.CDLL(LIB_PATH)
 

@@ -48,7 +48,7 @@ void EuclideanDistance::init(const float* meshp, const double* phi0, const int n
     ft_device_count(&count);
     m_devCount = count;
     if (count <= 0)
-        throw std::runtime_error("EuclideanDistance::init: no HIP device (this build has no CPU "
+        throw std::runtime_error("EuclideanDistance::init: no HIP device (this library has no CPU "
                                  "branch: the distance runs on the GPU only)");
     ft_handle h = handle_of(this);
     if (!h) {

@@ -1,6 +1,6 @@
 """CPU oracle for the reinit path -- TEST INFRASTRUCTURE ONLY.
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
-this package; the product (fluidtissue_amd / libftreinit.so) never does.
+Only tests/, the smoke check and bench.py's cpu_baseline leg may use this
+package; the product (fluidtissue_amd / libftreinit.so) never does.
 """
 from .oracle import *  # noqa: F401,F403

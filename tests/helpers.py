@@ -10,19 +10,19 @@ import oracle
 @functools.lru_cache(maxsize=None)
 def workload(name):
     """The benchmark workloads at test sizes: (mesh, phi, h)."""
-    if name == "cfg1":          # 32^3 sphere with a wave: sign right, phi not a distance
+    if name == "sphere_wave_32":          # 32^3 sphere with a wave: sign right, phi not a distance
         m = fa.kuhn_mesh(32)
         return m, fa.phi_field("sphere_wave", m.xyz), 1.0 / 31
-    if name == "cfg2":          # 128^3 star-shaped surface
+    if name == "star_128":          # 128^3 star-shaped surface
         m = fa.kuhn_mesh(128)
         return m, fa.phi_field("star", m.xyz), 1.0 / 127
-    if name == "cfg2s":         # 64^3 star: the same field on a smaller grid
+    if name == "star_64":         # 64^3 star: the same field on a smaller grid
         m = fa.kuhn_mesh(64)
         return m, fa.phi_field("star", m.xyz), 1.0 / 63
-    if name == "cfg3s":         # 96^3 jittered mesh, 64 spheres
+    if name == "spheres_96":         # 96^3 jittered mesh, 64 spheres
         m = fa.kuhn_mesh(96, jitter=0.2, seed=1)
         return m, fa.phi_field("spheres", m.xyz, seed=0), 1.0 / 95
-    if name == "cfg3":          # 256^3 jittered mesh, 64 spheres
+    if name == "spheres_256":          # 256^3 jittered mesh, 64 spheres
         m = fa.kuhn_mesh(256, jitter=0.2, seed=1)
         return m, fa.phi_field("spheres", m.xyz, seed=0), 1.0 / 255
     if name == "ellipsoids":

@@ -10,8 +10,8 @@ from tests.helpers import workload
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name,mode", [("cfg1", "points"), ("cfg1", "triangles"),
-                                       ("ellipsoids", "triangles"), ("cfg3s", "triangles")])
+@pytest.mark.parametrize("name,mode", [("sphere_wave_32", "points"), ("sphere_wave_32", "triangles"),
+                                       ("ellipsoids", "triangles"), ("spheres_96", "triangles")])
 def test_normals_match_oracle(name, mode):
     m, phi, h = workload(name)
     r = fa.Reinit(0)
@@ -70,7 +70,7 @@ def test_normals_batch_equals_alone_and_graph():
 
 
 def test_normals_need_a_reinit_with_them():
-    m, phi, h = workload("cfg1")
+    m, phi, h = workload("sphere_wave_32")
     r = fa.Reinit(0)
     r.set_mesh(m)
     r.reset_phi(phi)
@@ -79,8 +79,8 @@ def test_normals_need_a_reinit_with_them():
     r.close()
 
 
-@pytest.mark.parametrize("name,mode,blend", [("cfg1", "points", "update"), ("ellipsoids", "triangles", "init"),
-                                             ("cfg2s", "triangles", "update")])
+@pytest.mark.parametrize("name,mode,blend", [("sphere_wave_32", "points", "update"), ("ellipsoids", "triangles", "init"),
+                                             ("star_64", "triangles", "update")])
 def test_interface_curvature_matches_oracle(name, mode, blend):
     """LevelSet::getInterfaceCurvature (LevelSet.cpp:215-253) on the device,
     bit for bit against the oracle from the same reinit outputs."""
@@ -109,8 +109,8 @@ def test_interface_curvature_sphere_on_device():
     assert abs(np.median(k) - 2 / R) < 0.05 * 2 / R
 
 
-@pytest.mark.parametrize("name,mode,blend", [("cfg1", "points", "update"), ("ellipsoids", "triangles", "init"),
-                                             ("cfg2s", "triangles", "update")])
+@pytest.mark.parametrize("name,mode,blend", [("sphere_wave_32", "points", "update"), ("ellipsoids", "triangles", "init"),
+                                             ("star_64", "triangles", "update")])
 def test_surface_tension_matches_oracle(name, mode, blend):
     """Flow::surfaceTension_ (src/growth/Flow.cpp:307-341) on the device, added
     into given force arrays, bit for bit against the oracle's serial loop; the
@@ -141,7 +141,7 @@ def test_surface_tension_host_arrays_equal_device_arrays():
     of the host entry point; the result is the device entry point's."""
     import ctypes as C
     import torch
-    m, phi, h = workload("cfg2")
+    m, phi, h = workload("star_128")
     r = fa.Reinit(0)
     r.set_mesh(m)
     r.reset_phi(phi, mode="triangles", blend="update", want_normals=True)
@@ -162,13 +162,13 @@ def test_interface_obj_matches_reference_text(tmp_path, unit):
     """The OBJ export of the interface, byte for byte the text the reference's
     writer prints for the oracle's interface."""
     from tests.helpers import EXPECTED_MTL, expected_interface_obj, oracle_interface
-    m, phi, h = workload("cfg1")
+    m, phi, h = workload("sphere_wave_32")
     r = fa.Reinit(0)
     r.set_mesh(m)
     r.reset_phi(phi, mode="triangles")
     base = str(tmp_path / "shape")
     r.write_obj(base, unit_normals=unit)
     r.close()
-    ref = oracle_interface("cfg1")
+    ref = oracle_interface("sphere_wave_32")
     assert open(base + ".obj").read() == expected_interface_obj(ref["ifp"], ref["tris"], base, unit)
     assert open(base + ".mtl").read() == EXPECTED_MTL

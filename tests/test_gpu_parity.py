@@ -31,7 +31,7 @@ def _run(rein, name, **kw):
     return m, phi, h, out, ifn, cnt
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2s", "cfg3s", "ellipsoids"])
+@pytest.mark.parametrize("name", ["sphere_wave_32", "star_64", "spheres_96", "ellipsoids"])
 def test_marching_tets_bit_exact(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name)
     ref = oracle_interface(name)
@@ -41,12 +41,12 @@ def test_marching_tets_bit_exact(rein, name):
     assert_interface_bit_exact(rein.interface(), ref)
 
 
-def test_marching_tets_bit_exact_cfg2(rein):
-    m, phi, h, out, ifn, cnt = _run(rein, "cfg2")
-    assert_interface_bit_exact(rein.interface(), oracle_interface("cfg2"))
+def test_marching_tets_bit_exact_star_128(rein):
+    m, phi, h, out, ifn, cnt = _run(rein, "star_128")
+    assert_interface_bit_exact(rein.interface(), oracle_interface("star_128"))
 
 
-@pytest.mark.parametrize("name", ["cfg1", "ellipsoids"])
+@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids"])
 def test_points_mode_bit_exact_full(rein, name):
     """Reference semantics (EuclideanDistance.cu:14-35,108-117) on every node."""
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="points", blend="init", want_nearest=True)
@@ -62,7 +62,7 @@ def test_points_mode_bit_exact_full(rein, name):
     assert np.array_equal(out_u.view(np.uint64), ref_u["phi"].view(np.uint64))
 
 
-@pytest.mark.parametrize("name", ["cfg1", "ellipsoids"])
+@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids"])
 def test_update_blend_on_first_and_grown_calls(name):
     """The retained values of the update blend (LevelSet.cpp:443-447, 148) on
     the first call of a fresh handle, and after its buffers were regrown by
@@ -82,7 +82,7 @@ def test_update_blend_on_first_and_grown_calls(name):
     r.close()
 
 
-@pytest.mark.parametrize("name", ["cfg2", "cfg3s"])
+@pytest.mark.parametrize("name", ["star_128", "spheres_96"])
 def test_points_mode_bit_exact_sampled(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="points", want_nearest=True)
     ref = oracle_interface(name)
@@ -96,7 +96,7 @@ def test_points_mode_bit_exact_sampled(rein, name):
     assert np.array_equal(rein.nearest()[nodes], j)
 
 
-@pytest.mark.parametrize("name", ["cfg1", "ellipsoids"])
+@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids"])
 def test_triangle_mode_full(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="triangles", want_nearest=True)
     nodes = np.arange(m.n_nodes)
@@ -111,7 +111,7 @@ def test_triangle_mode_full(rein, name):
     assert np.all(np.abs(ifn) <= np.abs(ifn_p) * (1 + 1e-6))
 
 
-@pytest.mark.parametrize("name", ["cfg2s", "cfg3s"])
+@pytest.mark.parametrize("name", ["star_64", "spheres_96"])
 def test_triangle_mode_sampled(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="triangles", want_nearest=True)
     rng = np.random.default_rng(12)
@@ -124,7 +124,7 @@ def test_triangle_mode_sampled(rein, name):
     assert mism == 0
 
 
-@pytest.mark.parametrize("name,band_h,n_sample", [("cfg3", 0.0, 2000), ("cfg2", 6.0, 3000)])
+@pytest.mark.parametrize("name,band_h,n_sample", [("spheres_256", 0.0, 2000), ("star_128", 6.0, 3000)])
 def test_full_size_workloads(name, band_h, n_sample):
     """The benchmark configurations at full size: 256^3 jittered, 64 spheres,
     full domain; 128^3 star, narrow band 6h.  Interface bit-exact against the
@@ -158,7 +158,7 @@ def test_full_size_workloads(name, band_h, n_sample):
 
 def test_triangle_mode_band(rein):
     """Narrow band 6h: exact inside, clamped to +-band outside."""
-    name = "cfg2s"
+    name = "star_64"
     m, phi, h, _, _, _ = _run(rein, name)
     band = 6 * h
     out, ifn, cnt = rein.reset_phi(phi, mode="triangles", band=band, want_nearest=True)
@@ -178,8 +178,8 @@ def test_triangle_mode_band(rein):
 
 def test_euclidean_distance_abi(rein):
     """EuclideanDistance::init/resetPhi drop-in (host pointers)."""
-    m, phi, h = workload("cfg1")
-    ref = oracle_interface("cfg1")
+    m, phi, h = workload("sphere_wave_32")
+    ref = oracle_interface("sphere_wave_32")
     ed = fa.EuclideanDistance(0)
     ed.init(m.xyz, phi, m.n_nodes, 8)
     phi_n = phi.copy()
@@ -194,9 +194,9 @@ def test_euclidean_distance_abi(rein):
 
 
 def test_no_crossing_gives_flt_max(rein):
-    m, phi, h = workload("cfg1")
+    m, phi, h = workload("sphere_wave_32")
     rein.set_mesh(m)
-    rein._mesh_name = "cfg1"
+    rein._mesh_name = "sphere_wave_32"
     pos = np.abs(phi) + 1.0
     out, ifn, cnt = rein.reset_phi(pos, mode="points")
     assert cnt.n_ifp == 0 and cnt.n_tri == 0
@@ -247,7 +247,7 @@ def test_set_mesh_rejects_out_of_range(rein):
 
 def test_device_resident_matches_host(rein):
     import torch
-    m, phi, h, out, ifn, cnt = _run(rein, "cfg2s", mode="triangles")
+    m, phi, h, out, ifn, cnt = _run(rein, "star_64", mode="triangles")
     d_phi = torch.from_numpy(phi.copy()).cuda()
     d_ifn = torch.empty_like(d_phi)
     torch.cuda.synchronize()
