@@ -130,3 +130,28 @@ def test_advect_host_arrays_equal_device_arrays():
     r.close()
     assert info["iterations"] == di.iterations and info["error"] == di.error
     assert np.array_equal(host.view(np.uint64), d_phi.cpu().numpy().view(np.uint64))
+
+
+def test_advect_and_assembly_argument_errors():
+    import ctypes as C
+    meshes = [fa.kuhn_mesh(8), fa.kuhn_mesh(9)]
+    b = fa.Reinit(0)
+    b.set_batch(meshes)                                   # two fields: one solver per field is not built
+    n = sum(len(m.xyz) for m in meshes)
+    with pytest.raises(fa._lib.FtError) as e:
+        b.advect(np.zeros(n), np.ones(4 * n), 0.1)
+    assert e.value.code == fa._lib.FT_ERR_STATE
+    b.close()
+    m = fa.kuhn_mesh(8)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    off = np.empty(len(m.xyz) + 1, np.int64)
+    lib = fa._lib.lib
+    assert lib.ft_fem_assemble(r._h, 2, None, off.ctypes.data_as(C.c_void_p), None, None) == fa._lib.FT_ERR_ARG
+    cols = np.empty(10, np.int32)
+    vals = np.empty(10, np.float64)
+    assert lib.ft_fem_assemble(r._h, 1, None, off.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                               vals.ctypes.data_as(C.c_void_p)) == fa._lib.FT_ERR_ARG   # form 1 needs a velocity
+    with pytest.raises(ValueError):
+        r.advect(np.zeros(5), np.zeros(20), 0.1)          # wrong sizes
+    r.close()

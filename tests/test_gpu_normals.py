@@ -136,6 +136,20 @@ def test_surface_tension_matches_oracle(name, mode, blend):
     assert np.any(got[0] != f0[0])
 
 
+def test_surface_tension_needs_a_reinit_with_normals():
+    m, phi, h = workload("sphere_wave_32")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    with pytest.raises(fa._lib.FtError) as e:
+        r.surface_tension(1.0, 1)
+    assert e.value.code == fa._lib.FT_ERR_STATE
+    r.reset_phi(phi)                                      # no normals kept
+    with pytest.raises(fa._lib.FtError) as e:
+        r.surface_tension(1.0, 1)
+    assert e.value.code == fa._lib.FT_ERR_STATE
+    r.close()
+
+
 def test_surface_tension_host_arrays_equal_device_arrays():
     """At 128^3 the force arrays (16.8 MB each) go through the pinned staging
     of the host entry point; the result is the device entry point's."""
