@@ -1109,12 +1109,13 @@ struct Searcher {
     float sec;       // mode 1: second best squared distance and its triangle
     int32_t sj = -1;
 
-    // One triangle for all lanes: a wave-uniform early out when the disk
-    // bound rules it out for every lane, else the exact FP32 distance.
+    // One triangle for all lanes: a wave-uniform early out when the bounding
+    // rectangle of its frame rules it out for every lane, else the exact FP32
+    // distance.
     __device__ __forceinline__ void test_tri(const TriRec& r, int32_t j)
     {
         const ftd::TriProj pj = ftd::tri_project(x, y, z, r);
-        if (__ballot(ftd::tri_disk_lb2(pj, r) <= bnd) == 0ull) return;
+        if (__ballot(ftd::tri_rect_lb2(pj, r) <= bnd) == 0ull) return;
         consider(ftd::tri_dist2_from(pj, r), j);
     }
 
