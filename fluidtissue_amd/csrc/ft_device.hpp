@@ -253,9 +253,9 @@ __device__ inline double tri_dist2_f64(const double* x, const double* a, const d
 //   F0 = (a, +-R)  F1 = (u, B)  F2 = (v, n.x)  F3 = (n.y, n.z, Cx, Cy)
 //   F4 = (1/B, 1/|ac|^2, 1/|bc|^2, Cx - B)
 // (inverse lengths are 0 for zero-length edges; R is the radius, rounded up,
-// of the disk about the midpoint of ab that holds the triangle, negative for
-// collinear vertices, which have no interior and whose distance is that of
-// the edge segments).  The culling test reads the first 14 dwords only, one
+// of the disk about the midpoint of ab that holds the triangle; its sign is
+// negative for collinear vertices, which have no interior and whose distance
+// is that of the edge segments).  The culling test reads F0-F3 only, one
 // contiguous run of scalar loads.  The point is
 // projected once (X, Y, Z); the in-plane distance is 0 inside the triangle,
 // else the nearest clamped edge projection; d^2 = d2D^2 + Z^2.  With ab the
@@ -277,17 +277,6 @@ __device__ __forceinline__ TriProj tri_project(float x, float y, float z, const 
     p.Y = __fmaf_rn(wz, r.F2.z, __fmaf_rn(wy, r.F2.y, wx * r.F2.x));
     p.Z = __fmaf_rn(wz, r.F3.y, __fmaf_rn(wy, r.F3.x, wx * r.F2.w));
     return p;
-}
-
-// Lower bound from the disk about the midpoint of ab that holds the triangle
-// (radius |F0.w|); costs little once the point is projected.
-__device__ __forceinline__ float tri_disk_lb2(const TriProj& p, const TriRec& r)
-{
-    const float rho = fabsf(r.F0.w) * (1.0f + 0x1.0p-18f);
-    const float mx = __fmaf_rn(-0.5f, r.F1.w, p.X);
-    const float lat = sqrt_lb(__fmaf_rn(p.Y, p.Y, mx * mx));
-    const float g = fmaxf(lat - rho, 0.0f) * (1.0f - 0x1.0p-18f);
-    return __fmaf_rn(g, g, p.Z * p.Z) * (1.0f - 0x1.0p-18f);
 }
 
 // Lower bound from the rectangle [0, B] x [0, Cy] of the triangle's frame,
