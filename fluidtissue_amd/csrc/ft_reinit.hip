@@ -243,31 +243,46 @@ __global__ __launch_bounds__(kBS) void k_node_signs(const double* phi, int64_t n
     if (ftd::lane_id() == 0 && i < n) bits[i >> 6] = m;
 }
 
-// One tet per thread.  Crossing iff 1..3 of its nodes are negative
-// (LevelSet.cpp:380-383); 2 negative nodes -> 4 crossing edges -> 2 triangles.
+// Crossing iff 1..3 of a tet's nodes are negative (LevelSet.cpp:380-383); 2
+// negative nodes -> 4 crossing edges -> 2 triangles.  A workgroup handles
+// kClassifyGroups consecutive groups of kBS tets, one tet of each per
+// thread, so that each thread has that many tet loads in flight; per group
+// it writes the crossing bitmap words and blk[group] = (crossing tets << 32)
+// | triangles.
+constexpr int kClassifyGroups = 2;
 __global__ __launch_bounds__(kBS) void k_classify(const int4* tets, int64_t T,
                                                   const uint64_t* sbits, uint64_t* tbits,
                                                   uint64_t* blk)
 {
-    const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    int s = 0;
-    if (t < T) {
-        const int4 q = tets[t];
-        s = ftd::get_bit(sbits, q.x) + ftd::get_bit(sbits, q.y) + ftd::get_bit(sbits, q.z) +
-            ftd::get_bit(sbits, q.w);
+    __shared__ uint64_t part[kClassifyGroups][kBS / 64];
+    int4 q[kClassifyGroups];
+    int64_t t[kClassifyGroups];
+#pragma unroll
+    for (int g = 0; g < kClassifyGroups; g++) {
+        t[g] = ((int64_t)blockIdx.x * kClassifyGroups + g) * kBS + threadIdx.x;
+        if (t[g] < T) q[g] = tets[t[g]];
     }
-    const bool cross = (t < T) && s != 0 && s != 4;
-    const uint64_t m = __ballot(cross);
-    const uint64_t m2 = __ballot(cross && s == 2);
-    if (ftd::lane_id() == 0 && t < T) tbits[t >> 6] = m;
-    __shared__ uint64_t part[kBS / 64];
-    if (ftd::lane_id() == 0)
-        part[threadIdx.x / 64] = ((uint64_t)__popcll(m) << 32) | (uint64_t)(__popcll(m) + __popcll(m2));
+#pragma unroll
+    for (int g = 0; g < kClassifyGroups; g++) {
+        int sgn = 0;
+        if (t[g] < T)
+            sgn = ftd::get_bit(sbits, q[g].x) + ftd::get_bit(sbits, q[g].y) + ftd::get_bit(sbits, q[g].z) +
+                  ftd::get_bit(sbits, q[g].w);
+        const bool cross = (t[g] < T) && sgn != 0 && sgn != 4;
+        const uint64_t m = __ballot(cross);
+        const uint64_t m2 = __ballot(cross && sgn == 2);
+        if (ftd::lane_id() == 0 && t[g] < T) tbits[t[g] >> 6] = m;
+        if (ftd::lane_id() == 0)
+            part[g][threadIdx.x / 64] = ((uint64_t)__popcll(m) << 32) | (uint64_t)(__popcll(m) + __popcll(m2));
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t s4 = 0;
-        for (int w = 0; w < kBS / 64; w++) s4 += part[w];
-        blk[blockIdx.x] = s4;
+    if (threadIdx.x < kClassifyGroups) {
+        const int64_t grp = (int64_t)blockIdx.x * kClassifyGroups + threadIdx.x;
+        if (grp * kBS < T) {
+            uint64_t s4 = 0;
+            for (int w = 0; w < kBS / 64; w++) s4 += part[threadIdx.x][w];
+            blk[grp] = s4;
+        }
     }
 }
 
@@ -2867,8 +2882,9 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
     FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_EDGES, 1, 0ull));   // the edge scan writes the low 32 bits
     if (T > 0)
-        hipLaunchKernelGGL(k_classify, dim3((unsigned)nblk), dim3(kBS), 0, s, c->tets.as<int4>(), T,
-                           sbits, c->tbits.as<uint64_t>(), c->blk.as<uint64_t>());
+        hipLaunchKernelGGL(k_classify, dim3((unsigned)((nblk + kClassifyGroups - 1) / kClassifyGroups)),
+                           dim3(kBS), 0, s, c->tets.as<int4>(), T, sbits, c->tbits.as<uint64_t>(),
+                           c->blk.as<uint64_t>());
     FT_TRY(scan<uint64_t, uint64_t>(c, s, c->blk.as<uint64_t>(), c->blk.as<uint64_t>(), nblk,
                                     sc + S_BLK, LoadSame{}));
     if (eager) {
