@@ -1146,7 +1146,12 @@ struct Searcher {
                 const float4 p = ftd::cload(prec, s);
                 const int32_t j = ftd::cload(pidx, s);
                 const float dx = p.x - x, dy = p.y - y, dz = p.z - z;
-                const float d = sqrtf(dx * dx + dy * dy + dz * dz);   // EuclideanDistance.cu:20-28
+                const float s2 = dx * dx + dy * dy + dz * dz;
+                // bnd = best^2 (1 + 2^-16): beyond it sqrtf(s2) > best, which the
+                // strict compare would reject anyway, so the point is skipped
+                // when no lane is within it
+                if (__ballot(s2 <= bnd) == 0ull) continue;
+                const float d = sqrtf(s2);                            // EuclideanDistance.cu:20-28
                 consider(d, j);
             }
         } else {
