@@ -721,34 +721,91 @@ __global__ __launch_bounds__(kBS) void k_prim_scatter(int mode, const uint64_t* 
     trec[slot] = r;
 }
 
-// Leaf bounds: box of the vertices and, for triangles, a cylinder about the
-// area-weighted mean normal of the leaf's triangles.
-__global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, const uint64_t* sc, const uint32_t* leaf_start,
+// Unit eigenvector of the smallest eigenvalue of the symmetric 3x3 matrix C
+// (c00, c11, c22, c01, c02, c12): the eigenvalue in closed form (O. K. Smith,
+// "Eigenvalues of a symmetric 3 x 3 matrix", CACM 4(4), 1961), the vector as
+// the longest cross product of two rows of C - lambda I.  False when the
+// points are (nearly) isotropic or there is no clear smallest direction.
+__device__ inline bool smallest_eigvec(const double c[6], double n[3])
+{
+    const double p1 = c[3] * c[3] + c[4] * c[4] + c[5] * c[5];
+    const double q = (c[0] + c[1] + c[2]) / 3.0;
+    const double a = c[0] - q, b = c[1] - q, d = c[2] - q;
+    const double p2 = a * a + b * b + d * d + 2.0 * p1;
+    if (!(p2 > 0.0)) return false;
+    const double p = sqrt(p2 / 6.0);
+    const double B[6] = {a / p, b / p, d / p, c[3] / p, c[4] / p, c[5] / p};
+    const double det = B[0] * (B[1] * B[2] - B[5] * B[5]) - B[3] * (B[3] * B[2] - B[5] * B[4]) +
+                       B[4] * (B[3] * B[5] - B[1] * B[4]);
+    const double r = fmin(1.0, fmax(-1.0, det / 2.0));
+    const double lam = q + 2.0 * p * cos(acos(r) / 3.0 + 2.0943951023931957);   // smallest
+    const double r0[3] = {c[0] - lam, c[3], c[4]}, r1[3] = {c[3], c[1] - lam, c[5]},
+                 r2[3] = {c[4], c[5], c[2] - lam};
+    double x[3][3];
+    ftd::cross3d(r0, r1, x[0]);
+    ftd::cross3d(r0, r2, x[1]);
+    ftd::cross3d(r1, r2, x[2]);
+    int best = 0;
+    double bl = ftd::dot3d(x[0], x[0]);
+    for (int k = 1; k < 3; k++) {
+        const double lk = ftd::dot3d(x[k], x[k]);
+        if (lk > bl) { bl = lk; best = k; }
+    }
+    if (!(bl > 1e-24 * p2 * p2)) return false;
+    const double inv = 1.0 / sqrt(bl);
+    for (int k = 0; k < 3; k++) n[k] = x[best][k] * inv;
+    return true;
+}
+
+// The vertices a leaf's bound must hold: the triangles' three vertices, or
+// the crossing points themselves.
+template <int MODE>
+struct LeafVerts {
+    static constexpr int kPerPrim = MODE == 0 ? 1 : 3;
+    const int32_t* pidx;
+    const float* ifp;
+    const int4* tris;
+    // the vertices of primitive s into v[0..kPerPrim)
+    __device__ void get(uint32_t s, float v[3][3]) const
+    {
+        const int32_t j = pidx[s];
+        if (MODE == 0) {
+            for (int k = 0; k < 3; k++) v[0][k] = ifp[3 * (int64_t)j + k];
+        } else {
+            tri_vertices(tris, ifp, j, v[0], v[1], v[2]);
+        }
+    }
+};
+
+// Leaf bounds: box of the vertices and a cylinder about an axis: for
+// triangles the area-weighted mean normal, for crossing points the normal of
+// their least-squares plane (three or more points that are not collinear).
+template <int MODE>
+__global__ __launch_bounds__(kBS) void k_leaf_bounds(const uint64_t* sc, const uint32_t* leaf_start,
                                                      const uint32_t* leaf_cnt, const int32_t* pidx,
                                                      const float* ifp, const int4* tris, Bound* leaf_b)
 {
     if (halted(sc)) return;
     const int64_t l = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (l >= cnt_leaves(sc)) return;
+    using LV = LeafVerts<MODE>;
+    const LV lv{pidx, ifp, tris};
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    double ns[3] = {0.0, 0.0, 0.0};
+    double ns[3] = {0.0, 0.0, 0.0}, m[3] = {0.0, 0.0, 0.0};
     const uint32_t b = leaf_start[l], e = b + leaf_cnt[l];
     for (uint32_t s = b; s < e; s++) {
-        const int32_t j = pidx[s];
-        if (mode == 0) {
+        float v[3][3];
+        lv.get(s, v);
+        for (int q = 0; q < LV::kPerPrim; q++) {
             for (int k = 0; k < 3; k++) {
-                const float v = ifp[3 * (int64_t)j + k];
-                lo[k] = fminf(lo[k], v); hi[k] = fmaxf(hi[k], v);
+                lo[k] = fminf(lo[k], v[q][k]);
+                hi[k] = fmaxf(hi[k], v[q][k]);
+                if (MODE == 0) m[k] += v[q][k];
             }
-        } else {
-            float va[3], vb[3], vc[3];
-            tri_vertices(tris, ifp, j, va, vb, vc);
-            for (int k = 0; k < 3; k++) {
-                lo[k] = fminf(lo[k], fminf(va[k], fminf(vb[k], vc[k])));
-                hi[k] = fmaxf(hi[k], fmaxf(va[k], fmaxf(vb[k], vc[k])));
-            }
+        }
+        if (MODE == 1) {
             double e0[3], e1[3], nn[3];
-            for (int k = 0; k < 3; k++) { e0[k] = (double)vb[k] - va[k]; e1[k] = (double)vc[k] - va[k]; }
+            for (int k = 0; k < 3; k++) { e0[k] = (double)v[1][k] - v[0][k]; e1[k] = (double)v[2][k] - v[0][k]; }
             ftd::cross3d(e0, e1, nn);
             for (int k = 0; k < 3; k++) ns[k] += nn[k];
         }
@@ -756,21 +813,40 @@ __global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, const uint64_t* s
     Bound B;
     for (int k = 0; k < 3; k++) { B.lo[k] = lo[k]; B.hi[k] = hi[k]; B.c[k] = 0.5f * (lo[k] + hi[k]); }
     B.pad[0] = B.pad[1] = 0.0f;
-    const double nl = sqrt(ftd::dot3d(ns, ns));
-    if (mode == 0 || !(nl > 0.0)) {
+    double n[3];
+    bool axis = false;
+    if (MODE == 1) {
+        const double nl = sqrt(ftd::dot3d(ns, ns));
+        if (nl > 0.0) {
+            for (int k = 0; k < 3; k++) n[k] = ns[k] / nl;
+            axis = true;
+        }
+    } else if (e - b >= 3) {
+        double cov[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double cnt = (double)(e - b);
+        for (int k = 0; k < 3; k++) m[k] /= cnt;
+        for (uint32_t s = b; s < e; s++) {
+            float v[3][3];
+            lv.get(s, v);
+            const double d0 = v[0][0] - m[0], d1 = v[0][1] - m[1], d2 = v[0][2] - m[2];
+            cov[0] += d0 * d0; cov[1] += d1 * d1; cov[2] += d2 * d2;
+            cov[3] += d0 * d1; cov[4] += d0 * d2; cov[5] += d1 * d2;
+        }
+        axis = smallest_eigvec(cov, n);
+    }
+    if (!axis) {
         B.n[0] = 0.0f; B.n[1] = 0.0f; B.n[2] = 1.0f;
         B.rho = 0.0f; B.tau = -1.0f;                 // box only
     } else {
-        for (int k = 0; k < 3; k++) B.n[k] = (float)(ns[k] / nl);
-        const double n[3] = {B.n[0], B.n[1], B.n[2]};
+        for (int k = 0; k < 3; k++) { B.n[k] = (float)n[k]; n[k] = B.n[k]; }
         // The slab is centred on the vertices' extent along n, not on the box
         // centre: a surface patch can pass well off its box's centre, and a
         // slab centred off the patch is thicker by that offset.
         double hlo = INFINITY, hhi = -INFINITY;
         for (uint32_t s = b; s < e; s++) {
             float v[3][3];
-            tri_vertices(tris, ifp, pidx[s], v[0], v[1], v[2]);
-            for (int q = 0; q < 3; q++) {
+            lv.get(s, v);
+            for (int q = 0; q < LV::kPerPrim; q++) {
                 double d[3];
                 for (int k = 0; k < 3; k++) d[k] = (double)v[q][k] - B.c[k];
                 const double hv = ftd::dot3d(n, d);
@@ -783,8 +859,8 @@ __global__ __launch_bounds__(kBS) void k_leaf_bounds(int mode, const uint64_t* s
         double tau = 0.0, rho = 0.0;
         for (uint32_t s = b; s < e; s++) {
             float v[3][3];
-            tri_vertices(tris, ifp, pidx[s], v[0], v[1], v[2]);
-            for (int q = 0; q < 3; q++) {
+            lv.get(s, v);
+            for (int q = 0; q < LV::kPerPrim; q++) {
                 double d[3], cr[3];
                 for (int k = 0; k < 3; k++) d[k] = (double)v[q][k] - B.c[k];
                 ftd::cross3d(n, d, cr);
@@ -2996,7 +3072,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         hipLaunchKernelGGL(k_prim_scatter, dim3(grid1d(n_cap)), dim3(kBS), 0, s, mode, sc, ifp, tris,
                            c->pcode.as<uint32_t>(), bcnt, pre, c->prec.as<float4>(),
                            c->trec.as<TriRec>(), c->pidx.as<int32_t>());
-        hipLaunchKernelGGL(k_leaf_bounds, dim3(grid1d(L_cap)), dim3(kBS), 0, s, mode, sc,
+        hipLaunchKernelGGL(mode == 0 ? k_leaf_bounds<0> : k_leaf_bounds<1>, dim3(grid1d(L_cap)), dim3(kBS), 0, s, sc,
                            c->leaf_start.as<uint32_t>(), c->leaf_cnt.as<uint32_t>(),
                            c->pidx.as<int32_t>(), ifp, tris, c->leaf_b.as<Bound>());
     }
