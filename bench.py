@@ -496,7 +496,8 @@ def main():
             d_phi.copy_(d_phi0)
         bs, qb, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
                                       mode=args.mode, reps=3, sync=torch.cuda.synchronize, copy=copy,
-                                      bin_scales=(1.0, 1.25, 1.5, 2.0), query_blocks=(64, 256), band=band)
+                                      bin_scales=autotune.MODE_BIN_SCALES[args.mode], query_blocks=(64, 256),
+                                      band=band)
         args.bin_scale, args.query_block = bs, qb
         tuned = {"bin_scale": bs, "query_block": qb,
                  "table_ms": [[a, q, round(t, 3)] for a, q, t in table]}

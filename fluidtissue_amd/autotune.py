@@ -11,18 +11,22 @@ import time
 
 import numpy as np
 
-BIN_SCALES = (0.75, 1.0, 1.25, 1.5, 2.0, 3.0)
+# bin edges tried per mode: the neighbourhood of each mode's measured optimum
+# (points 2.0-2.5, triangles 1.25 node spacings on 32^3..256^3 fields)
+MODE_BIN_SCALES = {"points": (1.5, 2.0, 2.5, 3.0), "triangles": (1.0, 1.25, 1.5, 2.0)}
 QUERY_BLOCKS = (64, 128, 256)
 
 _cache = {}
 
 
 def tune(handle, phi_ptr, phi_backup_ptr, nbytes, mode="triangles", reps=2, sync=None, copy=None,
-         signature=None, bin_scales=BIN_SCALES, query_blocks=QUERY_BLOCKS, log=None, band=0.0):
+         signature=None, bin_scales=None, query_blocks=QUERY_BLOCKS, log=None, band=0.0):
     """handle: a Reinit with its batch set; phi_ptr: device phi (overwritten);
     phi_backup_ptr: device copy of the input, restored before every run by
     copy(dst, src, nbytes); sync(): waits for the device.  Returns
     (bin_scale, query_block, table of (bin_scale, query_block, ms))."""
+    if bin_scales is None:
+        bin_scales = MODE_BIN_SCALES[mode]
     key = (mode, band, signature)
     if signature is not None and key in _cache:
         return _cache[key]

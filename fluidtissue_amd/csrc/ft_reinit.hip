@@ -2772,11 +2772,18 @@ int sort_nodes(ft_ctx* c)
 // fixes them when it is captured; the device then raises S_CAP when a total
 // outgrows them, every later kernel does nothing, and the host captures again
 // with more room (phi is only written by the last kernel, so nothing is lost).
-// Per-field bin grids of the index for one bin_scale, on the device; cached
-// per field layout.
-int index_setup(ft_ctx* c, hipStream_t s, float bin_scale)
+// Default bin edge, in node spacings, per distance mode (0 points, 1
+// triangles).  A bin's primitives form one leaf.  Points are cheap to test and
+// their least-squares-plane cylinder tightens with more points, so their
+// optimum is larger: measured on 32^3..256^3 fields, points 2.0-2.5, triangles
+// 1.25.
+constexpr float kDefaultBinScale[2] = {2.0f, 1.25f};
+
+// Per-field bin grids of the index for one bin_scale (0: the mode's default),
+// on the device; cached per field layout.
+int index_setup(ft_ctx* c, hipStream_t s, float bin_scale, int mode)
 {
-    const float bs = bin_scale > 0.0f ? bin_scale : 1.0f;
+    const float bs = bin_scale > 0.0f ? bin_scale : kDefaultBinScale[mode];
     if (c->idx_layout == c->layout_gen && c->idx_bin_scale == bs) return FT_OK;
     const int nf = c->nf;
     const int fb = field_bits(nf);
@@ -3282,9 +3289,9 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     c->have_normals = false;
     ft_counts cnt{};
     cnt.bad_tet = -1;
-    int rc = index_setup(c, s, o->bin_scale);
     Caps cp;
     const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
+    int rc = index_setup(c, s, o->bin_scale, sp.mode);
     if (rc == FT_OK && sp.want_normals) rc = build_node_tets(c, s);
     // Speculative pass: the capacities of the last call on these fields, no
     // readback between stages (the device guards them, and halts before phi is
@@ -3303,7 +3310,7 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
         cp = Caps{};
         cnt = ft_counts{};
         cnt.bad_tet = -1;
-        rc = index_setup(c, s, o->bin_scale);
+        rc = index_setup(c, s, o->bin_scale, sp.mode);
         if (rc == FT_OK)
             rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cp, true, false,
                                   &cnt, c->timing);
@@ -3411,7 +3418,7 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
     GraphState& g = c->graph;
     ft_counts cnt{};
     cnt.bad_tet = -1;
-    FT_TRY(index_setup(c, s, o->bin_scale));
+    FT_TRY(index_setup(c, s, o->bin_scale, o->mode == FT_DIST_POINTS ? 0 : 1));
     const bool same = g.exec && g.phi == d_phi && g.phi_ifn == d_phi_ifn && same_opts(g.opts, *o) &&
                       g.layout == c->layout_gen;
     Caps cp = g.caps;
@@ -4290,7 +4297,7 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     sp.epi = EPI_RAW;
     sp.want_nearest = true;
     uint64_t* sc = h->scalars.as<uint64_t>();
-    FT_TRY(index_setup(h, s, 0.0f));
+    FT_TRY(index_setup(h, s, 0.0f, sp.mode));
     FT_TRY(ensure_stage(h, 0, cp, 0, true));
     FT_TRY(fill<uint64_t>(s, sc + S_BLK, 1, 0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_EDGES, 1, (uint64_t)n_ifp));

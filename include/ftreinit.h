@@ -70,7 +70,7 @@ typedef struct ft_opts {
                             band get +-band and are counted in n_clamped.
                             0: full domain                                      */
     float   bin_scale;   /* bin edge in units of the mean node spacing; 0 =
-                            default                                             */
+                            the mode's default (points 2.0, triangles 1.25)     */
     int32_t want_nearest;/* 1: keep the per-node nearest primitive index       */
     int32_t query_block; /* threads per block of the distance query: 64, 128,
                             256 (waves per workgroup 1, 2, 4); 0 = default     */
