@@ -147,6 +147,35 @@ __global__ __launch_bounds__(1024) void k_scan_top(TO* bsum, int64_t nb, TO* tot
     if (threadIdx.x == 0) *total = carry;
 }
 
+// K consecutive elements at a 16-byte aligned address, as 16-byte accesses.
+template <class T, int K>
+__device__ __forceinline__ void load_run16(const T* p, T (&v)[K])
+{
+    static_assert((sizeof(T) * K) % 16 == 0, "whole 16-byte words");
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) * K / 16); i++) {
+        const uint4 u = q[i];
+        memcpy(reinterpret_cast<char*>(v) + 16 * i, &u, 16);
+    }
+}
+
+template <class T, int K>
+__device__ __forceinline__ void store_run16(T* p, const T (&v)[K])
+{
+    static_assert((sizeof(T) * K) % 16 == 0, "whole 16-byte words");
+    uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) * K / 16); i++) {
+        uint4 u;
+        memcpy(&u, reinterpret_cast<const char*>(v) + 16 * i, 16);
+        q[i] = u;
+    }
+}
+
+// Each thread scans IPT consecutive elements: read and written as 16-byte
+// accesses when the run is whole and both arrays are 16-byte aligned (the
+// run starts at a multiple of IPT elements, at least 32 bytes).
 template <class TI, class TO, class Ld>
 __global__ __launch_bounds__(kBS) void k_scan_down(const TI* in, TO* out, int64_t n,
                                                    const TO* bsum, Ld ld)
@@ -154,21 +183,41 @@ __global__ __launch_bounds__(kBS) void k_scan_down(const TI* in, TO* out, int64_
     __shared__ TO lds[kBS / 64 + 1];
     constexpr int IPT = kScanTile / kBS;
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * IPT;
+    const bool vec = base + IPT <= n &&
+                     ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
     TO v[IPT];
     TO s = 0;
+    if (vec) {
+        TI r[IPT];
+        load_run16(in + base, r);
 #pragma unroll
-    for (int k = 0; k < IPT; k++) {
-        const int64_t i = base + k;
-        v[k] = i < n ? (TO)ld(in[i]) : TO(0);
-        s += v[k];
+        for (int k = 0; k < IPT; k++) v[k] = (TO)ld(r[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            const int64_t i = base + k;
+            v[k] = i < n ? (TO)ld(in[i]) : TO(0);
+        }
     }
+#pragma unroll
+    for (int k = 0; k < IPT; k++) s += v[k];
     TO tot;
     TO ex = ftd::block_excl_scan<TO, kBS>(s, lds, &tot) + bsum[blockIdx.x];
+    if (vec) {
+        TO o[IPT];
 #pragma unroll
-    for (int k = 0; k < IPT; k++) {
-        const int64_t i = base + k;
-        if (i < n) out[i] = ex;
-        ex += v[k];
+        for (int k = 0; k < IPT; k++) {
+            o[k] = ex;
+            ex += v[k];
+        }
+        store_run16(out + base, o);
+    } else {
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            const int64_t i = base + k;
+            if (i < n) out[i] = ex;
+            ex += v[k];
+        }
     }
 }
 
@@ -247,12 +296,13 @@ __global__ __launch_bounds__(kBS) void k_node_signs(const double* phi, int64_t n
 // negative nodes -> 4 crossing edges -> 2 triangles.  A workgroup handles
 // kClassifyGroups consecutive groups of kBS tets, one tet of each per
 // thread, so that each thread has that many tet loads in flight; per group
-// it writes the crossing bitmap words and blk[group] = (crossing tets << 32)
-// | triangles.
+// it writes the words of two bitmaps, crossing tets (tbits) and those with
+// two triangles (t2bits, a subset), and blk[group] = (crossing tets << 32) |
+// triangles.
 constexpr int kClassifyGroups = 2;
 __global__ __launch_bounds__(kBS) void k_classify(const int4* tets, int64_t T,
                                                   const uint64_t* sbits, uint64_t* tbits,
-                                                  uint64_t* blk)
+                                                  uint64_t* t2bits, uint64_t* blk)
 {
     __shared__ uint64_t part[kClassifyGroups][kBS / 64];
     int4 q[kClassifyGroups];
@@ -271,7 +321,10 @@ __global__ __launch_bounds__(kBS) void k_classify(const int4* tets, int64_t T,
         const bool cross = (t[g] < T) && sgn != 0 && sgn != 4;
         const uint64_t m = __ballot(cross);
         const uint64_t m2 = __ballot(cross && sgn == 2);
-        if (ftd::lane_id() == 0 && t[g] < T) tbits[t[g] >> 6] = m;
+        if (ftd::lane_id() == 0 && t[g] < T) {
+            tbits[t[g] >> 6] = m;
+            t2bits[t[g] >> 6] = m2;
+        }
         if (ftd::lane_id() == 0)
             part[g][threadIdx.x / 64] = ((uint64_t)__popcll(m) << 32) | (uint64_t)(__popcll(m) + __popcll(m2));
     }
@@ -288,24 +341,15 @@ __global__ __launch_bounds__(kBS) void k_classify(const int4* tets, int64_t T,
 
 // One thread per 64-tet word of the crossing bitmap.  Lanes 4k..4k+3 share a
 // k_classify block, whose exclusive offset is blk_prefix[w/4].
-__global__ __launch_bounds__(kBS) void k_compact(const int4* tets, const uint64_t* sbits,
-                                                 const uint64_t* tbits, int64_t nwords,
-                                                 const uint64_t* blk_prefix, int32_t* xtet,
-                                                 int32_t* xtri_off, const uint64_t* sc)
+__global__ __launch_bounds__(kBS) void k_compact(const uint64_t* tbits, const uint64_t* t2bits,
+                                                 int64_t nwords, const uint64_t* blk_prefix,
+                                                 int32_t* xtet, int32_t* xtri_off, const uint64_t* sc)
 {
     if (halted(sc)) return;
     const int64_t w = (int64_t)blockIdx.x * kBS + threadIdx.x;
     const uint64_t word = w < nwords ? tbits[w] : 0ull;
-    uint32_t ct = 0;
-    uint64_t m = word;
-    while (m) {
-        const int b = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        const int4 q = tets[w * 64 + b];
-        const int s = ftd::get_bit(sbits, q.x) + ftd::get_bit(sbits, q.y) +
-                      ftd::get_bit(sbits, q.z) + ftd::get_bit(sbits, q.w);
-        ct += (s == 2) ? 2u : 1u;
-    }
+    const uint64_t two = w < nwords ? t2bits[w] : 0ull;
+    const uint32_t ct = (uint32_t)(__popcll(word) + __popcll(two));
     const uint64_t mine = ((uint64_t)__popcll(word) << 32) | ct;
     uint64_t inc = mine;
 #pragma unroll
@@ -316,18 +360,14 @@ __global__ __launch_bounds__(kBS) void k_compact(const int4* tets, const uint64_
     if (w >= nwords || word == 0) return;
     const uint64_t base = blk_prefix[w >> 2] + (inc - mine);
     uint32_t bx = (uint32_t)(base >> 32), bt = (uint32_t)base;
-    m = word;
+    uint64_t m = word;
     while (m) {
         const int b = __ffsll((unsigned long long)m) - 1;
         m &= m - 1;
-        const int64_t t = w * 64 + b;
-        const int4 q = tets[t];
-        const int s = ftd::get_bit(sbits, q.x) + ftd::get_bit(sbits, q.y) +
-                      ftd::get_bit(sbits, q.z) + ftd::get_bit(sbits, q.w);
-        xtet[bx] = (int32_t)t;
+        xtet[bx] = (int32_t)(w * 64 + b);
         xtri_off[bx] = (int32_t)bt;
         bx++;
-        bt += (s == 2) ? 2u : 1u;
+        bt += 1u + (uint32_t)((two >> b) & 1ull);
     }
 }
 
@@ -396,9 +436,27 @@ __device__ __forceinline__ int32_t edge_rank(const uint64_t* ebits, const uint32
     return (int32_t)(eprefix[e >> 6] + (uint32_t)__popcll(below));
 }
 
-// Crossing points in increasing edge order (LevelSet.cpp:278-299): one thread
-// per 64-edge word.  FP64 interpolation, no fma; marks both edge nodes as
-// retained (LevelSet.cpp:443-447).
+// Position of the n-th (from 0) set bit of w; n < popcount(w).
+__device__ __forceinline__ int nth_set_bit(uint64_t w, uint32_t n)
+{
+    int pos = 0;
+#pragma unroll
+    for (int sh = 32; sh; sh >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(w & ((1ull << sh) - 1ull));
+        const bool skip = n >= c;
+        n -= skip ? c : 0u;
+        w = skip ? (w >> sh) : w;
+        pos += skip ? sh : 0;
+    }
+    return pos;
+}
+
+// Crossing points in increasing edge order (LevelSet.cpp:278-299).  A wave
+// takes 64 consecutive words of the edge bitmap and their crossing edges 64
+// at a time, one per lane: the lane finds the word holding its edge by a
+// binary search over the wave's prefix counts, then the bit.  FP64
+// interpolation, no fma; marks both edge nodes as retained
+// (LevelSet.cpp:443-447).
 __global__ __launch_bounds__(kBS) void k_points(const uint64_t* ebits, const uint32_t* eprefix,
                                                 int64_t nwords, const int2* edges,
                                                 const float* xyz, const double* phi,
@@ -406,14 +464,28 @@ __global__ __launch_bounds__(kBS) void k_points(const uint64_t* ebits, const uin
                                                 unsigned long long* retain, const uint64_t* sc)
 {
     if (halted(sc)) return;
+    const int lane = ftd::lane_id();
     const int64_t w = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (w >= nwords) return;
-    uint64_t m = ebits[w];
-    uint32_t r = eprefix[w];
-    while (m) {
-        const int b = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        const int64_t e = w * 64 + b;
+    const int64_t w0 = w - lane;                       // the wave's first word
+    if (w0 >= nwords) return;                          // whole wave out of range
+    const uint64_t word = w < nwords ? ebits[w] : 0ull;
+    const uint32_t cnt = (uint32_t)__popcll(word);
+    const uint32_t inc = ftd::wave_incl_scan(cnt);
+    const uint32_t total = __shfl(inc, 63);
+    const uint32_t r0 = eprefix[w0];                   // rank of the wave's first crossing edge
+    for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        int L = 0;                                     // lanes whose words end at or before k
+#pragma unroll
+        for (int step = 32; step; step >>= 1) {
+            const uint32_t v = __shfl(inc, L + step - 1);
+            L += (v <= k) ? step : 0;
+        }
+        const uint64_t wl = __shfl(word, L);
+        const uint32_t ex = __shfl(inc, L) - __shfl(cnt, L);
+        if (k >= total) continue;
+        const int64_t e = (w0 + L) * 64 + nth_set_bit(wl, k - ex);
+        const uint32_t r = r0 + k;
         const int2 ab = edges[e];
         const double pa = phi[ab.x], pb = phi[ab.y];
         double t = 0.5;
@@ -429,7 +501,6 @@ __global__ __launch_bounds__(kBS) void k_points(const uint64_t* ebits, const uin
         ifp2e[r] = (int32_t)e;
         atomicOr(&retain[ab.x >> 6], 1ull << (ab.x & 63));
         atomicOr(&retain[ab.y >> 6], 1ull << (ab.y & 63));
-        r++;
     }
 }
 
@@ -2834,7 +2905,7 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
     if (stage == 0) {
         FT_TRY(ensure(c->sbits, 8 * at_least1(nw_nodes)));
         FT_TRY(ensure(c->retain, 8 * at_least1(nw_nodes)));
-        FT_TRY(ensure(c->tbits, 8 * at_least1(nw_tets)));
+        FT_TRY(ensure(c->tbits, 8 * at_least1(2 * nw_tets)));   // crossing, then two-triangle
         FT_TRY(ensure(c->blk, 8 * at_least1(nblk)));
         FT_TRY(ensure(c->ebits, 8 * at_least1(nw_edges)));
         FT_TRY(ensure(c->eprefix, 4 * at_least1(nw_edges)));
@@ -2972,7 +3043,7 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
     if (T > 0)
         hipLaunchKernelGGL(k_classify, dim3((unsigned)((nblk + kClassifyGroups - 1) / kClassifyGroups)),
                            dim3(kBS), 0, s, c->tets.as<int4>(), T, sbits, c->tbits.as<uint64_t>(),
-                           c->blk.as<uint64_t>());
+                           c->tbits.as<uint64_t>() + nw_tets, c->blk.as<uint64_t>());
     FT_TRY(scan<uint64_t, uint64_t>(c, s, c->blk.as<uint64_t>(), c->blk.as<uint64_t>(), nblk,
                                     sc + S_BLK, LoadSame{}));
     if (eager) {
@@ -2986,8 +3057,8 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
     }
     FT_TRY(ensure_stage(c, 1, cp, mode, want_nearest));
     if (nw_tets > 0)
-        hipLaunchKernelGGL(k_compact, dim3(grid1d(nw_tets)), dim3(kBS), 0, s, c->tets.as<int4>(), sbits,
-                           c->tbits.as<uint64_t>(), nw_tets, c->blk.as<uint64_t>(),
+        hipLaunchKernelGGL(k_compact, dim3(grid1d(nw_tets)), dim3(kBS), 0, s, c->tbits.as<uint64_t>(),
+                           c->tbits.as<uint64_t>() + nw_tets, nw_tets, c->blk.as<uint64_t>(),
                            c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), sc);
     if (cp.X > 0)
         hipLaunchKernelGGL(k_xtet, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tets.as<int4>(),
