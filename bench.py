@@ -519,7 +519,7 @@ def main():
     P, F, X = cnt.n_ifp, cnt.n_tri, cnt.n_if_elems
     ph = [p / max(calls, 1) for p in phase_ms]
     log(f"step {ms:.3f} ms; phases: marching tets {ph[0]:.3f} ms, index {ph[1]:.3f} ms, "
-        f"query {ph[2]:.3f} ms, settle {ph[3]:.3f} ms")
+        f"query {ph[2]:.3f} ms, normals {ph[3]:.3f} ms")
 
     # Algorithmic bytes (per reinit): 28N + 16T + 20P + 24Tx + 16F; the query
     # kernel's share is 28N (coords 12 + phi_old 8 + phi_out 8).
@@ -581,7 +581,7 @@ def main():
                    "triangles": F, "crossing_tets": X, "h": h, "band": band,
                    "clamped_nodes": cnt.n_clamped},
         "phases_ms": {"marching_tets": ph[0], "index_build": ph[1], "query": ph[2],
-                      "settle": ph[3]},
+                      "normals": ph[3]},
         "roofline": {"kernel": f"k_query_wave<{1 if args.mode == 'triangles' else 0},"
                                f"{args.query_block or 256}> (nearest-primitive search)",
                      "bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",

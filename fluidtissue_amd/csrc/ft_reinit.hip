@@ -1191,7 +1191,6 @@ struct QueryArgs {
     int probe;                       // greedy first candidate
     const uint64_t* sc;              // stage totals and halt flags
     unsigned xcd_chunk;              // tiles per XCD chunk (0: dispatch order)
-    int2* near2;                     // mode 1: (nearest, close runner-up or -1) per node
 };
 
 // The dispatcher hands workgroup b to XCD b % 8, each XCD with its own L2.
@@ -1335,24 +1334,6 @@ __device__ __forceinline__ void node_epilogue(const QueryArgs& a, int64_t node, 
     a.phi_out[node] = out;
 }
 
-// End of the search of one node.  Points: the float distance, as the
-// reference, and the epilogue here.  Triangles: the FP32 search knows the
-// nearest triangle up to its rounding; the best and, when that close, the
-// runner-up go to k_settle, which evaluates them in FP64 (at 256^3 float
-// rounding alone nears 1e-5 h) and runs the epilogue.  Keeping the FP64 work
-// out of this kernel keeps its register count, hence its occupancy.
-template <int MODE>
-__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t node)
-{
-    if (MODE == 0) {
-        const float d = (a.band > 0.0f && S.bj < 0) ? a.band : S.best;
-        node_epilogue(a, node, (double)d, S.bj);
-    } else {
-        const bool close = S.sj >= 0 && S.sec <= S.best * (1.0f + 0x1.0p-16f);
-        a.near2[node] = make_int2(S.bj, close ? S.sj : -1);
-    }
-}
-
 // FP64 squared distance from point (x, y, z) to triangle f of the interface.
 __device__ inline double node_tri_dist2(const QueryArgs& a, float x, float y, float z, int32_t f)
 {
@@ -1366,28 +1347,34 @@ __device__ inline double node_tri_dist2(const QueryArgs& a, float x, float y, fl
     return ftd::tri_dist2_f64(p, va, vb, vc);
 }
 
-// Triangles: per node (node order), the FP64 distance to the nearest
-// triangle (the lower of the two candidates, ties to the lower id), then
-// the epilogue.  No candidate: +-band in a narrow band, else +-FLT_MAX.
-__global__ __launch_bounds__(kBS) void k_settle(QueryArgs a, const float* xyz, int64_t n)
+// End of the search of one node, then its epilogue.  Points: the float
+// distance, as the reference.  Triangles: the FP32 search knows the nearest
+// triangle up to its rounding (at 256^3 float rounding alone nears 1e-5 h),
+// so the best and, when that close, the runner-up are evaluated in FP64; the
+// lower wins, ties to the lower id.  No candidate: +-band in a narrow band,
+// else +-FLT_MAX.
+template <int MODE>
+__device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t node)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (i >= n || halted(a.sc)) return;
-    const int2 c = a.near2[i];
-    double d;
-    int32_t near = c.x;
-    if (c.x >= 0) {
-        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-        double d2 = node_tri_dist2(a, x, y, z, c.x);
-        if (c.y >= 0) {
-            const double e2 = node_tri_dist2(a, x, y, z, c.y);
-            if (e2 < d2 || (e2 == d2 && c.y < near)) { d2 = e2; near = c.y; }
-        }
-        d = sqrt(d2);
+    if (MODE == 0) {
+        const float d = (a.band > 0.0f && S.bj < 0) ? a.band : S.best;
+        node_epilogue(a, node, (double)d, S.bj);
     } else {
-        d = (double)(a.band > 0.0f ? a.band : FLT_MAX);
+        double d;
+        int32_t near = S.bj;
+        if (S.bj >= 0) {
+            double d2 = node_tri_dist2(a, S.x, S.y, S.z, S.bj);
+            const bool close = S.sj >= 0 && S.sec <= S.best * (1.0f + 0x1.0p-16f);
+            if (close) {
+                const double e2 = node_tri_dist2(a, S.x, S.y, S.z, S.sj);
+                if (e2 < d2 || (e2 == d2 && S.sj < near)) { d2 = e2; near = S.sj; }
+            }
+            d = sqrt(d2);
+        } else {
+            d = (double)(a.band > 0.0f ? a.band : FLT_MAX);
+        }
+        node_epilogue(a, node, d, near);
     }
-    node_epilogue(a, i, d, near);
 }
 
 template <int MODE>
@@ -1412,9 +1399,12 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
 // together.  A child is entered when any lane's bound admits it; each lane
 // keeps its own best.  All control flow is wave-uniform (ballots), node and
 // primitive records are read at wave-uniform addresses through scalar loads,
-// the stack is one per wave in LDS.
+// the stack is one per wave in LDS.  Launch bound: 8 waves per SIMD, hence at
+// most 64 VGPRs.  Unbounded, the FP64 settle of the epilogue takes 84 and the
+// occupancy drops to 5 (measured 25% slower at 256^3); bounded, it spills a
+// few registers that are touched once per node.
 template <int MODE, int BS>
-__global__ __launch_bounds__(BS) void k_query_wave(QueryArgs a)
+__global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
 {
     __shared__ int32_t wstack[BS / 64][kStack];
     if (halted(a.sc)) return;                         // an earlier stage failed: phi untouched
@@ -2457,7 +2447,7 @@ struct ft_ctx {
     DevBuf phi_tmp, phi_ifn_tmp;
     DevBuf sbits, retain, tbits, blk, xtet, xtri_off, xedges, ebits, eprefix;
     DevBuf ifp, ifp2e, tris, tri_norm, if_elems, p2t_off, p2t_cursor, p2t, ifp_area;
-    DevBuf nearest, near2, fcounts, fclamp;
+    DevBuf nearest, fcounts, fclamp;
     // normals: node -> tets lists (per field layout), tet gradients, node normals
     DevBuf n2t_off, n2t_ids, grad_t, normals, phi_ifn_int;
     DevBuf gsplit, kmesh, k_if;      // curvature: unit-normal components, node and point curvature
@@ -2516,7 +2506,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk,
             &h->xtet, &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp,
             &h->ifp2e, &h->tris, &h->tri_norm, &h->if_elems, &h->p2t_off,
-            &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->near2, &h->bin_count,
+            &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
             &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
             &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
@@ -2914,7 +2904,6 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
         FT_TRY(ensure(c->roots, sizeof(int32_t) * nf));
         FT_TRY(ensure(c->leaf_off, sizeof(int32_t) * (nf + 1)));
         if (want_nearest) FT_TRY(ensure(c->nearest, sizeof(int32_t) * at_least1(N)));
-        if (mode == 1) FT_TRY(ensure(c->near2, sizeof(int2) * at_least1(N)));
         if (want_normals) {
             FT_TRY(ensure(c->normals, 12 * at_least1(N)));
             FT_TRY(ensure(c->grad_t, 12 * at_least1(T)));
@@ -3210,7 +3199,6 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     q.sc = sc;
     q.xcd_chunk = kXcdChunk;
-    q.near2 = c->near2.as<int2>();
     if (timed) (void)hipEventRecord(c->ev[2], s);
     if (c->n_slots > 0) {
         const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
@@ -3227,10 +3215,6 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         FT_TRY(check_launch());
     }
     if (timed) (void)hipEventRecord(c->ev[3], s);
-    if (mode == 1 && c->N > 0) {
-        hipLaunchKernelGGL(k_settle, dim3(grid1d(c->N)), dim3(kBS), 0, s, q, c->xyz.as<float>(), c->N);
-        FT_TRY(check_launch());
-    }
     if (sp.want_normals && c->N > 0) {
         if (c->n2t_layout != c->layout_gen)
             return ft::set_error(FT_ERR_STATE, "node -> tets lists not built for this field layout");

@@ -300,7 +300,7 @@ class Reinit(_Handle):
         check(lib.ft_set_timing(self._h, 1 if on else 0))
 
     def timing(self):
-        """(marching tets, index build, query, settle) cumulative ms, number of calls."""
+        """(marching tets, index build, query, normals) cumulative ms, number of calls."""
         ms = (C.c_double * 4)()
         n = C.c_int64()
         check(lib.ft_get_timing(self._h, ms, C.byref(n)))

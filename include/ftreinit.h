@@ -298,9 +298,9 @@ int ft_interface_device(ft_handle h, const float** ifp, const int32_t** tris,
  * reference, whose only timing is tic/toc around each phase,
  * LevelSet.cpp:272-274,360-362,434-436).  on = 1 enables and clears it.
  * ft_get_timing: cumulative milliseconds of [0] marching tets, [1] bin
- * index build, [2] distance query kernel, [3] FP64 settle and epilogue of
- * triangle mode (0 in points mode, whose epilogue is in the query), over
- * `calls` reinits; phase_ms has 4 entries. */
+ * index build, [2] distance query kernel (with the FP64 settle of triangle
+ * mode and the epilogue), [3] level-set normals when requested (else about
+ * 0), over `calls` reinits; phase_ms has 4 entries. */
 int ft_set_timing(ft_handle h, int32_t on);
 int ft_get_timing(ft_handle h, double* phase_ms, int64_t* calls);
 
