@@ -910,35 +910,38 @@ __global__ __launch_bounds__(kBS) void k_leaf_bounds(const uint64_t* sc, const u
         B.rho = 0.0f; B.tau = -1.0f;                 // box only
     } else {
         for (int k = 0; k < 3; k++) { B.n[k] = (float)n[k]; n[k] = B.n[k]; }
-        // The slab is centred on the vertices' extent along n, not on the box
-        // centre: a surface patch can pass well off its box's centre, and a
-        // slab centred off the patch is thicker by that offset.
-        double hlo = INFINITY, hhi = -INFINITY;
-        for (uint32_t s = b; s < e; s++) {
-            float v[3][3];
-            lv.get(s, v);
-            for (int q = 0; q < LV::kPerPrim; q++) {
-                double d[3];
-                for (int k = 0; k < 3; k++) d[k] = (double)v[q][k] - B.c[k];
-                const double hv = ftd::dot3d(n, d);
-                hlo = fmin(hlo, hv);
-                hhi = fmax(hhi, hv);
-            }
-        }
-        const double mid = 0.5 * (hlo + hhi);
-        for (int k = 0; k < 3; k++) B.c[k] = (float)(B.c[k] + mid * n[k]);
-        double tau = 0.0, rho = 0.0;
+        // One pass over the vertices, relative to the box centre c: their
+        // extent [hlo, hhi] along n and their largest distance rho0 from the
+        // axis through c.  The slab is then centred on that extent, not on the
+        // box centre: a surface patch can pass well off its box's centre, and
+        // a slab centred off the patch is thicker by that offset.  Moving the
+        // centre along n leaves the axis distances as they are, up to the
+        // float rounding of the moved centre, which is added to rho.
+        double hlo = INFINITY, hhi = -INFINITY, rho0 = 0.0;
         for (uint32_t s = b; s < e; s++) {
             float v[3][3];
             lv.get(s, v);
             for (int q = 0; q < LV::kPerPrim; q++) {
                 double d[3], cr[3];
                 for (int k = 0; k < 3; k++) d[k] = (double)v[q][k] - B.c[k];
+                const double hv = ftd::dot3d(n, d);
+                hlo = fmin(hlo, hv);
+                hhi = fmax(hhi, hv);
                 ftd::cross3d(n, d, cr);
-                tau = fmax(tau, fabs(ftd::dot3d(n, d)));
-                rho = fmax(rho, sqrt(ftd::dot3d(cr, cr)));
+                rho0 = fmax(rho0, ftd::dot3d(cr, cr));
             }
         }
+        const double mid = 0.5 * (hlo + hhi);
+        double sh[3], cr[3];
+        for (int k = 0; k < 3; k++) {
+            const float ck = (float)(B.c[k] + mid * n[k]);
+            sh[k] = (double)ck - B.c[k];              // the centre's actual move
+            B.c[k] = ck;
+        }
+        const double m = ftd::dot3d(n, sh);
+        ftd::cross3d(n, sh, cr);
+        const double tau = fmax(hhi - m, m - hlo);
+        const double rho = sqrt(rho0) + sqrt(ftd::dot3d(cr, cr));
         B.tau = (float)(tau * (1.0 + 1e-6));
         B.rho = (float)(rho * (1.0 + 1e-6));
     }
