@@ -202,14 +202,24 @@ __device__ inline void cross3d(const double* u, const double* v, double* w)
 }
 
 // ---- FP64 point-triangle distance (the search's final word) -----------------
-// Squared distance from x to triangle abc: the plane distance when the
-// projection of x falls inside (edge functions n.(e x (x - v)) >= 0 for the
-// three edges), else the nearest of the three edge segments; nearly
-// collinear vertices (|n|^2 <= 1e-14 lmax^2) use the segments only.
-__device__ inline double seg_dist2_f64(const double* x, const double* a, const double* b)
+// Squared distance from x to triangle abc (float points, arithmetic in
+// FP64): the plane distance when the projection of x falls inside (edge
+// functions n.(e x (x - v)) >= 0 for the three edges), else the nearest of
+// the three edge segments; nearly collinear vertices (|n|^2 <= 1e-14 lmax^2)
+// use the segments only.  Each difference (x - v, an edge) is formed from the
+// float inputs where it is used, the same way each time, and each segment
+// distance is taken beside its edge function, so few doubles are live at
+// once: this runs at the end of the search kernel, whose occupancy depends on
+// its register count.
+__device__ inline void diff3d(const float* u, const float* v, double* w)
 {
-    double ab[3], ax[3];
-    for (int k = 0; k < 3; k++) { ab[k] = b[k] - a[k]; ax[k] = x[k] - a[k]; }
+    for (int k = 0; k < 3; k++) w[k] = (double)u[k] - (double)v[k];
+}
+
+// Squared distance from the point at ax (relative to a) to the segment a,
+// a + ab.
+__device__ inline double seg_dist2_rel(const double* ax, const double* ab)
+{
     const double l2 = dot3d(ab, ab);
     double t = l2 > 0.0 ? dot3d(ax, ab) / l2 : 0.0;
     t = fmin(fmax(t, 0.0), 1.0);
@@ -218,32 +228,33 @@ __device__ inline double seg_dist2_f64(const double* x, const double* a, const d
     return r;
 }
 
-__device__ inline double tri_dist2_f64(const double* x, const double* a, const double* b,
-                                       const double* c)
+__device__ inline double tri_dist2_f64(const float* x, const float* a, const float* b, const float* c)
 {
-    double e0[3], e1[3], e2[3], n[3];
-    for (int k = 0; k < 3; k++) { e0[k] = b[k] - a[k]; e1[k] = c[k] - b[k]; e2[k] = a[k] - c[k]; }
-    cross3d(e0, e1, n);
-    const double nn = dot3d(n, n);
-    const double lmax = fmax(dot3d(e0, e0), fmax(dot3d(e1, e1), dot3d(e2, e2)));
-    if (nn > 1e-14 * lmax * lmax) {
-        const double* v[3] = {a, b, c};
-        const double* e[3] = {e0, e1, e2};
-        bool inside = true;
-        for (int q = 0; q < 3; q++) {
-            double xv[3], t[3];
-            for (int k = 0; k < 3; k++) xv[k] = x[k] - v[q][k];
-            cross3d(e[q], xv, t);
-            inside = inside && dot3d(n, t) >= 0.0;
-        }
-        if (inside) {
-            double xa[3];
-            for (int k = 0; k < 3; k++) xa[k] = x[k] - a[k];
-            const double hgt = dot3d(n, xa);
-            return hgt * hgt / nn;
-        }
+    double n[3], nn, lmax;
+    {
+        double e0[3], e1[3], e2[3];
+        diff3d(b, a, e0);
+        diff3d(c, b, e1);
+        diff3d(a, c, e2);
+        cross3d(e0, e1, n);
+        nn = dot3d(n, n);
+        lmax = fmax(dot3d(e0, e0), fmax(dot3d(e1, e1), dot3d(e2, e2)));
     }
-    return fmin(seg_dist2_f64(x, a, b), fmin(seg_dist2_f64(x, b, c), seg_dist2_f64(x, c, a)));
+    bool inside = nn > 1e-14 * lmax * lmax;
+    const float* v[3] = {a, b, c};
+    const float* w[3] = {b, c, a};
+    double hgt = 0.0, sd[3];
+    for (int q = 0; q < 3; q++) {
+        double xv[3], e[3], t[3];
+        diff3d(x, v[q], xv);
+        diff3d(w[q], v[q], e);
+        if (q == 0) hgt = dot3d(n, xv);
+        cross3d(e, xv, t);
+        inside = inside && dot3d(n, t) >= 0.0;
+        sd[q] = seg_dist2_rel(xv, e);
+    }
+    if (inside) return hgt * hgt / nn;
+    return fmin(sd[0], fmin(sd[1], sd[2]));
 }
 
 // ---- FP32 point-triangle distance -------------------------------------------

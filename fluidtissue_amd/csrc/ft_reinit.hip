@@ -1341,7 +1341,8 @@ __device__ __forceinline__ void node_epilogue(const QueryArgs& a, int64_t node, 
 __device__ inline double node_tri_dist2(const QueryArgs& a, float x, float y, float z, int32_t f)
 {
     const int4 t = a.tris[f];
-    double p[3] = {x, y, z}, va[3], vb[3], vc[3];
+    const float p[3] = {x, y, z};
+    float va[3], vb[3], vc[3];
     for (int k = 0; k < 3; k++) {
         va[k] = a.ifp[3 * (int64_t)t.x + k];
         vb[k] = a.ifp[3 * (int64_t)t.y + k];
