@@ -290,16 +290,30 @@ __device__ __forceinline__ TriProj tri_project(float x, float y, float z, const 
     return p;
 }
 
-// Lower bound from the rectangle [0, B] x [0, Cy] of the triangle's frame,
+// Squared distance to the rectangle [0, B] x [0, Cy] of the triangle's frame,
 // which holds it: ab is the longest edge, so C projects inside [0, B], and
-// Cy >= 0 (0 for collinear vertices, a segment).  Tighter than the disk for
-// thin triangles and with no square root; the same projected coordinates and
-// the same relative slack as the disk bound.
-__device__ __forceinline__ float tri_rect_lb2(const TriProj& p, const TriRec& r)
+// Cy >= 0 (0 for collinear vertices, a segment).  Tighter than a disk about
+// the triangle for thin triangles, and with no square root.  As computed it
+// may exceed the exact value by its rounding, within a relative 2^-18: the
+// caller compares it with a bound raised by that much (rect_bound).
+__device__ __forceinline__ float tri_rect_d2(const TriProj& p, const TriRec& r)
 {
     const float dx = fmaxf(fmaxf(-p.X, p.X - r.F1.w), 0.0f);
     const float dy = fmaxf(fmaxf(-p.Y, p.Y - r.F3.w), 0.0f);
-    return __fmaf_rn(dx, dx, __fmaf_rn(dy, dy, p.Z * p.Z)) * (1.0f - 0x1.0p-18f);
+    return __fmaf_rn(dx, dx, __fmaf_rn(dy, dy, p.Z * p.Z));
+}
+
+// The bound to compare tri_rect_d2 with: b (1 + 2^-17), which after rounding
+// is still at least b / (1 - 2^-18), so that d2 (1 - 2^-18) <= b implies
+// d2 <= rect_bound(b).  Computed once per change of b, not per triangle.
+__device__ __forceinline__ float rect_bound(float b) { return b * (1.0f + 0x1.0p-17f); }
+
+// Larger of two non-negative floats (squared distances): ordered as their
+// bit patterns, so one integer max gives fmaxf's value without the NaN
+// canonicalisation fmaxf costs.
+__device__ __forceinline__ float max_nonneg(float a, float b)
+{
+    return __int_as_float(max(__float_as_int(a), __float_as_int(b)));
 }
 
 __device__ __forceinline__ float tri_dist2_from(const TriProj& p, const TriRec& r)

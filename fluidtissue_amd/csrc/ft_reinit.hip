@@ -1239,10 +1239,12 @@ struct Searcher {
 
     __device__ __forceinline__ float best2() const { return MODE == 0 ? best * best : best; }
     float bnd;       // prune_bound(best2()), refreshed whenever best changes
+    float bnd_r;     // mode 1: rect_bound(bnd), the triangle cull's bound
     __device__ __forceinline__ void set_best(float d)
     {
         best = d;
         bnd = prune_bound(best2());
+        bnd_r = ftd::rect_bound(bnd);
     }
 
     // Mode 0 keeps the nearest point (ties: lowest id, the reference's strict
@@ -1266,7 +1268,9 @@ struct Searcher {
         }
         best = nb ? d : best;
         bj = nb ? j : bj;
-        bnd = nb ? prune_bound(MODE == 0 ? d * d : d) : bnd;
+        const float nbnd = prune_bound(MODE == 0 ? d * d : d);
+        bnd = nb ? nbnd : bnd;
+        if (MODE == 1) bnd_r = nb ? ftd::rect_bound(nbnd) : bnd_r;
     }
 
     uint32_t n_leaves = 0, n_prims = 0, n_pairs = 0;   // statistics (FT_QUERY_STATS)
@@ -1279,7 +1283,7 @@ struct Searcher {
     __device__ __forceinline__ void test_tri(const TriRec& r, int32_t j)
     {
         const ftd::TriProj pj = ftd::tri_project(x, y, z, r);
-        if (__ballot(ftd::tri_rect_lb2(pj, r) <= bnd) == 0ull) return;
+        if (__ballot(ftd::tri_rect_d2(pj, r) <= bnd_r) == 0ull) return;
         consider(ftd::tri_dist2_from(pj, r), j);
     }
 
@@ -1460,8 +1464,8 @@ __global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
             // box bound first; the cylinder only where some lane's box admits it
             float ll = ftd::box_dist2(px, py, pz, nd.l.lo, nd.l.hi);
             float lr = ftd::box_dist2(px, py, pz, nd.r.lo, nd.r.hi);
-            if (__ballot(ll <= bnd)) ll = fmaxf(ll, ftd::cyl_dist2(px, py, pz, nd.l));
-            if (__ballot(lr <= bnd)) lr = fmaxf(lr, ftd::cyl_dist2(px, py, pz, nd.r));
+            if (__ballot(ll <= bnd)) ll = ftd::max_nonneg(ll, ftd::cyl_dist2(px, py, pz, nd.l));
+            if (__ballot(lr <= bnd)) lr = ftd::max_nonneg(lr, ftd::cyl_dist2(px, py, pz, nd.r));
             n_nodes++;
             const uint64_t mL = __ballot(ll <= bnd), mR = __ballot(lr <= bnd);
             const uint64_t any = mL | mR;
