@@ -280,10 +280,13 @@ def run_serving(args):
     multiples of 8; phi from the star and 64-sphere generators) served with
     continuous batching against serial plain calls, at each rate.
 
-    Each job's phi lives in pinned host memory (the simulation's own buffer)
-    and is reinitialised in place.  Serial plain: one reinit call per job, in
+    Each job's phi is the simulation's own buffer, reinitialised in place:
+    in pinned host memory (--phi-on host, the default), or in HBM
+    (--phi-on device: a simulation whose state stays on the GPU, as with the
+    device advection step).  Serial plain: one reinit call per job, in
     arrival order, on a handle holding that job's resident mesh, with the
-    same transfers."""
+    same transfers (none for device phi: the call works on the job's
+    buffer)."""
     import ctypes as C
     import torch
     import fluidtissue_amd as fa
@@ -302,7 +305,12 @@ def run_serving(args):
         fields[(n, 1)] = torch.from_numpy(fa.phi_field("spheres", xyz, seed=0))
     sizes = [n ** 3 for n in dims]
     offs = np.concatenate([[0], np.cumsum(sizes)])
-    store = torch.empty(int(offs[-1]), dtype=torch.float64, pin_memory=True)
+    on_dev = args.phi_on == "device"
+    dev = torch.device("cuda:0")
+    if on_dev:
+        store = torch.empty(int(offs[-1]), dtype=torch.float64, device=dev)
+    else:
+        store = torch.empty(int(offs[-1]), dtype=torch.float64, pin_memory=True)
     bufs = [store[int(offs[k]):int(offs[k + 1])] for k in range(args.jobs)]
 
     def refill():
@@ -311,8 +319,8 @@ def run_serving(args):
 
     refill()
     log(f"serving inputs: {args.jobs} jobs over {len(set(dims))} sizes, "
-        f"{store.numel() * 8 / 1e9:.1f} GB pinned ({time.perf_counter() - t0:.1f} s)")
-    dev = torch.device("cuda:0")
+        f"{store.numel() * 8 / 1e9:.1f} GB {'in HBM' if on_dev else 'pinned'} "
+        f"({time.perf_counter() - t0:.1f} s)")
     kw = dict(mode=args.mode, blend="init")
 
     # serial plain: a handle per size with its resident mesh, a device buffer per size
@@ -327,6 +335,10 @@ def run_serving(args):
 
     def serial_job(k):
         h, _, d = singles[dims[k]]
+        if on_dev:
+            h.reset_phi_batch_device(bufs[k].data_ptr(), None, stream=sstream.cuda_stream, **kw)
+            sstream.synchronize()
+            return
         with torch.cuda.stream(sstream):
             d.copy_(bufs[k], non_blocking=True)
         h.reset_phi_batch_device(d.data_ptr(), None, stream=sstream.cuda_stream, **kw)
@@ -425,8 +437,9 @@ def run_serving(args):
         "vs_baseline": None, "dtype": "f32 distance search; f64 phi and crossing points",
         "data": "synthetic",
         "config": {"workload": f"serving: {args.jobs} jobs, sizes log-uniform 32^3-192^3 (multiples "
-                               f"of 8), star/64-sphere phi, {args.mode} distance, phi in pinned host "
-                               f"memory reinitialised in place",
+                               f"of 8), star/64-sphere phi, {args.mode} distance, phi in "
+                               f"{'HBM' if on_dev else 'pinned host memory'} reinitialised in place",
+                   "phi_on": args.phi_on,
                    "rates_per_s": args.rates},
         "rates": results,
         "serial_call_ms_by_size": {str(n): t for n, t in call_ms.items()},
@@ -443,6 +456,8 @@ def main():
     ap.add_argument("--fields", type=int, default=512)
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 500.0])
+    ap.add_argument("--phi-on", choices=["host", "device"], default="host",
+                    help="serving: where each job's phi lives")
     ap.add_argument("--mode", default="triangles", choices=["triangles", "points"])
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--query-block", type=int, default=0)
