@@ -1204,7 +1204,10 @@ struct QueryArgs {
 // slower: the XCD holding the costliest region finished last).  A bijection
 // on [0, nb).
 constexpr unsigned kXcds = 8;
-constexpr unsigned kXcdChunk = 256;     // measured: 0 -> 22.9 ms, 64/256 -> 22.2 ms at 256^3
+// Workgroups per run, measured (256-thread workgroups, triangles): 512 against
+// 256 is 6% faster on a 128^3 field of 64 spheres and the same at 256^3; 1024
+// is 3% slower at 256^3.
+constexpr unsigned kXcdChunk = 512;
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned chunk)
 {
     if (chunk == 0) return b;
