@@ -184,12 +184,7 @@ __device__ __forceinline__ float cyl_dist2(float x, float y, float z, const Boun
     return __fmaf_rn(r, r, a * a);
 }
 
-__device__ __forceinline__ float bound_dist2(float x, float y, float z, const Bound& b)
-{
-    return fmaxf(box_dist2(x, y, z, b.lo, b.hi), cyl_dist2(x, y, z, b));
-}
-
-// ---- FP64 vector helpers (index build) --------------------------------------
+// ---- FP64 vector helpers (index build, FP64 settle) -------------------------
 __device__ inline double dot3d(const double* u, const double* v)
 {
     return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
@@ -338,11 +333,6 @@ __device__ __forceinline__ float tri_dist2_from(const TriProj& p, const TriRec& 
     ey = __fmaf_rn(-t, Cy, Y);
     d = fminf(d, __fmaf_rn(ey, ey, ex * ex));
     return __fmaf_rn(Z, Z, inside ? 0.0f : d);
-}
-
-__device__ __forceinline__ float tri_dist2_f32(float x, float y, float z, const TriRec& r)
-{
-    return tri_dist2_from(tri_project(x, y, z, r), r);
 }
 
 }  // namespace ftd
