@@ -111,6 +111,37 @@ def test_triangle_mode_full(rein, name):
     assert np.all(np.abs(ifn) <= np.abs(ifn_p) * (1 + 1e-6))
 
 
+def test_checkerboard_every_tet_crossing(rein):
+    """phi = +-1 by node parity: every Kuhn tet has two negative nodes (four
+    crossing edges, two triangles) and every axis edge and body diagonal
+    crosses, so the edge bitmap's words are dense and each wave deals out
+    many more crossing points than it has lanes."""
+    n = 20
+    m = fa.kuhn_mesh(n)
+    h = 1.0 / (n - 1)
+    ijk = np.rint(m.xyz.astype(np.float64) / h).astype(np.int64)
+    phi = np.where(ijk.sum(axis=1) % 2 == 0, 1.0, -1.0)
+    rein.set_mesh(m)
+    rein._mesh_name = "checkerboard_20"
+    out, ifn, cnt = rein.reset_phi(phi, mode="points", blend="init", want_nearest=True)
+    ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
+    tris, elems, area = oracle.reconstruct_interface(m.tets, m.t2e, m.edges, m.xyz, phi, ifp, ifp2e)
+    n_tets = len(m.tets)
+    print(f"checkerboard {n}^3: P={cnt.n_ifp} F={cnt.n_tri} Tx={cnt.n_if_elems} of {n_tets} tets",
+          flush=True)
+    assert cnt.n_if_elems == n_tets and cnt.n_tri == 2 * n_tets
+    assert_interface_bit_exact(rein.interface(),
+                               dict(ifp=ifp, ifp2e=ifp2e, tris=tris, if_elems=elems, ifp_area=area))
+    ref = oracle.reinit(m.xyz, m.tets, m.edges, m.t2e, phi, blend=0)
+    assert np.array_equal(ifn.view(np.uint64), ref["phi_ifn"].view(np.uint64))
+    assert np.array_equal(out.view(np.uint64), ref["phi"].view(np.uint64))
+    _, ifn_t, _ = rein.reset_phi(phi, mode="triangles")
+    nodes = np.random.default_rng(5).choice(len(m.xyz), 1000, replace=False)
+    d64, _ = oracle.triangle_distance(m.xyz, ifp, tris, nodes)
+    assert np.max(np.abs(np.abs(ifn_t[nodes]) - d64)) <= 1e-5 * h
+    assert np.array_equal(np.signbit(ifn_t), phi < 0.0)
+
+
 @pytest.mark.parametrize("name", ["star_64", "spheres_96"])
 def test_triangle_mode_sampled(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="triangles", want_nearest=True)
