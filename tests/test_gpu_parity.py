@@ -187,6 +187,25 @@ def test_full_size_workloads(name, band_h, n_sample):
     assert mism == 0
 
 
+def test_points_mode_full_size_sampled():
+    """Points mode (the reference's own distance, EuclideanDistance.cu:14-35)
+    on the 256^3 workload: a seeded node sample bit-exact against the
+    oracle's brute force over all 601k crossing points, nearest index
+    included (its leaves use the plane-fit cylinder bounds)."""
+    m, phi, h = workload("spheres_256")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    out, ifn, cnt = r.reset_phi(phi, mode="points", blend="init", want_nearest=True)
+    ifp = r.interface().ifp
+    near = r.nearest()
+    r.close()
+    nodes = np.random.default_rng(15).choice(m.n_nodes, 2000, replace=False)
+    ref, ref_near = oracle.reset_phi_points(m.xyz[nodes], ifp, phi[nodes], want_nearest=True)
+    print(f"spheres_256 points: P={cnt.n_ifp}, {len(nodes)} nodes checked", flush=True)
+    assert np.array_equal(ifn[nodes].view(np.uint64), ref.view(np.uint64))
+    assert np.array_equal(near[nodes], ref_near)
+
+
 def test_triangle_mode_band(rein):
     """Narrow band 6h: exact inside, clamped to +-band outside."""
     name = "star_64"
