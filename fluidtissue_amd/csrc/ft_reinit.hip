@@ -1935,10 +1935,18 @@ __device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const 
         }
         __syncthreads();
         if (lane == 0) {
+            // a tet's four nodes are four distinct columns: its four adds are
+            // independent, so they are issued together; tets stay in order
             const int nt = min(kFemLanes, na - r0);
             for (int t = 0; t < nt; t++) {
                 const uint32_t pos = fmap[a0 + r0 + t];
-                for (int j = 0; j < 4; j++) acc[(pos >> (8 * j)) & 0xffu] += qb[4 * t + j];
+                int c[4];
+                double s4[4];
+                for (int j = 0; j < 4; j++) {
+                    c[j] = (int)((pos >> (8 * j)) & 0xffu);
+                    s4[j] = acc[c[j]] + qb[4 * t + j];
+                }
+                for (int j = 0; j < 4; j++) acc[c[j]] = s4[j];
             }
         }
         __syncthreads();
