@@ -110,7 +110,7 @@ struct Grid {
     float lo[3];
     float inv_b;          // 1 / bin edge
     int dim[3];
-    int pad;
+    float h;              // the field's node spacing
 };
 
 __device__ __forceinline__ uint32_t grid_code(const Grid& g, float x, float y, float z)
