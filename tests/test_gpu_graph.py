@@ -76,6 +76,45 @@ def test_graph_replays_equal_eager(mode, blend):
     r.close()
 
 
+@pytest.mark.parametrize("mode", ["triangles", "points"])
+def test_graph_deep_tiles_first(mode):
+    """Fields with nodes deep inside the surface (a sphere of radius 0.4 on
+    40^3 and 33^3 grids: nodes down to ~15h inside), so the query's schedule
+    lists deep tiles first: the captured graph, replayed, equals the eager
+    reinit bit for bit, and the batch equals each field alone."""
+    import torch
+    dims = [40, 33]
+    r = fa.Reinit(0)
+    r.set_kuhn_batch(dims)
+    m = r.mesh()
+    no, _, _ = r.batch_info()
+    phi = np.concatenate([np.linalg.norm(m.xyz[no[k]:no[k + 1]].astype(np.float64) - 0.5, axis=1) - 0.4
+                          for k in range(len(dims))])
+    assert np.min(phi[:no[1]]) <= -10.0 / (dims[0] - 1)            # deep tiles exist
+    ref_out, ref_ifn, ref_cnt, ref_itf, ref_near = _eager(r, phi, mode, "init", want_nearest=True)
+    d = torch.empty(len(phi), dtype=torch.float64, device="cuda")
+    di = torch.empty_like(d)
+    src = torch.from_numpy(phi).cuda()
+    for rep in range(4):
+        d.copy_(src)
+        di.fill_(7.0)
+        cnts = r.reset_phi_graph_device(d.data_ptr(), di.data_ptr(), mode=mode, blend="init",
+                                        want_nearest=True)
+        torch.cuda.synchronize()
+        assert [(c.n_ifp, c.n_tri, c.n_if_elems) for c in cnts] == ref_cnt, rep
+        assert np.array_equal(d.cpu().numpy().view(np.uint64), ref_out.view(np.uint64)), rep
+        assert np.array_equal(di.cpu().numpy().view(np.uint64), ref_ifn.view(np.uint64)), rep
+        assert np.array_equal(r.nearest(), ref_near)
+    r.close()
+    for k, n in enumerate(dims):                                     # each field alone
+        one = fa.Reinit(0)
+        one.set_kuhn_batch([n])
+        assert np.array_equal(one.mesh().xyz, m.xyz[no[k]:no[k + 1]])
+        out1, ifn1, _, _, _ = _eager(one, phi[no[k]:no[k + 1]], mode, "init")
+        assert np.array_equal(out1.view(np.uint64), ref_out[no[k]:no[k + 1]].view(np.uint64))
+        one.close()
+
+
 def test_graph_regrows_and_matches_eager():
     """A replay whose interface outgrows the captured headroom is captured
     again and still gives the eager result; a smaller one reuses the graph."""
