@@ -392,8 +392,9 @@ __device__ __forceinline__ void flag_error(unsigned long long* err, int code, in
 
 // Crossing tets: collect crossing edges in t2e order (LevelSet.cpp:385-391),
 // check the count (3 or 4, and consistent with the node signs used for the
-// triangle offsets), mark them in the edge bitmap.
-__global__ __launch_bounds__(kBS) void k_xtet(const int4* tets, const int2* t2e3,
+// triangle offsets: 4 exactly for the tets of the two-triangle bitmap t2bits),
+// mark them in the edge bitmap.
+__global__ __launch_bounds__(kBS) void k_xtet(const uint64_t* t2bits, const int2* t2e3,
                                               const int2* edges, const uint64_t* sbits,
                                               const int32_t* xtet, uint64_t* sc,
                                               unsigned long long* ebits, int4* xedges)
@@ -415,10 +416,7 @@ __global__ __launch_bounds__(kBS) void k_xtet(const int4* tets, const int2* t2e3
             n++;
         }
     }
-    const int4 q = tets[t];
-    const int s = ftd::get_bit(sbits, q.x) + ftd::get_bit(sbits, q.y) + ftd::get_bit(sbits, q.z) +
-                  ftd::get_bit(sbits, q.w);
-    const int expect = (s == 2) ? 4 : 3;
+    const int expect = ftd::get_bit(t2bits, t) ? 4 : 3;
     if (n != expect) {
         flag_error(err, 1, t);
         xedges[x] = make_int4(-1, -1, -1, -1);
@@ -3139,7 +3137,7 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
                            c->tbits.as<uint64_t>() + nw_tets, nw_tets, c->blk.as<uint64_t>(),
                            c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), sc);
     if (cp.X > 0)
-        hipLaunchKernelGGL(k_xtet, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tets.as<int4>(),
+        hipLaunchKernelGGL(k_xtet, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tbits.as<uint64_t>() + nw_tets,
                            c->t2e.as<int2>(), c->edges.as<int2>(), sbits, c->xtet.as<int32_t>(), sc,
                            c->ebits.as<unsigned long long>(), c->xedges.as<int4>());
     FT_TRY(scan<uint64_t, uint32_t>(c, s, c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(),
