@@ -92,6 +92,14 @@ def test_graph_deep_tiles_first(mode):
                           for k in range(len(dims))])
     assert np.min(phi[:no[1]]) <= -10.0 / (dims[0] - 1)            # deep tiles exist
     ref_out, ref_ifn, ref_cnt, ref_itf, ref_near = _eager(r, phi, mode, "init", want_nearest=True)
+    if mode == "points":                       # every node written: against the oracle, bitwise
+        import oracle
+        for k in range(len(dims)):
+            sl = slice(no[k], no[k + 1])
+            n = dims[k]
+            mk = fa.kuhn_mesh(n)
+            o = oracle.reinit(mk.xyz, mk.tets, mk.edges, mk.t2e, phi[sl], blend=0)
+            assert np.array_equal(ref_ifn[sl].view(np.uint64), o["phi_ifn"].view(np.uint64)), k
     d = torch.empty(len(phi), dtype=torch.float64, device="cuda")
     di = torch.empty_like(d)
     src = torch.from_numpy(phi).cuda()
