@@ -1228,11 +1228,19 @@ __device__ __forceinline__ float prune_bound(float best2)
     return __fmaf_rn(best2, 1.0f + 0x1.0p-16f, 1.0e-30f);
 }
 
+// A candidate as one 64-bit key: the distance's bits above the primitive's
+// id.  Distances are non-negative, so their bit patterns order as their
+// values, and key order is the order (distance, then lower id); no candidate
+// is id -1, the largest id as unsigned.
+__device__ __forceinline__ uint64_t cand_key(float d, int32_t j)
+{
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
+}
+
 template <int MODE>
 struct Searcher {
     float x, y, z;
-    float best;      // mode 0: distance; mode 1: squared distance
-    int32_t bj;      // best primitive (original id)
+    uint64_t bkey;   // best: mode 0 distance, mode 1 squared distance; primitive (original id)
     const float4* prec;
     const TriRec* trec;
     const int32_t* pidx;
@@ -1241,12 +1249,18 @@ struct Searcher {
     const float* ifp;
     const int4* tris;
 
-    __device__ __forceinline__ float best2() const { return MODE == 0 ? best * best : best; }
+    __device__ __forceinline__ float best() const { return __uint_as_float((uint32_t)(bkey >> 32)); }
+    __device__ __forceinline__ int32_t bj() const { return (int32_t)(uint32_t)bkey; }
+    __device__ __forceinline__ float sec() const { return __uint_as_float((uint32_t)(skey >> 32)); }
+    __device__ __forceinline__ int32_t sj() const { return (int32_t)(uint32_t)skey; }
+    __device__ __forceinline__ float best2() const { return MODE == 0 ? best() * best() : best(); }
     float bnd;       // prune_bound(best2()), refreshed whenever best changes
     float bnd_r;     // mode 1: rect_bound(bnd), the triangle cull's bound
-    __device__ __forceinline__ void set_best(float d)
+    // no candidate yet, distance d (the band, or no limit)
+    __device__ __forceinline__ void init_best(float d)
     {
-        best = d;
+        bkey = cand_key(d, -1);
+        skey = bkey;
         bnd = prune_bound(best2());
         bnd_r = ftd::rect_bound(bnd);
     }
@@ -1257,29 +1271,27 @@ struct Searcher {
     // primitive's original id (wave-uniform).  Written with selects, not
     // branches: a lane-divergent branch costs exec-mask bookkeeping on every
     // call, which is most of the work here.
+    // d is finite and non-negative.  A primitive seen again has the same key
+    // as before, so it is neither a new best nor a new second.
     __device__ __forceinline__ void consider(float d, int32_t j)
     {
         // (bitwise & and |: short-circuit operators compile to branches)
-        const bool fin = d < INFINITY;
-        const bool nb = (d < best) | ((d == best) & fin & ((bj < 0) | (j < bj)));
+        const uint64_t key = cand_key(d, j);
+        const bool nb = key < bkey;
         if (MODE == 1) {
-            const bool demote = nb & (bj >= 0) & (j != bj);        // old best becomes second
-            const bool ns = !nb & (j != bj) & ((d < sec) | ((d == sec) & fin & ((sj < 0) | (j < sj))));
-            const float s2 = demote ? best : (ns ? d : sec);
-            const int32_t j2 = demote ? bj : (ns ? j : sj);
-            sec = s2;
-            sj = j2;
+            // the old best becomes second (no candidate yet: it equals the
+            // initial second); else a key between the two is the new second
+            const bool ns = !nb & (key != bkey) & (key < skey);
+            skey = nb ? bkey : (ns ? key : skey);
         }
-        best = nb ? d : best;
-        bj = nb ? j : bj;
+        bkey = nb ? key : bkey;
         const float nbnd = prune_bound(MODE == 0 ? d * d : d);
         bnd = nb ? nbnd : bnd;
         if (MODE == 1) bnd_r = nb ? ftd::rect_bound(nbnd) : bnd_r;
     }
 
     uint32_t n_leaves = 0, n_prims = 0, n_pairs = 0;   // statistics (FT_QUERY_STATS)
-    float sec;       // mode 1: second best squared distance and its triangle
-    int32_t sj = -1;
+    uint64_t skey;   // mode 1: second best squared distance and its triangle
 
     // One triangle for all lanes: a wave-uniform early out when the bounding
     // rectangle of its frame rules it out for every lane, else the exact FP32
@@ -1369,17 +1381,18 @@ template <int MODE>
 __device__ __forceinline__ void finish_node(const QueryArgs& a, const Searcher<MODE>& S, int64_t node)
 {
     if (MODE == 0) {
-        const float d = (a.band > 0.0f && S.bj < 0) ? a.band : S.best;
-        node_epilogue(a, node, (double)d, S.bj);
+        const float d = (a.band > 0.0f && S.bj() < 0) ? a.band : S.best();
+        node_epilogue(a, node, (double)d, S.bj());
     } else {
         double d;
-        int32_t near = S.bj;
-        if (S.bj >= 0) {
-            double d2 = node_tri_dist2(a, S.x, S.y, S.z, S.bj);
-            const bool close = S.sj >= 0 && S.sec <= S.best * (1.0f + 0x1.0p-16f);
+        int32_t near = S.bj();
+        if (near >= 0) {
+            double d2 = node_tri_dist2(a, S.x, S.y, S.z, near);
+            const int32_t sj = S.sj();
+            const bool close = sj >= 0 && S.sec() <= S.best() * (1.0f + 0x1.0p-16f);
             if (close) {
-                const double e2 = node_tri_dist2(a, S.x, S.y, S.z, S.sj);
-                if (e2 < d2 || (e2 == d2 && S.sj < near)) { d2 = e2; near = S.sj; }
+                const double e2 = node_tri_dist2(a, S.x, S.y, S.z, sj);
+                if (e2 < d2 || (e2 == d2 && sj < near)) { d2 = e2; near = sj; }
             }
             d = sqrt(d2);
         } else {
@@ -1394,7 +1407,7 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
                                              int64_t node, uint32_t n_nodes)
 {
     // buckets: distance < 2h, < 8h, < 32h, beyond; +4 when phi < 0
-    const float dd = MODE == 0 ? S.best : sqrtf(S.best);
+    const float dd = MODE == 0 ? S.best() : sqrtf(S.best());
     const float hh = a.h_est;
     int bkt = dd < 2 * hh ? 0 : dd < 8 * hh ? 1 : dd < 32 * hh ? 2 : 3;
     if (a.phi_old[node] < 0.0) bkt += 4;
@@ -1494,10 +1507,8 @@ __global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
     const int32_t root = __builtin_amdgcn_readfirstlane(ftd::cload(a.roots, field));
     Searcher<MODE> S;
     S.x = px; S.y = py; S.z = pz;
-    S.bj = -1;
-    if (MODE == 0) S.set_best(a.band > 0.0f ? a.band : FLT_MAX);
-    else S.set_best(a.band > 0.0f ? a.band * a.band : INFINITY);
-    S.sec = S.best;
+    if (MODE == 0) S.init_best(a.band > 0.0f ? a.band : FLT_MAX);
+    else S.init_best(a.band > 0.0f ? a.band * a.band : INFINITY);
     S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
     S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
     S.ifp = a.ifp; S.tris = a.tris;
@@ -1559,7 +1570,7 @@ __global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
         }
         if (budget <= 0) atomicOr(a.overflow, 2u);
     }
-    const bool clamped = node >= 0 && a.band > 0.0f && S.bj < 0;
+    const bool clamped = node >= 0 && a.band > 0.0f && S.bj() < 0;
     const uint64_t cm = __ballot(clamped);
     if ((threadIdx.x & 63) == 0 && cm) atomicAdd(&a.n_clamped[field], (unsigned long long)__popcll(cm));
     if (node < 0) return;
