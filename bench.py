@@ -402,7 +402,7 @@ def run_serving(args):
         return st
 
     results = {}
-    with Server(device=0, lanes=2, max_fields=64, budget_bytes=200e9, want_ifn=False, **kw) as srv:
+    with Server(device=0, lanes=args.lanes, max_fields=64, budget_bytes=200e9, want_ifn=False, **kw) as srv:
         warm = [srv.submit((n, n, n), fields[(n, 0)].clone()) for n in sorted(set(dims))]
         for j in warm:
             j.wait(600)
@@ -439,7 +439,7 @@ def run_serving(args):
         "config": {"workload": f"serving: {args.jobs} jobs, sizes log-uniform 32^3-192^3 (multiples "
                                f"of 8), star/64-sphere phi, {args.mode} distance, phi in "
                                f"{'HBM' if on_dev else 'pinned host memory'} reinitialised in place",
-                   "phi_on": args.phi_on,
+                   "phi_on": args.phi_on, "lanes": args.lanes,
                    "rates_per_s": args.rates},
         "rates": results,
         "serial_call_ms_by_size": {str(n): t for n, t in call_ms.items()},
@@ -456,6 +456,7 @@ def main():
     ap.add_argument("--fields", type=int, default=512)
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 500.0])
+    ap.add_argument("--lanes", type=int, default=2, help="serving: worker lanes (handles and streams)")
     ap.add_argument("--phi-on", choices=["host", "device"], default="host",
                     help="serving: where each job's phi lives")
     ap.add_argument("--mode", default="triangles", choices=["triangles", "points"])
