@@ -21,7 +21,10 @@
 //  10 index           uniform-bin counting sort of the primitives (points or
 //                     triangles), leaves = non-empty bins in Morton order,
 //                     binary radix tree over the leaves
-//  11 k_query         per node (Morton-sorted order): exact nearest primitive,
+//  11 schedule        64-node tiles deep inside the surface listed first
+//                     (k_tile_deep, scan, k_tile_list; full domain only)
+//  12 k_query         per node (Morton-sorted order): exact nearest primitive
+//                     (triangles: FP32 search, FP64 settle of the best two),
 //                     sign from phi < 0, retained values and the blend fused
 //                     in the epilogue (EuclideanDistance.cu:45-58,
 //                     LevelSet.cpp:438-448, 74, 148)
