@@ -2162,18 +2162,26 @@ constexpr int kBcgGrid = 2048;         // workgroups of the solver kernels (bloc
 #define BCG_ROWS(i) for (int64_t i = (int64_t)blockIdx.x * kBcgBS + threadIdx.x; i < N; \
                          i += (int64_t)gridDim.x * kBcgBS)
 
-// p = r + beta (p - w v), beta = (rho / rho_old) (alpha / w).
+// The preconditioned vectors y = P p and z = P s (P = invdiag) are formed
+// once per row and stored, so that each SpMV entry gathers one value instead
+// of forming them from two or three gathers; each is the same product as
+// when formed where read.
+
+// p = r + beta (p - w v), beta = (rho / rho_old) (alpha / w); y = invdiag p.
 __global__ __launch_bounds__(kBcgBS) void k_bcg_p(int64_t N, const BcgScalars* sc, const double* r,
-                                                  const double* v, double* p)
+                                                  const double* v, const double* invdiag, double* p, double* y)
 {
     const double beta = (sc->rho / sc->rho_old) * (sc->alpha / sc->w), w = sc->w;
-    BCG_ROWS(i) p[i] = r[i] + beta * (p[i] - w * v[i]);
+    BCG_ROWS(i) {
+        const double pi = r[i] + beta * (p[i] - w * v[i]);
+        p[i] = pi;
+        y[i] = invdiag[i] * pi;
+    }
 }
 
-// v = L y with y = invdiag p (y = P p, then v = A y); partial r0.v.
+// v = L y (y = P p, then v = A y); partial r0.v.
 __global__ __launch_bounds__(kBcgBS) void k_bcg_v(int64_t N, FemEll e, const double* lvals,
-                                                  const double* invdiag, const double* p, const double* r0,
-                                                  double* v, double* part)
+                                                  const double* y, const double* r0, double* v, double* part)
 {
     double d = 0.0;
     BCG_ROWS(i) {
@@ -2181,8 +2189,7 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_v(int64_t N, FemEll e, const dou
         double t = 0.0;
         for (int k = 0; k < len; k++) {
             const int64_t at = ell_at(e, i, k);
-            const int32_t c = e.cols[at];
-            t += lvals[at] * (invdiag[c] * p[c]);
+            t += lvals[at] * y[e.cols[at]];
         }
         v[i] = t;
         d += r0[i] * t;
@@ -2190,42 +2197,47 @@ __global__ __launch_bounds__(kBcgBS) void k_bcg_v(int64_t N, FemEll e, const dou
     block_sum2<kBcgBS>(d, 0.0, part);
 }
 
-// t = L z with z = invdiag s, s = r - alpha v (formed where read); partials
-// t.t and t.s.
-__global__ __launch_bounds__(kBcgBS) void k_bcg_t(int64_t N, FemEll e, const double* lvals,
-                                                  const double* invdiag, const double* r, const double* v,
-                                                  const BcgScalars* sc, double* t, double* part)
+// s = r - alpha v; z = invdiag s.
+__global__ __launch_bounds__(kBcgBS) void k_bcg_s(int64_t N, const BcgScalars* sc, const double* invdiag,
+                                                  const double* r, const double* v, double* s_, double* z)
 {
     const double alpha = sc->alpha;
+    BCG_ROWS(i) {
+        const double si = r[i] - alpha * v[i];
+        s_[i] = si;
+        z[i] = invdiag[i] * si;
+    }
+}
+
+// t = L z; partials t.t and t.s.
+__global__ __launch_bounds__(kBcgBS) void k_bcg_t(int64_t N, FemEll e, const double* lvals,
+                                                  const double* z, const double* s_, double* t, double* part)
+{
     double tt = 0.0, ts = 0.0;
     BCG_ROWS(i) {
         const int len = e.len[i];
         double acc = 0.0;
         for (int k = 0; k < len; k++) {
             const int64_t at = ell_at(e, i, k);
-            const int32_t c = e.cols[at];
-            acc += lvals[at] * (invdiag[c] * (r[c] - alpha * v[c]));
+            acc += lvals[at] * z[e.cols[at]];
         }
         t[i] = acc;
         tt += acc * acc;
-        ts += acc * (r[i] - alpha * v[i]);
+        ts += acc * s_[i];
     }
     block_sum2<kBcgBS>(tt, ts, part);
 }
 
 // x += alpha y + w z;  r = s - w t;  partials r.r and r0.r.
-__global__ __launch_bounds__(kBcgBS) void k_bcg_x(int64_t N, const double* invdiag, const double* p,
-                                                  const double* v, const double* t, const double* r0,
+__global__ __launch_bounds__(kBcgBS) void k_bcg_x(int64_t N, const double* y, const double* s_,
+                                                  const double* z, const double* t, const double* r0,
                                                   const BcgScalars* sc, double* x, double* r, double* part)
 {
     const double alpha = sc->alpha, w = sc->w;
     double rr = 0.0, r0r = 0.0;
     BCG_ROWS(i) {
-        const double y = invdiag[i] * p[i];
-        const double si = r[i] - alpha * v[i];
-        const double z = invdiag[i] * si;
-        x[i] = x[i] + (alpha * y + w * z);
-        const double ri = si - w * t[i];
+        x[i] = x[i] + (alpha * y[i] + w * z[i]);
+        const double ri = s_[i] - w * t[i];
         r[i] = ri;
         rr += ri * ri;
         r0r += r0[i] * ri;
@@ -4310,10 +4322,10 @@ int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol
     if (std::sqrt(hs->tt) < FLT_MIN) { info->skipped = 1; info->converged = 1; return FT_OK; }
     if (hs->ts > 0.0) return ft::set_error(FT_ERR_ARG, "the velocity has NaN (LevelSet.cpp:112-115)");
     FT_TRY(fem_setup(h, s));
-    FT_TRY(ensure(h->bcg_vec, 8 * 8 * (size_t)N));
+    FT_TRY(ensure(h->bcg_vec, 11 * 8 * (size_t)N));
     double* vec = h->bcg_vec.as<double>();
     double *x = vec, *r = vec + N, *r0 = vec + 2 * N, *p = vec + 3 * N, *v = vec + 4 * N, *t = vec + 5 * N,
-           *invdiag = vec + 6 * N, *b = vec + 7 * N;
+           *invdiag = vec + 6 * N, *b = vec + 7 * N, *y = vec + 8 * N, *sv = vec + 9 * N, *z = vec + 10 * N;
     const FemEll e = fem_ell(h);
     double* lv = h->fem_l.as<double>();
     FT_TRY(ensure(h->adv_vel3, 24 * (size_t)N));
@@ -4344,12 +4356,13 @@ int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol
             FT_TRY(bcg_fin(h, s, nb, BCG_RESTART, false));
             if (restarts++ == 0) it = 0;
         }
-        hipLaunchKernelGGL(k_bcg_p, g, bs, 0, s, N, sc, r, v, p);
-        hipLaunchKernelGGL(k_bcg_v, g, bs, 0, s, N, e, lv, invdiag, p, r0, v, part);
+        hipLaunchKernelGGL(k_bcg_p, g, bs, 0, s, N, sc, r, v, invdiag, p, y);
+        hipLaunchKernelGGL(k_bcg_v, g, bs, 0, s, N, e, lv, y, r0, v, part);
         FT_TRY(bcg_fin(h, s, nb, BCG_ALPHA, false));
-        hipLaunchKernelGGL(k_bcg_t, g, bs, 0, s, N, e, lv, invdiag, r, v, sc, t, part);
+        hipLaunchKernelGGL(k_bcg_s, g, bs, 0, s, N, sc, invdiag, r, v, sv, z);
+        hipLaunchKernelGGL(k_bcg_t, g, bs, 0, s, N, e, lv, z, sv, t, part);
         FT_TRY(bcg_fin(h, s, nb, BCG_W, false));
-        hipLaunchKernelGGL(k_bcg_x, g, bs, 0, s, N, invdiag, p, v, t, r0, sc, x, r, part);
+        hipLaunchKernelGGL(k_bcg_x, g, bs, 0, s, N, y, sv, z, t, r0, sc, x, r, part);
         FT_TRY(bcg_fin(h, s, nb, BCG_END, true));
         it++;
     }
