@@ -1208,10 +1208,11 @@ struct QueryArgs {
 // slower: the XCD holding the costliest region finished last).  A bijection
 // on [0, nb).
 constexpr unsigned kXcds = 8;
-// Workgroups per run, measured (256-thread workgroups, triangles): 512 against
-// 256 is 6% faster on a 128^3 field of 64 spheres and the same at 256^3; 1024
-// is 3% slower at 256^3.
+// Workgroups per run, measured at 256^3 (triangles, deep tiles first): with
+// 256-thread workgroups 512 (2048 tiles; 1024 is 4% slower), with 64-thread
+// workgroups 1024 (1024 tiles; 512 is 1.3% slower, 2048 0.5%).
 constexpr unsigned kXcdChunk = 512;
+constexpr unsigned kXcdChunk64 = 1024;
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned chunk)
 {
     if (chunk == 0) return b;
@@ -3331,6 +3332,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     }
     if (c->n_slots > 0) {
         const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
+        q.xcd_chunk = qb == 64 ? kXcdChunk64 : kXcdChunk;
         // with a tile list: up to 7 padding workgroups after the deep ones,
         // one more for the split of the remainder
         const dim3 g(grid1d(c->n_slots, qb) + (q.tile_list ? 8u : 0u)), b(qb);
