@@ -98,3 +98,22 @@ def test_autotune_picks_fastest_and_caches():
     finally:
         autotune.time.perf_counter = orig
         autotune._cache.pop(("triangles", 0.0, ("test", 1)), None)
+
+
+def test_autotune_default_candidates_follow_the_mode():
+    """Without bin_scales the tuner tries the neighbourhood of each mode's
+    measured optimum (points 2.0-2.5, triangles 1.25 node spacings)."""
+    from fluidtissue_amd import autotune
+    tried = []
+
+    class FakeHandle:
+        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block):
+            tried.append((mode, bin_scale))
+
+    for mode in ("points", "triangles"):
+        autotune.tune(FakeHandle(), 1, 2, 3, mode=mode, reps=1, sync=lambda: None,
+                      copy=lambda d, s, n: None, query_blocks=(64,))
+    pts = sorted({b for m, b in tried if m == "points"})
+    tri = sorted({b for m, b in tried if m == "triangles"})
+    assert pts == sorted(autotune.MODE_BIN_SCALES["points"]) and 2.0 in pts and 2.5 in pts
+    assert tri == sorted(autotune.MODE_BIN_SCALES["triangles"]) and 1.25 in tri
