@@ -361,10 +361,11 @@ def run_serving(args):
     refill()
 
     def run_serial(arrivals):
+        count = len(arrivals)
         start = time.perf_counter()
         lat = []
         t_end = start
-        for k in range(args.jobs):
+        for k in range(count):
             ta = start + arrivals[k]
             now = time.perf_counter()
             if ta > now:
@@ -374,19 +375,19 @@ def run_serving(args):
             lat.append(t_end - ta)
         lat = np.array(lat)
         return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                    mean_ms=float(lat.mean() * 1e3), jobs_per_s=args.jobs / (t_end - start),
-                    Mnodes_per_s=sum(sizes) / (t_end - start) / 1e6, mean_batch=1.0)
+                    mean_ms=float(lat.mean() * 1e3), jobs=count, jobs_per_s=count / (t_end - start),
+                    Mnodes_per_s=sum(sizes[:count]) / (t_end - start) / 1e6, mean_batch=1.0)
 
-    def run_served(srv, arrivals):
+    def run_served(srv, arrivals, count):
         srv.batches.clear()
         start = time.perf_counter()
         jobs = []
         if arrivals is None:                          # saturated: the whole stream at once
             with srv.cv:
                 jobs = [srv.submit((n, n, n), bufs[k], t_arrival=start, inplace=True)
-                        for k, n in enumerate(dims)]
+                        for k, n in enumerate(dims[:count])]
         else:
-            for k in range(args.jobs):
+            for k in range(count):
                 ta = start + arrivals[k]
                 now = time.perf_counter()
                 if ta > now:
@@ -397,7 +398,8 @@ def run_serving(args):
             j.wait(1200)
         end = max(j.t_done for j in jobs)
         st = latency_stats(jobs)
-        st.update(jobs_per_s=args.jobs / (end - start), Mnodes_per_s=sum(sizes) / (end - start) / 1e6,
+        st.update(jobs=count, jobs_per_s=count / (end - start),
+                  Mnodes_per_s=sum(sizes[:count]) / (end - start) / 1e6,
                   mean_batch=float(np.mean([b[0] for b in srv.batches])))
         return st
 
@@ -407,10 +409,11 @@ def run_serving(args):
         for j in warm:
             j.wait(600)
         for rate in args.rates:
-            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=args.jobs))
+            count = min(args.jobs, getattr(args, "jobs_at_rate", {}).get(rate, args.jobs))
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
             res = {}
             refill()
-            res["served"] = run_served(srv, arrivals)
+            res["served"] = run_served(srv, arrivals, count)
             log(f"rate {rate}/s served: {res['served']}")
             refill()
             res["serial_plain"] = run_serial(arrivals)
@@ -420,7 +423,7 @@ def run_serving(args):
         # saturated capacity: the whole stream queued at once
         sat = {}
         refill()
-        sat["served"] = run_served(srv, None)
+        sat["served"] = run_served(srv, None, args.jobs)
         log(f"saturated served: {sat['served']}")
         refill()
         sat["serial_plain"] = run_serial(np.zeros(args.jobs))
@@ -446,6 +449,224 @@ def run_serving(args):
     }
 
 
+def host_info():
+    """The host's CPU as nproc / lscpu report it (the CPU baseline's threads)."""
+    import subprocess
+    info = {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket",
+                             "Socket(s)"):
+                info["lscpu " + k.strip()] = v.strip()
+    except Exception as e:                      # noqa: BLE001 (informational only)
+        info["lscpu"] = f"unavailable: {e}"
+    share = info["nproc"]
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            info["cgroup cpu.max"] = f.read().strip()
+        quota, period = info["cgroup cpu.max"].split()[:2]
+        if quota != "max":
+            share = min(share, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        info["OMP_NUM_THREADS"] = int(os.environ["OMP_NUM_THREADS"])
+        share = min(share, info["OMP_NUM_THREADS"])
+    # the threads the CPU baseline runs on: every hardware thread this
+    # process may use (LevelSet.hpp:38-39 takes hardware_concurrency()),
+    # within the affinity mask, the cgroup CPU quota and OMP_NUM_THREADS
+    info["threads_used"] = share
+    return info
+
+
+def accuracy_check(r, m, phi, h, mode, band, bin_scale, query_block, stream, n_sample=2000, seed=3):
+    """One more reinit (untimed) with the nearest triangle kept: the interface
+    against the oracle's marching tets (bit-exact) and a seeded node sample's
+    distances against the oracle's FP64 brute force over all triangles."""
+    import torch
+    import oracle
+    t0 = time.perf_counter()
+    d = torch.from_numpy(phi).cuda()
+    r.reset_phi_device(d.data_ptr(), None, mode=mode, blend="init", band=band, bin_scale=bin_scale,
+                       query_block=query_block, want_nearest=True, stream=stream)
+    torch.cuda.synchronize()
+    ifn = d.cpu().numpy()
+    near = r.nearest()
+    itf = r.interface()
+    ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
+    tris, elems, area = oracle.reconstruct_interface(m.tets, m.t2e, m.edges, m.xyz, phi, ifp, ifp2e)
+    exact = bool(np.array_equal(itf.ifp.view(np.uint32), ifp.view(np.uint32)) and
+                 np.array_equal(itf.ifp2e, ifp2e) and np.array_equal(itf.tris, tris) and
+                 np.array_equal(itf.if_elems, elems) and
+                 np.array_equal(itf.ifp_area.view(np.uint32), area.view(np.uint32)))
+    rng = np.random.default_rng(seed)
+    nodes = rng.choice(m.n_nodes, n_sample, replace=False)
+    got = np.abs(ifn[nodes])
+    signs = bool(np.array_equal(np.signbit(ifn), phi < 0.0))
+    if mode == "points":
+        d64, j64 = oracle.point_distance(m.xyz, ifp, nodes)
+        d64 = d64.astype(np.float64)
+        prims = ifp
+    else:
+        d64, j64 = oracle.triangle_distance(m.xyz, ifp, tris, nodes)
+        prims = tris
+    inb = np.ones(len(nodes), bool)
+    clamped_ok = True
+    if band > 0:
+        outb = d64 > band * (1 + 1e-6)
+        clamped_ok = bool(np.all(near[nodes][outb] == -1) and
+                          np.all(got[outb] == np.float64(np.float32(band))))
+        inb = d64 < band * (1 - 1e-6)
+    err = np.abs(got[inb] - d64[inb])
+    ties = mism = 0
+    eps = 1e-5 * h
+    for k in np.nonzero(inb & (near[nodes] != j64))[0]:
+        if mode == "points":
+            dg = float(np.sqrt(np.sum((m.xyz[nodes[k]].astype(np.float64) -
+                                       ifp[near[nodes[k]]].astype(np.float64)) ** 2)))
+        else:
+            dg = oracle.node_triangle_distance(nodes[k], m.xyz, ifp, prims, near[nodes[k]])
+        if abs(dg - d64[k]) <= eps:
+            ties += 1
+        else:
+            mism += 1
+    out = {"nodes_checked": int(inb.sum()), "sample": n_sample, "seed": seed,
+           "max_err": float(err.max()) if err.size else 0.0,
+           "max_err_h": float(err.max() / h) if err.size else 0.0,
+           "rms_err_h": float(np.sqrt((err ** 2).mean()) / h) if err.size else 0.0,
+           "tolerance_h": 1e-5, "eps_ties": ties, "index_mismatches": mism,
+           "signs_bit_exact_all_nodes": signs, "interface_bit_exact": exact,
+           "reference": "FP64 brute force over the oracle's interface (all primitives)",
+           "seconds": round(time.perf_counter() - t0, 1)}
+    if band > 0:
+        out["outside_band_clamped_exactly"] = clamped_ok
+    return out
+
+
+def _read_counters(dirpath):
+    """{(kernel, dispatch): {counter: value}} from a rocprofv3 --pmc csv run."""
+    import csv
+    import glob
+    rows = {}
+    for fn in glob.glob(os.path.join(dirpath, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                key = (row.get("Kernel_Name", ""), row.get("Dispatch_Id", ""))
+                d = rows.setdefault(key, {})
+                name = row.get("Counter_Name", "")
+                d[name] = d.get(name, 0.0) + float(row.get("Counter_Value", 0.0))
+    return rows
+
+
+def live_counters(args, N, q_ms, out_root):
+    """rocprofv3 counter passes of one reinit of this workload (a child run of
+    this script per pass, `--profile-child`): FETCH_SIZE, WRITE_SIZE and the
+    instruction counts of the query kernel, scaled by what the same counters
+    read for k_node_signs, whose traffic is known (it reads 8 B of phi per
+    node and writes 1 bit per node)."""
+    import shutil
+    import subprocess
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return {"status": "rocprofv3 not found"}
+    child = [sys.executable, os.path.abspath(__file__), "--profile-child", "--workload", args.workload,
+             "--mode", args.mode, "--bin-scale", str(args.bin_scale), "--query-block", str(args.query_block)]
+    passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"],
+              "sq": ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_SMEM"]}
+    env = dict(os.environ, TMPDIR="/tmp")
+    got = {}
+    cmds = []
+    for name, ctrs in passes.items():
+        d = os.path.join(out_root, name)
+        shutil.rmtree(d, ignore_errors=True)
+        cmd = [prof, "--pmc"] + ctrs + ["--output-format", "csv", "-d", d, "-o", name, "--"] + child
+        cmds.append(" ".join(["rocprofv3", "--pmc"] + ctrs + ["--output-format", "csv", "-d", f"<out>/{name}",
+                                                             "-o", name, "--", "python3", "bench.py"] + child[2:]))
+        t0 = time.perf_counter()
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180,
+                           cwd="/tmp", env=env)
+        except subprocess.TimeoutExpired:
+            return {"status": f"counter pass {name} timed out"}
+        got[name] = _read_counters(d)
+        log(f"counter pass {name}: {len(got[name])} dispatches ({time.perf_counter() - t0:.1f} s)")
+
+    def last(rows, kname, ctr):
+        ks = [k for k in rows if kname in k[0] and ctr in rows[k]]
+        if not ks:
+            return None
+        k = max(ks, key=lambda k: int(k[1]) if k[1].isdigit() else 0)
+        return rows[k][ctr]
+
+    qk = "k_query_wave"
+    fetch_raw = last(got["fetch"], qk, "FETCH_SIZE")
+    write_raw = last(got["write"], qk, "WRITE_SIZE")
+    sig_f = last(got["fetch"], "k_node_signs", "FETCH_SIZE")
+    sig_w = last(got["write"], "k_node_signs", "WRITE_SIZE")
+    if None in (fetch_raw, write_raw, sig_f, sig_w):
+        return {"status": "counters missing from the rocprofv3 output", "commands": cmds}
+    f_scale = 8.0 * N / (sig_f * 1024.0)
+    w_scale = (N / 8.0) / (sig_w * 1024.0)
+    traffic = fetch_raw * 1024.0 * f_scale + write_raw * 1024.0 * w_scale
+    res = {"status": "ok", "traffic_bytes": traffic, "fetch_kib_raw": fetch_raw, "write_kib_raw": write_raw,
+           "fetch_scale": f_scale, "write_scale": w_scale,
+           "calibration": "k_node_signs: reads 8 B/node of phi, writes 1 bit/node", "commands": cmds}
+    waves = last(got["sq"], qk, "SQ_WAVES")
+    valu = last(got["sq"], qk, "SQ_INSTS_VALU")
+    if waves and valu:
+        res["waves"] = waves
+        res["valu_per_wave"] = valu / waves
+        res["salu_per_wave"] = (last(got["sq"], qk, "SQ_INSTS_SALU") or 0.0) / waves
+        res["smem_per_wave"] = (last(got["sq"], qk, "SQ_INSTS_SMEM") or 0.0) / waves
+        # each wave64 VALU instruction issues over 4 cycles of a 16-lane SIMD;
+        # 256 CUs x 4 SIMDs at 2.4 GHz, over the kernel's HIP-event time
+        res["valu_issue_frac"] = valu * 4.0 / (1024 * 2.4e9 * q_ms * 1e-3)
+    return res
+
+
+def profile_child(args):
+    """One reinit of the workload (after one warm-up), for the counter passes."""
+    import torch
+    import fluidtissue_amd as fa
+    wl = WORKLOADS[args.workload]
+    m = fa.kuhn_mesh(wl["n"], jitter=wl["jitter"], seed=1)
+    phi = fa.phi_field(wl["phi"], m.xyz, seed=0)
+    band = wl.get("band_h", 0.0) / (wl["n"] - 1)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    d0 = torch.from_numpy(phi).cuda()
+    for _ in range(2):
+        d = d0.clone()
+        torch.cuda.synchronize()
+        r.reset_phi_device(d.data_ptr(), None, mode=args.mode, blend="init", band=band,
+                           bin_scale=args.bin_scale, query_block=args.query_block)
+        torch.cuda.synchronize()
+    r.close()
+
+
+def served_block(args):
+    """The served form at the default workload's side: the serving stream
+    (sizes log-uniform 32^3-192^3) at each rate and saturated, served against
+    serial plain calls; p50/p99 and throughput ratios."""
+    sa = argparse.Namespace(**vars(args))
+    sa.jobs = args.served_jobs
+    sa.rates = args.served_rates
+    sa.jobs_at_rate = {50.0: min(args.served_jobs, 500)}
+    res = run_serving(sa)
+    out = {"jobs": sa.jobs, "lanes": res["config"]["lanes"], "phi_on": res["config"]["phi_on"],
+           "note": "jobs per rate: " + ", ".join(f"{r:g}/s {sa.jobs_at_rate.get(r, sa.jobs)}" for r in sa.rates)
+                   + f"; saturated {sa.jobs}"}
+    for key, v in res["rates"].items():
+        out[key] = {"served_p50_ms": v["served"]["p50_ms"], "served_p99_ms": v["served"]["p99_ms"],
+                    "serial_p50_ms": v["serial_plain"]["p50_ms"], "serial_p99_ms": v["serial_plain"]["p99_ms"],
+                    "served_Mnodes_per_s": v["served"]["Mnodes_per_s"],
+                    "serial_Mnodes_per_s": v["serial_plain"]["Mnodes_per_s"],
+                    "throughput_ratio": v["throughput_ratio"], "mean_batch": v["served"]["mean_batch"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -464,6 +685,12 @@ def main():
     ap.add_argument("--query-block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-served", action="store_true", help="skip the served block")
+    ap.add_argument("--served-jobs", type=int, default=2000)
+    ap.add_argument("--served-rates", type=float, nargs="+", default=[50.0, 200.0, 500.0])
+    ap.add_argument("--profile-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one field on one GPU; --gpus must be 1 "
@@ -473,6 +700,9 @@ def main():
     import torch
     import fluidtissue_amd as fa
 
+    if args.profile_child:
+        profile_child(args)
+        return
     if args.workload == "batched":
         print(json.dumps(run_batched(args)), flush=True)
         return
@@ -492,11 +722,15 @@ def main():
         f"({time.perf_counter() - t0:.1f} s)")
 
     dev = torch.device("cuda:0")
+    # one stream for the input copy and the reinit, so each step's reinit
+    # starts after its copy of phi
+    bstream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(bstream)
+    stream = bstream.cuda_stream
     r = fa.Reinit(0)
     r.set_mesh(m)
     d_phi0 = torch.from_numpy(phi).to(dev)
     d_phi = torch.empty_like(d_phi0)
-    stream = torch.cuda.current_stream().cuda_stream
 
     def step():
         d_phi.copy_(d_phi0)
@@ -513,11 +747,15 @@ def main():
         bs, qb, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
                                       mode=args.mode, reps=3, sync=torch.cuda.synchronize, copy=copy,
                                       bin_scales=autotune.MODE_BIN_SCALES[args.mode], query_blocks=(64, 256),
-                                      band=band)
+                                      band=band, stream=stream)
         args.bin_scale, args.query_block = bs, qb
         tuned = {"bin_scale": bs, "query_block": qb,
                  "table_ms": [[a, q, round(t, 3)] for a, q, t in table]}
         log(f"autotune picked bin_scale {bs}, query_block {qb}")
+    if args.bin_scale == 0.0:
+        args.bin_scale = 1.25 if args.mode == "triangles" else 2.0
+    if args.query_block == 0:
+        args.query_block = 256
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -529,6 +767,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     phase_ms, calls = r.timing()
+    r.set_timing(False)
     log(f"interface: P={cnt.n_ifp} F={cnt.n_tri} Tx={cnt.n_if_elems}")
     ms = (t1 - t0) * 1e3 / args.steps
     N, T = m.n_nodes, len(m.tets)
@@ -537,19 +776,19 @@ def main():
     log(f"step {ms:.3f} ms; phases: marching tets {ph[0]:.3f} ms, index {ph[1]:.3f} ms, "
         f"query {ph[2]:.3f} ms, normals {ph[3]:.3f} ms")
 
-    # Algorithmic bytes (per reinit): 28N + 16T + 20P + 24Tx + 16F; the query
-    # kernel's share is 28N (coords 12 + phi_old 8 + phi_out 8).
+    # Algorithmic bytes per reinit: B = 28N + 16T + 20P + 24Tx + 16F (coords
+    # 12 + phi in 8 + phi out 8 per node; tet connectivity once; crossing-edge
+    # endpoints and ifp per point; t2e per crossing tet; triangle indices).
+    # The query kernel's own share is 28N.
     b_path = 28 * N + 16 * T + 20 * P + 24 * X + 16 * F
     b_query = 28 * N
     t_query = ph[2] * 1e-3
     peak = 8000.0
-    achieved = b_query / t_query / 1e9
-    traffic = valu = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.mode}.json")
-    if os.path.exists(tf):                       # rocprofv3 counters of this kernel (profiles/)
-        prof = json.load(open(tf))
-        traffic = prof.get("query_bytes_per_launch")
-        valu = prof.get("valu_busy_frac")
+
+    accuracy = None
+    if not args.no_accuracy:
+        accuracy = accuracy_check(r, m, phi, h, args.mode, band, args.bin_scale, args.query_block, stream)
+        log(f"accuracy: {accuracy}")
 
     # the drop-in path with host phi (ft_reinit on the caller's own arrays, as
     # LevelSet::resetPhi_ would call it: H2D of phi, the reinit, D2H of phi and
@@ -570,14 +809,33 @@ def main():
         ms_host.append((time.perf_counter() - t_h) * 1e3)
     ms_host = float(np.median(ms_host))
     log(f"host-phi (PCIe-inclusive) reinit: {ms_host:.3f} ms")
+    r.close()
+    del d_phi, d_phi0
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    torch.cuda.set_stream(torch.cuda.default_stream(dev))
+
+    counters = None
+    if not args.no_counters:
+        counters = live_counters(args, N, ph[2], os.path.join(ROOT, "gpurun_out", "bench_counters"))
+        log(f"counters: { {k: v for k, v in counters.items() if k != 'commands'} }")
 
     cpu = None
+    hinfo = host_info()
     if not args.no_cpu:
-        nthreads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(m, phi, nthreads)
+        cpu = cpu_baseline(m, phi, hinfo["threads_used"])
         log(f"cpu baseline: {cpu['value']:.3f} Mnodes/s ({cpu['sample']})")
         cpu.pop("seconds")
+        cpu["host"] = hinfo
 
+    served = None
+    if not args.no_served:
+        served = served_block(args)
+        log(f"served: {served}")
+
+    traffic = counters.get("traffic_bytes") if counters and counters.get("status") == "ok" else None
+    valu_frac = counters.get("valu_issue_frac") if counters and counters.get("status") == "ok" else None
+    achieved = b_path / t_query / 1e9
     out = {
         "metric": "Mnodes/s reinit at 256^3 + served p50/p99 vs serial plain, <=1e-5*h err",
         "value": N / (ms * 1e-3) / 1e6,
@@ -595,17 +853,27 @@ def main():
                                + (f"narrow band {wl['band_h']:g}h" if band > 0 else "full domain"),
                    "nodes": N, "tets": T, "edges": len(m.edges), "crossing_points": P,
                    "triangles": F, "crossing_tets": X, "h": h, "band": band,
-                   "clamped_nodes": cnt.n_clamped},
+                   "clamped_nodes": cnt.n_clamped, "bin_scale": args.bin_scale,
+                   "query_block": args.query_block},
         "phases_ms": {"marching_tets": ph[0], "index_build": ph[1], "query": ph[2],
                       "normals": ph[3]},
         "roofline": {"kernel": f"k_query_wave<{1 if args.mode == 'triangles' else 0},"
-                               f"{args.query_block or 256}> (nearest-primitive search)",
-                     "bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
+                               f"{args.query_block}> (nearest-primitive search)",
+                     "bound": "valu", "achieved": achieved, "peak": peak, "unit": "GB/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "valu_busy_frac_profiled": valu},
+                     "basis": "achieved = B / query-kernel time, B = 28N + 16T + 20P + 24Tx + 16F "
+                              "bytes per reinit; the kernel is bound by vector instruction issue",
+                     "valu_issue_frac": valu_frac,
+                     "query_kernel_ms": ph[2],
+                     "query_own_bytes": b_query,
+                     "query_own_frac": b_query / t_query / 1e9 / peak,
+                     "step_frac": b_path / (ms * 1e-3) / 1e9 / peak},
         "path_roofline": {"algorithmic_bytes": b_path,
                           "achieved_GBps": b_path / (ms * 1e-3) / 1e9,
                           "frac": b_path / (ms * 1e-3) / 1e9 / peak},
+        "accuracy": accuracy,
+        "counters": counters,
+        "served": served,
         "cpu_baseline": cpu,
         "host_phi_reinit": {"ms": ms_host, "Mnodes_per_s": N / (ms_host * 1e-3) / 1e6,
                             "note": "ft_reinit on the caller's pageable host arrays (median of 3): H2D of "
@@ -613,7 +881,6 @@ def main():
         "autotune": tuned,
     }
     print(json.dumps(out), flush=True)
-    r.close()
 
 
 if __name__ == "__main__":

@@ -19,6 +19,7 @@ FT_ERR_DEGENERATE = -6
 FT_ERR_STATE = -7
 FT_ERR_BUDGET = -8
 FT_ERR_SOLVER = -9
+FT_ERR_INTERNAL = -10
 
 FT_DIST_POINTS = 0
 FT_DIST_TRIANGLES = 1

@@ -20,11 +20,12 @@ _cache = {}
 
 
 def tune(handle, phi_ptr, phi_backup_ptr, nbytes, mode="triangles", reps=2, sync=None, copy=None,
-         signature=None, bin_scales=None, query_blocks=QUERY_BLOCKS, log=None, band=0.0):
+         signature=None, bin_scales=None, query_blocks=QUERY_BLOCKS, log=None, band=0.0, stream=None):
     """handle: a Reinit with its batch set; phi_ptr: device phi (overwritten);
     phi_backup_ptr: device copy of the input, restored before every run by
-    copy(dst, src, nbytes); sync(): waits for the device.  Returns
-    (bin_scale, query_block, table of (bin_scale, query_block, ms))."""
+    copy(dst, src, nbytes); sync(): waits for the device; stream: the HIP
+    stream of the runs (None: the handle's).  Returns (bin_scale,
+    query_block, table of (bin_scale, query_block, ms))."""
     if bin_scales is None:
         bin_scales = MODE_BIN_SCALES[mode]
     key = (mode, band, signature)
@@ -40,7 +41,7 @@ def tune(handle, phi_ptr, phi_backup_ptr, nbytes, mode="triangles", reps=2, sync
                 sync()
                 t0 = time.perf_counter()
                 handle.reset_phi_batch_device(phi_ptr, None, mode=mode, band=band, bin_scale=bs,
-                                              query_block=qb)
+                                              query_block=qb, stream=stream)
                 sync()
                 ts.append(time.perf_counter() - t0)
             ms = float(np.median(ts[1:]) * 1e3)

@@ -47,6 +47,7 @@
 #include "ftreinit.h"
 #include "ft_device.hpp"
 #include "ft_internal.hpp"
+#include "ft_p2t.hpp"
 
 using ftd::Grid;
 using ftd::TriRec;
@@ -569,18 +570,24 @@ __global__ __launch_bounds__(kBS) void k_tris(const int4* tets, const float* xyz
     }
 }
 
-__global__ __launch_bounds__(kBS) void k_p2t_count(const int4* tris, const uint64_t* sc, uint32_t* deg)
+// Point -> triangle lists and point areas; the per-element steps, with their
+// index checks, are in ft_p2t.hpp.
+struct AtomicInc {
+    __device__ uint32_t operator()(uint32_t* p) const { return atomicAdd(p, 1u); }
+};
+
+__global__ __launch_bounds__(kBS) void k_p2t_count(const int4* tris, uint64_t* sc, uint32_t* deg)
 {
     if (halted(sc)) return;
     const int64_t f = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (f >= cnt_tris(sc)) return;
     const int4 t = tris[f];
-    atomicAdd(&deg[t.x], 1u);
-    atomicAdd(&deg[t.y], 1u);
-    atomicAdd(&deg[t.z], 1u);
+    const int32_t v[3] = {t.x, t.y, t.z};
+    if (!ftp::count_tri(v, cnt_points(sc), deg, AtomicInc{}))
+        flag_error((unsigned long long*)(sc + S_ERR), ftp::kErrIndex, f);
 }
 
-__global__ __launch_bounds__(kBS) void k_p2t_fill(const int4* tris, const uint64_t* sc,
+__global__ __launch_bounds__(kBS) void k_p2t_fill(const int4* tris, uint64_t* sc,
                                                   const uint32_t* off, uint32_t* cursor,
                                                   int32_t* p2t)
 {
@@ -588,34 +595,20 @@ __global__ __launch_bounds__(kBS) void k_p2t_fill(const int4* tris, const uint64
     const int64_t f = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (f >= cnt_tris(sc)) return;
     const int4 t = tris[f];
-    const int v[3] = {t.x, t.y, t.z};
-    for (int k = 0; k < 3; k++) {
-        const uint32_t slot = atomicAdd(&cursor[v[k]], 1u);
-        p2t[off[v[k]] + slot] = (int32_t)f;
-    }
+    const int32_t v[3] = {t.x, t.y, t.z};
+    if (!ftp::fill_tri(v, (int32_t)f, cnt_points(sc), off, cursor, p2t, AtomicInc{}))
+        flag_error((unsigned long long*)(sc + S_ERR), ftp::kErrIndex, f);
 }
 
-// Per point: sort its triangle list ascending (= emission order, the order of
-// p2t in the reference) and sum the areas in that order, each step as
-// float(double(area) + 0.5 * double(norm)) (LevelSet.cpp:356).
-__global__ __launch_bounds__(kBS) void k_point_area(const uint32_t* off, const uint64_t* sc,
+__global__ __launch_bounds__(kBS) void k_point_area(const uint32_t* off, uint64_t* sc,
                                                     int32_t* p2t, const float* tri_norm,
                                                     float* area)
 {
     if (halted(sc)) return;
     const int64_t v = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (v >= cnt_points(sc)) return;
-    const uint32_t b = off[v], e = off[v + 1];
-    for (uint32_t i = b + 1; i < e; i++) {
-        const int32_t key = p2t[i];
-        uint32_t j = i;
-        while (j > b && p2t[j - 1] > key) { p2t[j] = p2t[j - 1]; j--; }
-        p2t[j] = key;
-    }
-    float a = 0.0f;
-    for (uint32_t i = b; i < e; i++)
-        a = (float)((double)a + 0.5 * (double)tri_norm[p2t[i]]);
-    area[v] = a;
+    if (!ftp::point_area(v, off, p2t, cnt_tris(sc), tri_norm, area))
+        flag_error((unsigned long long*)(sc + S_ERR), ftp::kErrIndex, v);
 }
 
 // ============================================================================
@@ -2595,7 +2588,7 @@ struct ft_ctx {
     uint64_t* h_scalars = nullptr;   // pinned mirror
     ft_counts last{};
     std::vector<ft_counts> field_counts;
-    std::vector<int32_t> batch_meshes;   // resident meshes of the current batch
+    std::vector<uint64_t> batch_meshes;  // serials of the resident meshes of the current batch
     bool have_interface = false;
     bool have_nearest = false;
     // phase timing (ft_set_timing): events at the phase boundaries of a reinit
@@ -3068,6 +3061,11 @@ int stage_error(uint64_t ev, ft_counts* cnt)
 {
     if (ev == ~0ull) return FT_OK;
     cnt->bad_tet = (int64_t)(ev & ((1ull << 40) - 1));
+    if ((ev >> 40) == (uint64_t)ftp::kErrIndex)
+        return ft::set_error(FT_ERR_INTERNAL,
+                             "interface index out of range at triangle or point %lld: the "
+                             "point -> triangle stage stopped before writing",
+                             (long long)cnt->bad_tet);
     if ((ev >> 40) == 1)
         return ft::set_error(FT_ERR_T2E,
                              "tet %lld: crossing edges from t2e do not match its node signs "
@@ -3692,7 +3690,11 @@ int ft_create(int32_t device, ft_handle* out)
     FT_HIP(hipSetDevice(device));
     ft_ctx* c = new ft_ctx();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    // A blocking stream: a call that passes no stream (NULL, which is also
+    // the address of the null stream) runs on this one, and it must still see
+    // the caller's work on the null stream before it (a copy into phi) and
+    // come before the caller's null-stream work after it.
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess) {
         delete c;
         return ft::set_error(FT_ERR_HIP, "hipStreamCreate failed");
     }
@@ -3906,12 +3908,16 @@ int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges, int32_t*
 // A resident mesh is a one-field handle kept only for its mesh and its query
 // slots; batches are assembled from resident meshes by device copies.
 namespace {
+// Ids of released meshes are reused; the serial is not (it grows with every
+// mesh created in the process), so (id, serial) names one mesh for good.
 struct Resident {
     ft_ctx* c = nullptr;
     int64_t bytes = 0;
+    uint64_t serial = 0;
 };
 std::mutex g_res_mu;
 std::vector<Resident> g_res;
+uint64_t g_res_serial = 0;
 
 int resident_finish(ft_ctx* c, int32_t* id)
 {
@@ -3923,6 +3929,7 @@ int resident_finish(ft_ctx* c, int32_t* id)
     r.bytes = (int64_t)(c->xyz.cap + c->tets.cap + c->edges.cap + c->t2e.cap + c->perm.cap + c->sx.cap +
                         c->sy.cap + c->sz.cap + c->tile_field.cap + c->tile_sched.cap);
     std::lock_guard<std::mutex> lk(g_res_mu);
+    r.serial = ++g_res_serial;
     for (size_t i = 0; i < g_res.size(); i++)
         if (!g_res[i].c) { g_res[i] = r; *id = (int32_t)i; return FT_OK; }
     g_res.push_back(r);
@@ -3930,10 +3937,11 @@ int resident_finish(ft_ctx* c, int32_t* id)
     return FT_OK;
 }
 
-ft_ctx* resident_get(int32_t id)
+ft_ctx* resident_get(int32_t id, uint64_t* serial = nullptr)
 {
     std::lock_guard<std::mutex> lk(g_res_mu);
     if (id < 0 || id >= (int32_t)g_res.size()) return nullptr;
+    if (serial) *serial = g_res[id].serial;
     return g_res[id].c;
 }
 }  // namespace
@@ -3993,15 +4001,16 @@ int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
     FT_TRY(check_handle(h));
     if (n_fields < 1 || !mesh_ids) return ft::set_error(FT_ERR_ARG, "ft_set_batch_meshes: bad arguments");
     std::vector<ft_ctx*> ms(n_fields);
+    std::vector<uint64_t> serials(n_fields);
     for (int k = 0; k < n_fields; k++) {
-        ms[k] = resident_get(mesh_ids[k]);
+        ms[k] = resident_get(mesh_ids[k], &serials[k]);
         if (!ms[k]) return ft::set_error(FT_ERR_ARG, "no resident mesh %d", mesh_ids[k]);
         if (ms[k]->device != h->device)
             return ft::set_error(FT_ERR_ARG, "resident mesh %d lives on another device", mesh_ids[k]);
     }
-    if (h->has_conn && h->batch_meshes.size() == (size_t)n_fields &&
-        std::equal(h->batch_meshes.begin(), h->batch_meshes.end(), mesh_ids))
-        return FT_OK;                                  // already assembled
+    // already assembled from these very meshes: an id alone is not enough, a
+    // released id may since name a new mesh of other dimensions
+    if (h->has_conn && h->batch_meshes == serials) return FT_OK;
     std::vector<int64_t> no(n_fields + 1, 0), to(n_fields + 1, 0), eo(n_fields + 1, 0);
     for (int k = 0; k < n_fields; k++) {
         no[k + 1] = no[k] + ms[k]->N;
@@ -4049,7 +4058,7 @@ int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
     h->h_est = h->fh[0].h;
     FT_TRY(check_launch());
     FT_HIP(hipStreamSynchronize(s));
-    h->batch_meshes.assign(mesh_ids, mesh_ids + n_fields);
+    h->batch_meshes = serials;
     h->has_conn = true;
     return FT_OK;
 }

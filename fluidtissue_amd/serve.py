@@ -49,17 +49,37 @@ def _release(mid):
     check(lib.ft_mesh_release(mid))
 
 
+class _Entry:
+    """A resident mesh of the cache: id (None while it is being created),
+    bytes, pins; `ready` is set once the creator has made it (or failed,
+    then `error` holds why)."""
+    __slots__ = ("id", "bytes", "pins", "ready", "error")
+
+    def __init__(self, mid, nbytes, pins):
+        self.id, self.bytes, self.pins = mid, nbytes, pins
+        self.ready = threading.Event()
+        self.error = None
+
+
 class MeshCache:
     """Resident Kuhn meshes keyed by dims, within a byte budget (LRU).
 
     create(device, dims) -> (id, bytes) and release(id) default to the
-    library's resident meshes."""
+    library's resident meshes.
+
+    Admission reserves: reserve(dims, extra) pins the job's mesh and, when it
+    is not resident, books its estimated bytes at once (evicting unpinned
+    meshes least-recently-used), all under one lock, so that two lanes can
+    never both be admitted against the same free bytes.  The mesh itself is
+    made afterwards, outside the lock, by the lane that booked it
+    (materialize); a lane that pins a mesh another lane is still making waits
+    for it."""
 
     def __init__(self, device=0, budget_bytes=200e9, create=_create_kuhn, release=_release):
         self.device = device
         self.budget = float(budget_bytes)
-        self.meshes = collections.OrderedDict()   # dims -> [id, bytes, pins]
-        self.bytes = 0
+        self.meshes = collections.OrderedDict()   # dims -> _Entry
+        self.bytes = 0                            # booked: made meshes + meshes being made
         self.lock = threading.Lock()
         self._create = create
         self._release = release
@@ -71,47 +91,98 @@ class MeshCache:
         t = 6 * (nx - 1) * (ny - 1) * (nz - 1)
         return 12 * n + 40 * t + 8 * 7 * n + 24 * n
 
-    def _evict_for(self, need):
+    def _evict_for(self, need, keep=None):
+        """Releases unpinned made meshes, least recently used first, until
+        `need` more bytes fit; True when they do."""
         for key in list(self.meshes):
             if self.bytes + need <= self.budget:
                 return True
-            mid, b, pins = self.meshes[key]
-            if pins == 0:
-                self._release(mid)
-                self.bytes -= b
+            e = self.meshes[key]
+            if key != keep and e.pins == 0 and e.id is not None:
+                self._release(e.id)
+                self.bytes -= e.bytes
                 del self.meshes[key]
         return self.bytes + need <= self.budget
 
     def can_admit(self, dims, extra):
+        """Would reserve(dims, extra) succeed now?  (No side effects.)"""
         with self.lock:
-            need = extra + (0 if tuple(dims) in self.meshes else self.estimate_bytes(dims))
-            if self.bytes + need <= self.budget:
-                return True
-            free = sum(b for _, b, p in self.meshes.values() if p == 0)
+            e = self.meshes.get(tuple(dims))
+            need = extra + (0 if e is not None else self.estimate_bytes(dims))
+            free = sum(x.bytes for k, x in self.meshes.items()
+                       if x.pins == 0 and x.id is not None and k != tuple(dims))
             return self.bytes - free + need <= self.budget
 
-    def acquire(self, dims):
+    def reserve(self, dims, extra=0):
+        """Pins the mesh of `dims` and books `extra` bytes of work beside all
+        resident meshes: returns (entry, must_create) or None when it does
+        not fit (nothing changes then)."""
         key = tuple(dims)
         with self.lock:
-            if key in self.meshes:
+            e = self.meshes.get(key)
+            if e is not None:
+                if not self._evict_for(extra, keep=key):
+                    return None
                 self.meshes.move_to_end(key)
-                self.meshes[key][2] += 1
-                return self.meshes[key][0]
-            if not self._evict_for(self.estimate_bytes(key)):
-                raise _lib.FtError(_lib.FT_ERR_BUDGET, f"mesh {key} does not fit the budget")
-            mid, b = self._create(self.device, key)
-            self.meshes[key] = [mid, b, 1]
-            self.bytes += b
-            return mid
+                e.pins += 1
+                return e, False
+            est = self.estimate_bytes(key)
+            if not self._evict_for(est + extra):
+                return None
+            e = _Entry(None, est, 1)
+            self.meshes[key] = e
+            self.bytes += est
+            return e, True
+
+    def materialize(self, dims, entry, must_create):
+        """Makes a booked mesh (must_create) or waits until the lane that
+        booked it has; returns its id."""
+        if must_create:
+            try:
+                mid, b = self._create(self.device, tuple(dims))
+            except Exception as err:
+                with self.lock:
+                    entry.error = err
+                    self.bytes -= entry.bytes
+                    if self.meshes.get(tuple(dims)) is entry:
+                        del self.meshes[tuple(dims)]
+                entry.ready.set()
+                raise
+            with self.lock:
+                self.bytes += b - entry.bytes
+                entry.id, entry.bytes = mid, b
+            entry.ready.set()
+        else:
+            entry.ready.wait()
+            if entry.error is not None:
+                raise entry.error
+        return entry.id
+
+    def acquire(self, dims):
+        """reserve + materialize for one mesh; FtError(FT_ERR_BUDGET) when it
+        does not fit."""
+        r = self.reserve(dims, 0)
+        if r is None:
+            raise _lib.FtError(_lib.FT_ERR_BUDGET, f"mesh {tuple(dims)} does not fit the budget")
+        return self.materialize(dims, *r)
 
     def release(self, dims):
         with self.lock:
-            self.meshes[tuple(dims)][2] -= 1
+            e = self.meshes.get(tuple(dims))
+            if e is not None:
+                e.pins -= 1
+
+    def unpin(self, entry):
+        """Drops one pin of a reserved entry (a failed creation may already
+        have removed it, and a new entry may hold its dims since)."""
+        with self.lock:
+            entry.pins -= 1
 
     def close(self):
         with self.lock:
-            for mid, _, _ in self.meshes.values():
-                self._release(mid)
+            for e in self.meshes.values():
+                if e.id is not None:
+                    self._release(e.id)
             self.meshes.clear()
             self.bytes = 0
 
@@ -155,25 +226,28 @@ class Job:
         return self.out
 
 
-def admit(queue, max_fields, max_nodes, inflight_bytes, can_admit):
+def admit(queue, max_fields, max_nodes, inflight_bytes, reserve):
     """Takes from the front of `queue` the jobs that fit one batch: FIFO with no
     overtaking, at most max_fields jobs and max_nodes nodes (a single larger
-    job is still admitted alone), and only while can_admit(dims, bytes) says
-    the meshes plus the work of the in-flight and admitted jobs fit the
-    budget.  Returns (jobs, work bytes of the admitted jobs)."""
-    jobs, nodes, extra = [], 0, 0
+    job is still admitted alone), and only while reserve(dims, bytes) can pin
+    the job's mesh with the work of the in-flight and admitted jobs beside
+    the resident meshes.  Returns (jobs, reservations, work bytes of the
+    admitted jobs); the caller holds the reservations until the batch ends."""
+    jobs, res, nodes, extra = [], [], 0, 0
     while queue and len(jobs) < max_fields:
         j = queue[0]
         work = WORK_BYTES_PER_NODE * j.n
         if jobs and nodes + j.n > max_nodes:
             break
-        if not can_admit(j.dims, inflight_bytes + extra + work):
+        r = reserve(j.dims, inflight_bytes + extra + work)
+        if r is None:
             break
         queue.popleft()
         jobs.append(j)
+        res.append(r)
         nodes += j.n
         extra += work
-    return jobs, extra
+    return jobs, res, extra
 
 
 class Server:
@@ -209,10 +283,19 @@ class Server:
         return job
 
     def _admit(self):
-        jobs, extra = admit(self.queue, self.max_fields, self.max_nodes, self.inflight_bytes,
-                            self.cache.can_admit)
+        """Under cv: the next batch with its meshes pinned and booked.  A job
+        that cannot fit even with nothing in flight (every mesh evictable)
+        never will: it fails with FT_ERR_BUDGET instead of waiting."""
+        jobs, res, extra = admit(self.queue, self.max_fields, self.max_nodes, self.inflight_bytes,
+                                 self.cache.reserve)
+        if not jobs and self.queue and self.inflight_bytes == 0:
+            j = self.queue.popleft()
+            j.error = _lib.FtError(_lib.FT_ERR_BUDGET,
+                                   f"job of {j.dims} does not fit the {self.budget:.3g}-byte budget")
+            j.t_done = time.perf_counter()
+            j.done.set()
         self.inflight_bytes += extra
-        return jobs, extra
+        return jobs, res, extra
 
     def _lane(self, h):
         import torch
@@ -225,16 +308,16 @@ class Server:
                     self.cv.wait()
                 if self.stop and not self.queue:
                     return
-                jobs, extra = self._admit()
+                jobs, res, extra = self._admit()
                 if not jobs:              # memory is held by the other lane
                     self.cv.wait(0.001)
                     continue
             t0 = time.perf_counter()
             ids = []
             try:
-                for j in jobs:
+                for j, (entry, make) in zip(jobs, res):
                     j.t_start = t0
-                    ids.append(self.cache.acquire(j.dims))
+                    ids.append(self.cache.materialize(j.dims, entry, make))
                 h.set_batch_meshes(ids)
                 total = sum(j.n for j in jobs)
                 with torch.cuda.stream(stream):
@@ -282,8 +365,8 @@ class Server:
                     j.error = e
                     j.t_done = time.perf_counter()
             finally:
-                for j in jobs[:len(ids)]:
-                    self.cache.release(j.dims)
+                for entry, _ in res:      # every admitted job holds a pin
+                    self.cache.unpin(entry)
                 with self.cv:
                     self.inflight_bytes -= extra
                     self.cv.notify_all()

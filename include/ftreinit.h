@@ -51,6 +51,9 @@ extern "C" {
 #define FT_ERR_STATE       -7  /* call order (e.g. reinit before set_mesh)    */
 #define FT_ERR_BUDGET      -8  /* request exceeds the device memory budget    */
 #define FT_ERR_SOLVER      -9  /* the advection solve did not converge        */
+#define FT_ERR_INTERNAL   -10  /* a stage's own index check failed (a fault in
+                                  an earlier stage): nothing was written past
+                                  the check and phi is unchanged              */
 
 /* ---- options ----------------------------------------------------------- */
 #define FT_DIST_POINTS     0   /* reference semantics: nearest crossing point
@@ -85,7 +88,8 @@ typedef struct ft_counts {
     int64_t n_tri;       /* F: interface triangles                             */
     int64_t n_if_elems;  /* crossing tets                                      */
     int64_t n_clamped;   /* nodes clamped to the band                          */
-    int64_t bad_tet;     /* first offending tet on FT_ERR_T2E/_DEGENERATE      */
+    int64_t bad_tet;     /* first offending tet on FT_ERR_T2E/_DEGENERATE (the
+                            triangle or point on FT_ERR_INTERNAL)              */
 } ft_counts;
 
 typedef struct ft_ctx* ft_handle;
@@ -254,7 +258,8 @@ int ft_mesh_release(int32_t mesh_id);
 
 /* Makes the handle's batch the fields of the given resident meshes, in order
  * (a mesh may appear more than once); device copies only, skipped when the
- * list equals the current one. */
+ * list names the very meshes of the current batch (a released id that now
+ * names a new mesh is assembled afresh). */
 int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids);
 
 /* Per-field counts of the last reinit (n_fields entries). */
@@ -274,7 +279,10 @@ int ft_reinit(ft_handle h, double* phi, double* phi_ifn, const ft_opts* o,
               ft_counts* counts);
 
 /* Same, on device-resident phi (device pointers on the handle's device).
- * stream: a hipStream_t, or NULL for the handle's own stream. */
+ * stream: a hipStream_t, or NULL for the handle's own stream, which is
+ * ordered after the caller's earlier work on the null stream (stream 0, the
+ * same value as NULL: a caller that filled phi on the null stream may pass
+ * NULL) and before its later work there. */
 int ft_reinit_device(ft_handle h, double* d_phi, double* d_phi_ifn,
                      const ft_opts* o, ft_counts* counts, void* stream);
 

@@ -28,7 +28,7 @@ def test_cache_reuses_and_evicts_least_recently_used():
     assert made == [(20, 20, 20), (21, 20, 20)]
     c.acquire((22, 20, 20))                         # needs room: evicts (21,...), the LRU one
     assert freed == [m2] and (20, 20, 20) in c.meshes and (21, 20, 20) not in c.meshes
-    assert c.bytes == sum(b for _, b, _ in c.meshes.values()) <= c.budget
+    assert c.bytes == sum(e.bytes for e in c.meshes.values()) <= c.budget
 
 
 def test_cache_never_evicts_pinned_meshes():
@@ -51,13 +51,13 @@ def _job(n):
 
 def test_admit_fifo_counts_nodes_and_budget():
     q = collections.deque(_job(n) for n in (10, 12, 8, 30, 9))
-    always = lambda dims, extra: True
-    jobs, extra = admit(q, max_fields=3, max_nodes=10 ** 9, inflight_bytes=0, can_admit=always)
-    assert [j.n for j in jobs] == [1000, 1728, 512] and len(q) == 2
+    always = lambda dims, extra: ("entry", False)
+    jobs, res, extra = admit(q, max_fields=3, max_nodes=10 ** 9, inflight_bytes=0, reserve=always)
+    assert [j.n for j in jobs] == [1000, 1728, 512] and len(q) == 2 and len(res) == 3
     assert extra == WORK_BYTES_PER_NODE * (1000 + 1728 + 512)
     # node cap: the 30^3 job does not join a batch that would exceed it, but
     # is admitted alone next time
-    jobs, _ = admit(q, max_fields=8, max_nodes=20000, inflight_bytes=0, can_admit=always)
+    jobs, _, _ = admit(q, max_fields=8, max_nodes=20000, inflight_bytes=0, reserve=always)
     assert [j.n for j in jobs] == [27000]
     # budget: nothing is taken past a job that does not fit (no overtaking)
     q = collections.deque(_job(n) for n in (10, 40, 8))
@@ -65,10 +65,63 @@ def test_admit_fifo_counts_nodes_and_budget():
 
     def fits(dims, extra):
         seen.append((dims, extra))
-        return dims[0] != 40
-    jobs, extra = admit(q, 8, 10 ** 9, inflight_bytes=5, can_admit=fits)
+        return ("entry", False) if dims[0] != 40 else None
+    jobs, _, extra = admit(q, 8, 10 ** 9, inflight_bytes=5, reserve=fits)
     assert [j.dims[0] for j in jobs] == [10] and [j.dims[0] for j in q] == [40, 8]
     assert seen[0][1] == 5 + WORK_BYTES_PER_NODE * 1000    # in-flight work counted
+
+
+def test_admission_reserves_meshes_so_two_lanes_cannot_share_free_bytes():
+    """Two lanes each admitting a job whose mesh is not resident, with a budget
+    that holds one mesh: the first reservation books the bytes, so the second
+    lane is refused (it waits) instead of both passing a check and the second
+    failing at creation."""
+    b = MeshCache.estimate_bytes((20, 20, 20))
+    c, made, freed = _fake_cache(1.5 * b)
+    work = WORK_BYTES_PER_NODE * 8000
+    lane1 = admit(collections.deque([_job(20)]), 8, 10 ** 9, 0, c.reserve)
+    assert len(lane1[0]) == 1 and lane1[1][0][1] is True       # booked, to be made by lane 1
+    q2 = collections.deque([types.SimpleNamespace(dims=(20, 21, 20), n=8400)])
+    lane2 = admit(q2, 8, 10 ** 9, lane1[2], c.reserve)          # before lane 1 made its mesh
+    assert lane2[0] == [] and len(q2) == 1 and made == []
+    entry, make = lane1[1][0]
+    assert c.materialize((20, 20, 20), entry, make) == 1 and made == [(20, 20, 20)]
+    c.unpin(entry)                                              # lane 1's batch ends
+    lane2 = admit(q2, 8, 10 ** 9, 0, c.reserve)                 # now the pinned-free mesh is evicted
+    assert len(lane2[0]) == 1 and freed == [1]
+    assert c.bytes <= c.budget and work > 0
+
+
+def test_admission_shares_a_mesh_being_made():
+    """A second job for the same dims pins the booked entry and waits for the
+    lane that makes it, and a failed creation reaches both jobs."""
+    import threading
+    b = MeshCache.estimate_bytes((16, 16, 16))
+    gate = threading.Event()
+
+    def slow_fail(device, dims):
+        gate.wait(5)
+        raise RuntimeError("no memory")
+    c = MeshCache(0, 4 * b, create=slow_fail, release=lambda mid: None)
+    e1, make1 = c.reserve((16, 16, 16), 0)
+    e2, make2 = c.reserve((16, 16, 16), 0)
+    assert e1 is e2 and make1 and not make2 and e1.pins == 2
+    errs = []
+
+    def waiter():
+        try:
+            c.materialize((16, 16, 16), e2, make2)
+        except RuntimeError as err:
+            errs.append(err)
+    t = threading.Thread(target=waiter)
+    t.start()
+    gate.set()
+    with pytest.raises(RuntimeError):
+        c.materialize((16, 16, 16), e1, make1)
+    t.join(5)
+    assert len(errs) == 1 and c.bytes == 0 and (16, 16, 16) not in c.meshes
+    c.unpin(e1); c.unpin(e2)
+    assert c.reserve((16, 16, 16), 0)[1] is True                # booked afresh
 
 
 def test_autotune_picks_fastest_and_caches():
@@ -78,7 +131,7 @@ def test_autotune_picks_fastest_and_caches():
     cost = {(1.0, 64): 3.0, (1.0, 256): 2.0, (1.5, 64): 1.0, (1.5, 256): 4.0}
 
     class FakeHandle:
-        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block):
+        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block, stream=None):
             calls.append((bin_scale, query_block))
             clock[0] += cost[(bin_scale, query_block)]
 
@@ -107,7 +160,7 @@ def test_autotune_default_candidates_follow_the_mode():
     tried = []
 
     class FakeHandle:
-        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block):
+        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block, stream=None):
             tried.append((mode, bin_scale))
 
     for mode in ("points", "triangles"):
