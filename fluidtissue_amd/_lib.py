@@ -31,6 +31,8 @@ FT_PHI_STAR = 2
 FT_PHI_SPHERES = 3
 FT_PHI_ELLIPSOIDS = 4
 
+FT_NOALLOC_SCRATCH_BYTES = 128 << 20
+
 
 class FtOpts(C.Structure):
     _fields_ = [
@@ -101,6 +103,7 @@ _SIGS = {
     "ft_reset_phi_points": (C.c_int, [_H, _P, C.c_int64, _P]),
     "ft_reinit": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), C.POINTER(FtCounts)]),
     "ft_reinit_device": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), C.POINTER(FtCounts), _P]),
+    "ft_reinit_device_noalloc": (C.c_int, [_H, _P, _P, C.POINTER(FtOpts), C.POINTER(FtCounts), _P]),
     "ft_get_interface": (C.c_int, [_H, _P, _P, _P, _P, _P, _P, _P]),
     "ft_get_nearest": (C.c_int, [_H, _P]),
     "ft_interface_device": (C.c_int, [_H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),

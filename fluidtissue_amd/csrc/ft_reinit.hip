@@ -2628,10 +2628,31 @@ std::vector<void*> buffer_addresses(ft_ctx* c)
     return a;
 }
 
+// Workspace of an allocation-free call (ft_reinit_device_noalloc): while it
+// is set on the calling thread, ensure() hands out consecutive pieces of it
+// instead of allocating, and release() drops buffers without freeing.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+};
+thread_local Arena* t_arena = nullptr;
+
 int ensure(DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (bytes <= b.cap) return FT_OK;
+    if (t_arena) {
+        const size_t want = (bytes + 255) & ~(size_t)255;
+        if (t_arena->used + want > t_arena->size)
+            return ft::set_error(FT_ERR_BUDGET,
+                                 "the workspace of this reinit exceeds the %zu-byte scratch of "
+                                 "ft_reinit_device_noalloc (%zu bytes taken, %zu more asked); use "
+                                 "ft_reinit_device", t_arena->size, t_arena->used, want);
+        b.p = t_arena->base + t_arena->used;
+        b.cap = want;
+        t_arena->used += want;
+        return FT_OK;
+    }
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
@@ -2648,7 +2669,7 @@ int ensure(DevBuf& b, size_t bytes)
 
 void release(DevBuf& b)
 {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p && !t_arena) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -3465,7 +3486,7 @@ int validate_opts(const ft_opts* o)
 inline int64_t with_room(int64_t v) { return v + v / 4 + 256; }
 
 int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
-                  ft_counts* counts, hipStream_t s)
+                  ft_counts* counts, hipStream_t s, bool allow_spec = true)
 {
     FT_TRY(validate_opts(o));
     if (!c->has_conn) return ft::set_error(FT_ERR_STATE, "reinit before a mesh was set");
@@ -3483,7 +3504,7 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     // readback between stages (the device guards them, and halts before phi is
     // written if a stage outgrows them); else, or then, the exact pass, which
     // reads the stage totals back to size the buffers.
-    const bool spec = rc == FT_OK && c->spec_layout == c->layout_gen && c->spec_mode == sp.mode;
+    const bool spec = allow_spec && rc == FT_OK && c->spec_layout == c->layout_gen && c->spec_mode == sp.mode;
     if (spec) {
         cp = c->spec_caps;
         rc = enqueue_marching(c, s, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cp, false, false,
@@ -3646,6 +3667,86 @@ int reinit_graph(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, 
 }
 
 std::mutex g_create_mu;
+
+// ---- allocation-free reinit ---------------------------------------------------
+// Per-call workspaces of ft_reinit_device_noalloc: one file-scope device array
+// per GPU, held by every process that loads the library until it ends.  Its
+// device address comes from hipGetSymbolAddress, once per device (the host
+// name `scratch` is only a stand-in, not an address a kernel may use).
+constexpr size_t kScratchBytes = size_t(FT_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;    // per device; null until looked up
+
+int scratch_address(int device, void** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));   // device is current
+        if (e != hipSuccess || !p)
+            return ft::set_error(FT_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                                 hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return FT_OK;
+}
+
+// The handle's per-call buffers (everything but the resident mesh, its query
+// slots and tile lists, and the small scalars).
+std::vector<DevBuf*> call_buffers(ft_ctx* h)
+{
+    return {&h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk, &h->xtet,
+            &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp, &h->ifp2e, &h->tris,
+            &h->tri_norm, &h->if_elems, &h->p2t_off, &h->p2t_cursor, &h->p2t, &h->ifp_area,
+            &h->nearest, &h->fcounts, &h->fclamp, &h->normals, &h->grad_t, &h->phi_ifn_int,
+            &h->bin_count, &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
+            &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent, &h->bnodes, &h->bflags,
+            &h->stats, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots, &h->scan_bsum};
+}
+
+// reinit_device with every per-call buffer taken from the device's scratch:
+// the handle's own buffers are set aside for the call and put back after it,
+// and the interface, nearest indices and normals of the call are not kept.
+int reinit_noalloc(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, ft_counts* counts,
+                   hipStream_t s)
+{
+    FT_TRY(validate_opts(o));
+    if (o->want_nearest || o->want_normals)
+        return ft::set_error(FT_ERR_ARG, "ft_reinit_device_noalloc keeps no nearest indices or normals "
+                                         "(they would live in the shared scratch)");
+    void* base = nullptr;
+    FT_TRY(scratch_address(c->device, &base));
+    DevBuf saved[64];
+    const std::vector<DevBuf*> bufs = call_buffers(c);
+    if (bufs.size() > 64) return ft::set_error(FT_ERR_STATE, "too many call buffers");
+    for (size_t i = 0; i < bufs.size(); i++) { saved[i] = *bufs[i]; *bufs[i] = DevBuf{}; }
+    const uint64_t idx_layout = c->idx_layout;
+    const float idx_bs = c->idx_bin_scale;
+    const int64_t idx_G = c->idx_G;
+    const int idx_bits = c->idx_code_bits;
+    c->idx_layout = ~0ull;                           // grids are uploaded into the scratch
+    Arena arena;
+    arena.base = static_cast<char*>(base);
+    arena.size = kScratchBytes;
+    t_arena = &arena;
+    const int rc = reinit_device(c, d_phi, d_phi_ifn, o, counts, s, false);
+    t_arena = nullptr;
+    // a call stopped part way (e.g. out of scratch) leaves kernels in flight
+    // that write into the scratch: let them end before returning
+    const hipError_t es = hipStreamSynchronize(s);
+    for (size_t i = 0; i < bufs.size(); i++) *bufs[i] = saved[i];
+    c->idx_layout = idx_layout;
+    c->idx_bin_scale = idx_bs;
+    c->idx_G = idx_G;
+    c->idx_code_bits = idx_bits;
+    c->have_interface = c->have_nearest = c->have_normals = false;
+    if (rc == FT_OK && es != hipSuccess)
+        return ft::set_error(FT_ERR_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(es));
+    return rc;
+}
 
 }  // namespace
 
@@ -4516,6 +4617,15 @@ int ft_reinit_device(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opt
     FT_TRY(check_handle(h));
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     return reinit_device(h, d_phi, d_phi_ifn, o, counts, s);
+}
+
+int ft_reinit_device_noalloc(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts* o,
+                             ft_counts* counts, void* stream)
+{
+    FT_TRY(check_handle(h));
+    if (!d_phi) return ft::set_error(FT_ERR_ARG, "null phi");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    return reinit_noalloc(h, d_phi, d_phi_ifn, o, counts, s);
 }
 
 int ft_reinit(ft_handle h, double* phi, double* phi_ifn, const ft_opts* o, ft_counts* counts)

@@ -194,6 +194,19 @@ class Reinit(_Handle):
                                    C.byref(cnt), C.c_void_p(stream) if stream else None))
         return cnt
 
+    def reset_phi_device_noalloc(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
+                                 bin_scale=0.0, stream=None, query_block=0):
+        """reset_phi_device with no device allocation: every workspace comes
+        from the library's fixed scratch (FT_NOALLOC_SCRATCH_BYTES per GPU).
+        No nearest indices, normals or interface are kept."""
+        o = make_opts(mode, blend, band, bin_scale, False, query_block=query_block)
+        cnt = _lib.FtCounts()
+        self.counts = cnt
+        check(lib.ft_reinit_device_noalloc(self._h, C.c_void_p(phi_ptr),
+                                           C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o),
+                                           C.byref(cnt), C.c_void_p(stream) if stream else None))
+        return cnt
+
     # ---- batches of independent fields ------------------------------------
     def set_batch(self, meshes):
         """Resident batch of independent fields from host meshes."""

@@ -286,6 +286,23 @@ int ft_reinit(ft_handle h, double* phi, double* phi_ifn, const ft_opts* o,
 int ft_reinit_device(ft_handle h, double* d_phi, double* d_phi_ifn,
                      const ft_opts* o, ft_counts* counts, void* stream);
 
+/* ft_reinit_device without allocating (no counterpart in the reference; for
+ * callers that cannot allocate device memory: tight serving loops, code that
+ * must not call hipMalloc).  Every per-call workspace (bitmaps, interface,
+ * bin index, tree, scan scratch) is taken from one file-scope device array of
+ * FT_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its load to
+ * the end of the process; the handle's resident mesh is set up beforehand as
+ * usual.  Results are those of ft_reinit_device, bit for bit.  A reinit whose
+ * workspace exceeds the scratch fails with FT_ERR_BUDGET before phi is
+ * written (up to about 128^3 nodes fits, depending on the interface).
+ * want_nearest and want_normals must be 0, and ft_get_interface has nothing
+ * after this call: those outputs would live in the scratch.  Calls that
+ * overlap on one GPU (from several threads or streams) share that scratch and
+ * race; the call returns after its work is done. */
+#define FT_NOALLOC_SCRATCH_BYTES (128u << 20)
+int ft_reinit_device_noalloc(ft_handle h, double* d_phi, double* d_phi_ifn, const ft_opts* o,
+                             ft_counts* counts, void* stream);
+
 /* Copies the interface of the last reinit to host memory.  Any pointer may
  * be NULL; sizes come from ft_counts (p2t_offsets: n_ifp+1, p2t_tris:
  * 3*n_tri). */
