@@ -5,9 +5,14 @@
 // implemented in libftreinit.so on gfx950 HIP kernels; see include/ftreinit.h
 // for the C ABI underneath.
 //
-// Behaviour differences, all in failure cases: with no HIP device, or on a
-// device error, the methods throw std::runtime_error instead of falling back
-// to a CPU loop or failing silently.
+// With no HIP device the methods run the reference's CPU branch
+// (EuclideanDistance.cu:82-85, 107-117) in the library's own host loop, on
+// nThreads threads; FT_REQUIRE_DEVICE=1 in the environment makes init throw
+// instead.  On a device error the methods throw std::runtime_error where the
+// reference fails silently.  Copies and moved objects work as the
+// reference's do (each uploads its nodes to its own device handle on first
+// use); like the reference, objects hold device memory until the process
+// ends.
 //
 
 #pragma once

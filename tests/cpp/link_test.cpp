@@ -8,7 +8,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <new>
+#include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "EuclideanDistance.hpp"
@@ -44,10 +47,36 @@ int main(int argc, char** argv)
     const int nThreads = std::thread::hardware_concurrency();
     dist.init(p.data(), phi.data(), N, nThreads);
     std::vector<double> phi_n = phi;            // LevelSet.cpp:438
-    dist.resetPhi(ifp.data(), (int)(ifp.size() / 3), phi_n.data());
     const int P = (int)(ifp.size() / 3);
+    dist.resetPhi(ifp.data(), P, phi_n.data());
+    const bool copies = argc > 2 && std::string(argv[2]) == "copy";
+    // copies: a copy and a moved object of an initialised one (the implicit
+    // copy of the reference class), then an object copied into the storage
+    // of a destroyed one that had other nodes (the registry's address reused)
+    std::vector<double> phi_c = phi, phi_m = phi, phi_r = phi;
+    if (copies) {
+        EuclideanDistance copy = dist;
+        copy.resetPhi(ifp.data(), P, phi_c.data());
+        EuclideanDistance moved = std::move(copy);
+        moved.resetPhi(ifp.data(), P, phi_m.data());
+        std::vector<float> shifted(p.size());
+        for (size_t a = 0; a < p.size(); a++) shifted[a] = p[a] + 0.25f;
+        alignas(EuclideanDistance) unsigned char store[sizeof(EuclideanDistance)];
+        EuclideanDistance* other = new (store) EuclideanDistance();
+        other->init(shifted.data(), phi.data(), N, nThreads);
+        std::vector<double> scratch = phi;
+        other->resetPhi(ifp.data(), P, scratch.data());
+        other->~EuclideanDistance();
+        EuclideanDistance* reused = new (store) EuclideanDistance(dist);
+        reused->resetPhi(ifp.data(), P, phi_r.data());
+    }
     std::printf("%d %d\n", N, P);
     for (float v : ifp) std::printf("%a\n", (double)v);
-    for (int a = 0; a < N; a++) std::printf("%a %a\n", phi[a], phi_n[a]);
+    for (int a = 0; a < N; a++) {
+        if (copies)
+            std::printf("%a %a %a %a %a\n", phi[a], phi_n[a], phi_c[a], phi_m[a], phi_r[a]);
+        else
+            std::printf("%a %a\n", phi[a], phi_n[a]);
+    }
     return 0;
 }

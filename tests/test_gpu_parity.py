@@ -335,27 +335,13 @@ def test_triangle_mode_degenerate_triangles(rein):
 
 def test_cpp_drop_in_link(tmp_path):
     """A reference-style C++ TU links the drop-in class and gets the oracle's
-    bits (EuclideanDistance.cu:108-117 semantics)."""
-    import os
-    import subprocess
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    exe = tmp_path / "link_test"
-    lib_dir = os.path.join(root, "fluidtissue_amd")
-    subprocess.run(["g++", "-std=c++14", "-O2", "-I" + os.path.join(root, "include"),
-                    os.path.join(root, "tests", "cpp", "link_test.cpp"), "-L" + lib_dir,
-                    "-lftreinit", "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
-    out = subprocess.run([str(exe), "12"], check=True, capture_output=True, text=True).stdout
-    tok = out.split()
-    N, P = int(tok[0]), int(tok[1])
-    ifp = np.array([float.fromhex(x) for x in tok[2:2 + 3 * P]], np.float32).reshape(P, 3)
-    rest = np.array([float.fromhex(x) for x in tok[2 + 3 * P:]]).reshape(N, 2)
-    phi, got = rest[:, 0], rest[:, 1]
-    n = 12
-    h = np.float32(1.0 / (n - 1))
-    g = np.arange(n, dtype=np.float32) * h
-    xyz = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
-    ref, _ = oracle.reset_phi_points(xyz, ifp, phi)
-    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    bits (EuclideanDistance.cu:108-117 semantics) on the device, from the
+    object, a copy, a moved object and a copy placed at a reused address."""
+    from tests.drop_in import run_link_test
+    got, ref = run_link_test(tmp_path, copies=True)
+    assert "no HIP device" not in got["stderr"]
+    for col in got["phi_cols"]:
+        assert np.array_equal(col.view(np.uint64), ref.view(np.uint64))
 
 
 def test_repeated_calls_outgrowing_capacities():
