@@ -195,33 +195,30 @@ def run_batched(args):
     def copy(dst, src, nb):
         d_phi.copy_(d_phi0)
 
-    bs, qb, table = autotune.tune(b, d_phi.data_ptr(), d_phi0.data_ptr(), nbytes, mode=args.mode,
-                                  reps=1, sync=torch.cuda.synchronize, copy=copy, log=log)
-    log(f"autotune picked bin_scale {bs}, query_block {qb}")
+    best, table = autotune.tune(b, d_phi.data_ptr(), d_phi0.data_ptr(), nbytes, mode=args.mode,
+                                reps=1, sync=torch.cuda.synchronize, copy=copy, log=log)
+    log(f"autotune picked {best}")
+    L = {k: best[k] for k in ("bin_scale", "query_block", "waves_per_simd", "schedule")}
     for _ in range(args.warmup):
         d_phi.copy_(d_phi0)
-        b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
-                                 query_block=qb, stream=stream)
+        b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, stream=stream, **L)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         d_phi.copy_(d_phi0)
-        b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
-                                 query_block=qb, stream=stream)
+        b.reset_phi_batch_device(d_phi.data_ptr(), None, mode=args.mode, stream=stream, **L)
     torch.cuda.synchronize()
     ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
     eager_out = d_phi.cpu().numpy()
     # the same batch as one captured HIP graph per call (captured on the first)
     for _ in range(max(1, args.warmup)):
         d_phi.copy_(d_phi0)
-        b.reset_phi_graph_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
-                                 query_block=qb, stream=stream)
+        b.reset_phi_graph_device(d_phi.data_ptr(), None, mode=args.mode, stream=stream, **L)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         d_phi.copy_(d_phi0)
-        b.reset_phi_graph_device(d_phi.data_ptr(), None, mode=args.mode, bin_scale=bs,
-                                 query_block=qb, stream=stream)
+        b.reset_phi_graph_device(d_phi.data_ptr(), None, mode=args.mode, stream=stream, **L)
     torch.cuda.synchronize()
     ms_batched = (time.perf_counter() - t0) * 1e3 / args.steps
     batched_out = d_phi.cpu().numpy()
@@ -241,8 +238,7 @@ def run_batched(args):
     def serial():
         for k, n in enumerate(sizes):
             d_work[k].copy_(d_fields[k])
-            singles[int(n)].reset_phi_device(d_work[k].data_ptr(), None, mode=args.mode,
-                                             bin_scale=bs, query_block=qb, stream=stream)
+            singles[int(n)].reset_phi_device(d_work[k].data_ptr(), None, mode=args.mode, stream=stream, **L)
     serial()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -270,8 +266,8 @@ def run_batched(args):
         "eager_batched": {"ms": ms_eager, "Mnodes_per_s": N / (ms_eager * 1e-3) / 1e6},
         "graph": {"captures": captures, "launches": launches, "identical_to_eager": graph_same},
         "speedup_vs_serial": ms_serial / ms_batched, "identical_to_serial": same,
-        "autotune": {"bin_scale": bs, "query_block": qb,
-                     "table_ms": [[a, q, round(t, 3)] for a, q, t in table]},
+        "autotune": dict(best, table_ms=[[a, q, w, sch, round(t, 3)] for a, q, w, sch, t in table],
+                         table_columns=["bin_scale", "query_block", "waves_per_simd", "schedule", "ms"]),
     }
 
 
@@ -481,7 +477,7 @@ def host_info():
     return info
 
 
-def accuracy_check(r, m, phi, h, mode, band, bin_scale, query_block, stream, n_sample=2000, seed=3):
+def accuracy_check(r, m, phi, h, mode, band, stream, launch, n_sample=2000, seed=3):
     """One more reinit (untimed) with the nearest triangle kept: the interface
     against the oracle's marching tets (bit-exact) and a seeded node sample's
     distances against the oracle's FP64 brute force over all triangles."""
@@ -489,8 +485,8 @@ def accuracy_check(r, m, phi, h, mode, band, bin_scale, query_block, stream, n_s
     import oracle
     t0 = time.perf_counter()
     d = torch.from_numpy(phi).cuda()
-    r.reset_phi_device(d.data_ptr(), None, mode=mode, blend="init", band=band, bin_scale=bin_scale,
-                       query_block=query_block, want_nearest=True, stream=stream)
+    r.reset_phi_device(d.data_ptr(), None, mode=mode, blend="init", band=band, want_nearest=True,
+                       stream=stream, **launch)
     torch.cuda.synchronize()
     ifn = d.cpu().numpy()
     near = r.nearest()
@@ -572,7 +568,8 @@ def live_counters(args, N, q_ms, out_root):
     if prof is None:
         return {"status": "rocprofv3 not found"}
     child = [sys.executable, os.path.abspath(__file__), "--profile-child", "--workload", args.workload,
-             "--mode", args.mode, "--bin-scale", str(args.bin_scale), "--query-block", str(args.query_block)]
+             "--mode", args.mode, "--bin-scale", str(args.bin_scale), "--query-block", str(args.query_block),
+             "--waves-per-simd", str(args.waves_per_simd), "--schedule", args.schedule]
     passes = {"fetch": ["FETCH_SIZE"], "write": ["WRITE_SIZE"],
               "sq": ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_SMEM"]}
     env = dict(os.environ, TMPDIR="/tmp")
@@ -641,7 +638,8 @@ def profile_child(args):
         d = d0.clone()
         torch.cuda.synchronize()
         r.reset_phi_device(d.data_ptr(), None, mode=args.mode, blend="init", band=band,
-                           bin_scale=args.bin_scale, query_block=args.query_block)
+                           bin_scale=args.bin_scale, query_block=args.query_block,
+                           waves_per_simd=args.waves_per_simd, schedule=args.schedule)
         torch.cuda.synchronize()
     r.close()
 
@@ -683,6 +681,8 @@ def main():
     ap.add_argument("--mode", default="triangles", choices=["triangles", "points"])
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--query-block", type=int, default=0)
+    ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--schedule", default="static", choices=["static", "queue"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -736,7 +736,8 @@ def main():
         d_phi.copy_(d_phi0)
         return r.reset_phi_device(d_phi.data_ptr(), None, mode=args.mode, blend="init",
                                   band=band, bin_scale=args.bin_scale, stream=stream,
-                                  query_block=args.query_block)
+                                  query_block=args.query_block, waves_per_simd=args.waves_per_simd,
+                                  schedule=args.schedule)
 
     tuned = None
     if not args.no_tune:
@@ -744,18 +745,21 @@ def main():
 
         def copy(dst, src, nb):
             d_phi.copy_(d_phi0)
-        bs, qb, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
-                                      mode=args.mode, reps=3, sync=torch.cuda.synchronize, copy=copy,
-                                      bin_scales=autotune.MODE_BIN_SCALES[args.mode], query_blocks=(64, 256),
-                                      band=band, stream=stream)
-        args.bin_scale, args.query_block = bs, qb
-        tuned = {"bin_scale": bs, "query_block": qb,
-                 "table_ms": [[a, q, round(t, 3)] for a, q, t in table]}
-        log(f"autotune picked bin_scale {bs}, query_block {qb}")
+        best, table = autotune.tune(r, d_phi.data_ptr(), d_phi0.data_ptr(), d_phi0.numel() * 8,
+                                    mode=args.mode, reps=3, sync=torch.cuda.synchronize, copy=copy,
+                                    bin_scales=autotune.MODE_BIN_SCALES[args.mode], query_blocks=(64, 256),
+                                    band=band, stream=stream)
+        args.bin_scale, args.query_block = best["bin_scale"], best["query_block"]
+        args.waves_per_simd, args.schedule = best["waves_per_simd"], best["schedule"]
+        tuned = dict(best, table_ms=[[a, q, w, sch, round(t, 3)] for a, q, w, sch, t in table],
+                     table_columns=["bin_scale", "query_block", "waves_per_simd", "schedule", "ms"])
+        log(f"autotune picked {best}")
     if args.bin_scale == 0.0:
         args.bin_scale = 1.25 if args.mode == "triangles" else 2.0
     if args.query_block == 0:
         args.query_block = 256
+    if args.waves_per_simd == 0:
+        args.waves_per_simd = 8
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -787,7 +791,9 @@ def main():
 
     accuracy = None
     if not args.no_accuracy:
-        accuracy = accuracy_check(r, m, phi, h, args.mode, band, args.bin_scale, args.query_block, stream)
+        accuracy = accuracy_check(r, m, phi, h, args.mode, band, stream,
+                                  dict(bin_scale=args.bin_scale, query_block=args.query_block,
+                                       waves_per_simd=args.waves_per_simd, schedule=args.schedule))
         log(f"accuracy: {accuracy}")
 
     # the drop-in path with host phi (ft_reinit on the caller's own arrays, as
@@ -798,7 +804,8 @@ def main():
     from fluidtissue_amd.reinit import make_opts
     h_phi = phi.copy()
     h_ifn = np.zeros_like(phi)
-    o = make_opts(args.mode, "init", band, args.bin_scale, False, query_block=args.query_block)
+    o = make_opts(args.mode, "init", band, args.bin_scale, False, query_block=args.query_block,
+                  waves_per_simd=args.waves_per_simd, schedule=args.schedule)
     hc = _lib.FtCounts()
     ms_host = []
     for _ in range(3):
@@ -854,11 +861,13 @@ def main():
                    "nodes": N, "tets": T, "edges": len(m.edges), "crossing_points": P,
                    "triangles": F, "crossing_tets": X, "h": h, "band": band,
                    "clamped_nodes": cnt.n_clamped, "bin_scale": args.bin_scale,
-                   "query_block": args.query_block},
+                   "query_block": args.query_block, "waves_per_simd": args.waves_per_simd,
+                   "schedule": args.schedule},
         "phases_ms": {"marching_tets": ph[0], "index_build": ph[1], "query": ph[2],
                       "normals": ph[3]},
         "roofline": {"kernel": f"k_query_wave<{1 if args.mode == 'triangles' else 0},"
-                               f"{args.query_block}> (nearest-primitive search)",
+                               f"{args.query_block},{args.waves_per_simd if args.mode == 'triangles' else 8}> "
+                               f"(nearest-primitive search, {args.schedule} schedule)",
                      "bound": "valu", "achieved": achieved, "peak": peak, "unit": "GB/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "basis": "achieved = B / query-kernel time, B = 28N + 16T + 20P + 24Tx + 16F "

@@ -25,6 +25,8 @@ FT_DIST_POINTS = 0
 FT_DIST_TRIANGLES = 1
 FT_BLEND_INIT = 0
 FT_BLEND_UPDATE = 1
+FT_SCHED_STATIC = 0
+FT_SCHED_QUEUE = 1
 
 FT_PHI_SPHERE_WAVE = 1
 FT_PHI_STAR = 2
@@ -45,7 +47,9 @@ class FtOpts(C.Structure):
         ("want_nearest", C.c_int32),
         ("query_block", C.c_int32),
         ("want_normals", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("waves_per_simd", C.c_int32),
+        ("schedule", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

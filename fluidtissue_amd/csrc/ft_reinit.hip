@@ -84,7 +84,10 @@ inline unsigned grid1d(int64_t n, int bs = kBS) { return (unsigned)((n + bs - 1)
 //   S_DEEP   query tiles scheduled first (low 32 bits)
 //   S_OVF    traversal stack/budget overflow of the query
 //   S_CAP    a stage total exceeded the capacity its buffers were sized for
-enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_DEEP = 5, S_OVF = 6, S_CAP = 7, S_N = 8 };
+//   S_Q      the query's work-queue counters (9 x u32: deep tiles, then one
+//            per XCD), zeroed per launch
+enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_DEEP = 5, S_OVF = 6, S_CAP = 7,
+            S_Q = 8, S_N = 13 };
 
 // Stage totals of the running reinit as the kernels see them.  Kernels after
 // a scan read the total from the device, so a launch sized for a capacity
@@ -1191,6 +1194,7 @@ struct QueryArgs {
     int probe;                       // greedy first candidate
     const uint64_t* sc;              // stage totals and halt flags
     unsigned xcd_chunk;              // tiles per XCD chunk (0: dispatch order)
+    uint32_t* queue;                 // work-queue schedule: deep counter, then one per XCD (null: static)
 };
 
 // The dispatcher hands workgroup b to XCD b % 8, each XCD with its own L2.
@@ -1461,42 +1465,10 @@ __global__ __launch_bounds__(kBS) void k_tile_list(int64_t n_tiles, const uint32
 // together.  A child is entered when any lane's bound admits it; each lane
 // keeps its own best.  All control flow is wave-uniform (ballots), node and
 // primitive records are read at wave-uniform addresses through scalar loads,
-// the stack is one per wave in LDS.  Launch bound: 8 waves per SIMD, hence at
-// most 64 VGPRs.  Unbounded, the FP64 settle of the epilogue takes 84 and the
-// occupancy drops to 5 (measured 25% slower at 256^3); bounded, it spills a
-// few registers that are touched once per node.
-template <int MODE, int BS>
-__global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
+// the stack is one per wave in LDS.
+template <int MODE>
+__device__ __forceinline__ void search_tile(const QueryArgs& a, int64_t base, int32_t* stack)
 {
-    __shared__ int32_t wstack[BS / 64][kStack];
-    if (halted(a.sc)) return;                         // an earlier stage failed: phi untouched
-    const int w = threadIdx.x >> 6;
-    int64_t base;
-    if (a.tile_list) {
-        // deep tiles on the first Kb workgroups (Kb a multiple of the 8 XCDs),
-        // the rest on the workgroups after them in XCD-chunked order
-        constexpr unsigned wpb = BS / 64;
-        const int64_t n_tiles = a.n_slots / 64;
-        const unsigned K = (unsigned)ftd::cload(a.n_deep, 0);
-        const unsigned Kb = ((K + wpb - 1) / wpb + 7u) & ~7u;
-        const unsigned b = blockIdx.x;
-        int64_t r;
-        if (b < Kb) {
-            r = (int64_t)b * wpb + w;
-            if (r >= K) return;                        // whole wave: padding of the deep workgroups
-        } else {
-            const int64_t nn = n_tiles - K;
-            const unsigned nbn = (unsigned)((nn + wpb - 1) / wpb);
-            if (b - Kb >= nbn) return;                 // whole wave: past the last tile
-            r = (int64_t)xcd_block(b - Kb, nbn, a.xcd_chunk) * wpb + w;
-            if (r >= nn) return;
-            r += K;
-        }
-        base = (int64_t)a.tile_list[r] * 64;
-    } else {
-        base = (int64_t)xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk) * BS + (threadIdx.x & ~63);
-        if (base >= a.n_slots) return;                // whole wave out of range
-    }
     const int64_t slot = base + (threadIdx.x & 63);
     const int32_t node = a.perm[slot];                 // -1: padding lane
     const float px = a.sx[slot], py = a.sy[slot], pz = a.sz[slot];
@@ -1556,12 +1528,12 @@ __global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
             if (m1) {
                 if (c1 < 0) S.test_leaf_uniform(~c1, m1);
                 else if (next < 0) next = c1;
-                else if (sp < kStack) wstack[w][sp++] = c1;
+                else if (sp < kStack) stack[sp++] = c1;
                 else atomicOr(a.overflow, 1u);
             }
             if (next < 0) {
                 if (sp == 0) break;
-                next = wstack[w][--sp];
+                next = stack[--sp];
             }
             cur = __builtin_amdgcn_readfirstlane(next);
         }
@@ -1573,6 +1545,83 @@ __global__ __launch_bounds__(BS, 8) void k_query_wave(QueryArgs a)
     if (node < 0) return;
     if (a.stats) record_stats<MODE>(a, S, node, n_nodes);
     finish_node<MODE>(a, S, node);
+}
+
+// Query schedule of a launch: deep tiles first (kDeepTileH), the rest in
+// XCD-chunked Morton order (xcd_block).  Static: one tile per wave, the
+// deep tiles on the first workgroups (which the dispatcher deals over all
+// XCDs), the others after them.  Work queue (a.queue): a persistent grid
+// whose waves take tiles from counters -- the deep tiles from one shared
+// counter, then the rest from the counter of their XCD (workgroup b runs on
+// XCD b % 8), each XCD's share being its chunks of the static order; a wave
+// whose XCD has none left takes from the other XCDs' counters in turn.
+// Every wave leaves once all nine counters are past their ends.
+// Launch bound: WPS waves per SIMD (8: at most 64 VGPRs).  Unbounded, the
+// FP64 settle of the epilogue takes 84 VGPRs and the occupancy drops to 5
+// (measured 25% slower at 256^3); bounded at 8, it spills a few registers
+// that are touched once per node.
+template <int MODE, int BS, int WPS>
+__global__ __launch_bounds__(BS, WPS) void k_query_wave(QueryArgs a)
+{
+    __shared__ int32_t wstack[BS / 64][kStack];
+    if (halted(a.sc)) return;                         // an earlier stage failed: phi untouched
+    const int w = threadIdx.x >> 6;
+    if (a.queue) {
+        const int64_t n_tiles = a.n_slots / 64;
+        const uint32_t K = (uint32_t)ftd::cload(a.n_deep, 0);
+        const uint32_t nn = (uint32_t)(n_tiles - K);
+        const uint32_t chunk = a.xcd_chunk;
+        const unsigned xcd = blockIdx.x % kXcds;
+        unsigned src = 0;                              // 0: deep counter; 1..8: XCD counters
+        for (;;) {
+            uint32_t got = 0;
+            if (ftd::lane_id() == 0) got = atomicAdd(&a.queue[src == 0 ? 0 : 1 + (xcd + src - 1) % kXcds], 1u);
+            got = (uint32_t)__shfl(got, 0);
+            int64_t r = -1;
+            if (src == 0) {
+                if (got < K) r = got;
+            } else {
+                // the got-th tile of XCD x: its chunks are x, x + 8, x + 16, ...
+                // (increasing in got, so once past nn it stays past)
+                const unsigned x = (xcd + src - 1) % kXcds;
+                const uint64_t t = (uint64_t)(got / chunk) * kXcds * chunk + (uint64_t)x * chunk + got % chunk;
+                if (t < nn) r = (int64_t)K + (int64_t)t;
+            }
+            if (r >= 0) {
+                search_tile<MODE>(a, (int64_t)a.tile_list[r] * 64, wstack[w]);
+                continue;
+            }
+            if (++src > kXcds) break;                  // every counter exhausted
+        }
+        return;
+    }
+    int64_t base;
+    if (a.tile_list) {
+        // deep tiles on the first Kb workgroups (Kb a multiple of the 8 XCDs),
+        // the rest on the workgroups after them in XCD-chunked order
+        constexpr unsigned wpb = BS / 64;
+        const int64_t n_tiles = a.n_slots / 64;
+        const unsigned K = (unsigned)ftd::cload(a.n_deep, 0);
+        const unsigned Kb = ((K + wpb - 1) / wpb + 7u) & ~7u;
+        const unsigned b = blockIdx.x;
+        int64_t r;
+        if (b < Kb) {
+            r = (int64_t)b * wpb + w;
+            if (r >= K) return;                        // whole wave: padding of the deep workgroups
+        } else {
+            const int64_t nn = n_tiles - K;
+            const unsigned nbn = (unsigned)((nn + wpb - 1) / wpb);
+            if (b - Kb >= nbn) return;                 // whole wave: past the last tile
+            r = (int64_t)xcd_block(b - Kb, nbn, a.xcd_chunk) * wpb + w;
+            if (r >= nn) return;
+            r += K;
+        }
+        base = (int64_t)a.tile_list[r] * 64;
+    } else {
+        base = (int64_t)xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk) * BS + (threadIdx.x & ~63);
+        if (base >= a.n_slots) return;                // whole wave out of range
+    }
+    search_tile<MODE>(a, base, wstack[w]);
 }
 
 // ============================================================================
@@ -2536,6 +2585,7 @@ struct GraphState {
 
 struct ft_ctx {
     int device = 0;
+    int cu_count = 256;              // compute units of the device
     hipStream_t stream = nullptr;
     // fields and mesh (field k: nodes [node_off[k], node_off[k+1]), ...)
     int nf = 0;
@@ -3135,6 +3185,8 @@ struct SearchSpec {
     bool want_normals = false;       // Assembler::grad_nodes of phi_ifn after the epilogue
     bool int_ifn = false;            // phi_ifn into the handle's buffer (normals need it)
     int query_block = 0;
+    int waves_per_simd = 8;          // launch bound of the query (triangles: 5..8)
+    bool work_queue = false;         // persistent query over tile counters
 };
 
 // Enqueues the marching tets of one reinit on stream s.  eager: reads the
@@ -3233,6 +3285,32 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
     // a degenerate tet found by k_tris halts the rest on the device; the final
     // readback reports it
     return FT_OK;
+}
+
+// The query kernel for (mode, threads per workgroup, waves per SIMD); points
+// mode has the 8-wave bound only (its epilogue has no FP64 settle to spill).
+template <int MODE, int BS>
+int launch_query_bs(int wps, dim3 g, dim3 b, hipStream_t s, const QueryArgs& q)
+{
+    switch (MODE == 0 ? 8 : wps) {
+    case 5: hipLaunchKernelGGL((k_query_wave<MODE, BS, 5>), g, b, 0, s, q); break;
+    case 6: hipLaunchKernelGGL((k_query_wave<MODE, BS, 6>), g, b, 0, s, q); break;
+    case 7: hipLaunchKernelGGL((k_query_wave<MODE, BS, 7>), g, b, 0, s, q); break;
+    default: hipLaunchKernelGGL((k_query_wave<MODE, BS, 8>), g, b, 0, s, q); break;
+    }
+    return check_launch();
+}
+
+int launch_query(int mode, int qb, int wps, dim3 g, dim3 b, hipStream_t s, const QueryArgs& q)
+{
+    if (mode == 0) {
+        if (qb == 64) return launch_query_bs<0, 64>(wps, g, b, s, q);
+        if (qb == 128) return launch_query_bs<0, 128>(wps, g, b, s, q);
+        return launch_query_bs<0, 256>(wps, g, b, s, q);
+    }
+    if (qb == 64) return launch_query_bs<1, 64>(wps, g, b, s, q);
+    if (qb == 128) return launch_query_bs<1, 128>(wps, g, b, s, q);
+    return launch_query_bs<1, 256>(wps, g, b, s, q);
 }
 
 // Enqueues the bin index over the primitives (crossing points c->ifp, or
@@ -3352,19 +3430,19 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     if (c->n_slots > 0) {
         const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
         q.xcd_chunk = qb == 64 ? kXcdChunk64 : kXcdChunk;
+        const int wps = mode == 0 ? 8 : sp.waves_per_simd;
         // with a tile list: up to 7 padding workgroups after the deep ones,
         // one more for the split of the remainder
-        const dim3 g(grid1d(c->n_slots, qb) + (q.tile_list ? 8u : 0u)), b(qb);
-        if (mode == 0) {
-            if (qb == 64) hipLaunchKernelGGL((k_query_wave<0, 64>), g, b, 0, s, q);
-            else if (qb == 128) hipLaunchKernelGGL((k_query_wave<0, 128>), g, b, 0, s, q);
-            else hipLaunchKernelGGL((k_query_wave<0, 256>), g, b, 0, s, q);
-        } else {
-            if (qb == 64) hipLaunchKernelGGL((k_query_wave<1, 64>), g, b, 0, s, q);
-            else if (qb == 128) hipLaunchKernelGGL((k_query_wave<1, 128>), g, b, 0, s, q);
-            else hipLaunchKernelGGL((k_query_wave<1, 256>), g, b, 0, s, q);
+        dim3 g(grid1d(c->n_slots, qb) + (q.tile_list ? 8u : 0u)), b(qb);
+        if (sp.work_queue && q.tile_list) {
+            // persistent: as many waves as the device holds at this launch
+            // bound (4 SIMDs per CU), whole workgroups, at least one per XCD
+            const int64_t waves = (int64_t)c->cu_count * 4 * wps;
+            g = dim3((unsigned)std::max<int64_t>(kXcds, waves / (qb / 64)));
+            q.queue = reinterpret_cast<uint32_t*>(sc + S_Q);
+            FT_TRY(fill<uint32_t>(s, q.queue, 9, 0u));
         }
-        FT_TRY(check_launch());
+        FT_TRY(launch_query(mode, qb, wps, g, b, s, q));
     }
     if (timed) (void)hipEventRecord(c->ev[3], s);
     if (sp.want_normals && c->N > 0) {
@@ -3463,6 +3541,8 @@ SearchSpec reinit_search_spec(ft_ctx* c, double* phi, double* phi_ifn, const ft_
     sp.want_normals = o->want_normals != 0;
     sp.int_ifn = sp.want_normals && !phi_ifn;
     sp.query_block = o->query_block;
+    sp.waves_per_simd = o->waves_per_simd > 0 ? o->waves_per_simd : 8;
+    sp.work_queue = o->schedule == FT_SCHED_QUEUE;
     return sp;
 }
 
@@ -3479,6 +3559,10 @@ int validate_opts(const ft_opts* o)
         return ft::set_error(FT_ERR_ARG, "bin_scale must be >= 0");
     if (o->query_block != 0 && o->query_block != 64 && o->query_block != 128 && o->query_block != 256)
         return ft::set_error(FT_ERR_ARG, "query_block must be 0, 64, 128 or 256");
+    if (o->waves_per_simd != 0 && (o->waves_per_simd < 5 || o->waves_per_simd > 8))
+        return ft::set_error(FT_ERR_ARG, "waves_per_simd must be 0 or 5..8");
+    if (o->schedule != FT_SCHED_STATIC && o->schedule != FT_SCHED_QUEUE)
+        return ft::set_error(FT_ERR_ARG, "unknown query schedule %d", o->schedule);
     return FT_OK;
 }
 
@@ -3554,7 +3638,8 @@ bool same_opts(const ft_opts& a, const ft_opts& b)
 {
     return a.mode == b.mode && a.blend == b.blend && a.w_phi == b.w_phi && a.w_ifn == b.w_ifn &&
            a.band == b.band && a.bin_scale == b.bin_scale && a.want_nearest == b.want_nearest &&
-           a.query_block == b.query_block && a.want_normals == b.want_normals;
+           a.query_block == b.query_block && a.want_normals == b.want_normals &&
+           a.waves_per_simd == b.waves_per_simd && a.schedule == b.schedule;
 }
 
 void drop_graph(ft_ctx* c)
@@ -3791,6 +3876,9 @@ int ft_create(int32_t device, ft_handle* out)
     FT_HIP(hipSetDevice(device));
     ft_ctx* c = new ft_ctx();
     c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->cu_count = cus;
     // A blocking stream: a call that passes no stream (NULL, which is also
     // the address of the null stream) runs on this one, and it must still see
     // the caller's work on the null stream before it (a copy into phi) and

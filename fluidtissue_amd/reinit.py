@@ -94,8 +94,12 @@ def phi_field(kind, xyz, seed=0):
     return phi
 
 
+SCHEDULES = {"static": _lib.FT_SCHED_STATIC, "queue": _lib.FT_SCHED_QUEUE}
+
+
 def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest=False,
-              w_phi=0.75, w_ifn=0.25, query_block=0, want_normals=False):
+              w_phi=0.75, w_ifn=0.25, query_block=0, want_normals=False, waves_per_simd=0,
+              schedule="static"):
     o = _lib.FtOpts()
     lib.ft_opts_default(C.byref(o))
     o.mode = MODES[mode]
@@ -107,6 +111,8 @@ def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest
     o.w_ifn = float(w_ifn)
     o.query_block = int(query_block)
     o.want_normals = 1 if want_normals else 0
+    o.waves_per_simd = int(waves_per_simd)
+    o.schedule = SCHEDULES[schedule]
     return o
 
 
@@ -170,12 +176,13 @@ class Reinit(_Handle):
         self.n = xyz.shape[0]
 
     def reset_phi(self, phi, mode="points", blend="init", band=0.0, bin_scale=0.0,
-                  want_ifn=True, want_nearest=False, query_block=0, want_normals=False):
+                  want_ifn=True, want_nearest=False, query_block=0, want_normals=False,
+                  waves_per_simd=0, schedule="static"):
         """Returns (phi_out, phi_ifn, counts); phi is not modified."""
         out = np.array(phi, dtype=np.float64, copy=True)
         ifn = np.empty_like(out) if want_ifn else None
         o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
-                      want_normals=want_normals)
+                      want_normals=want_normals, waves_per_simd=waves_per_simd, schedule=schedule)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit(self._h, _ptr(out), _ptr(ifn), C.byref(o), C.byref(cnt)))
@@ -183,10 +190,10 @@ class Reinit(_Handle):
 
     def reset_phi_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
                          bin_scale=0.0, want_nearest=False, stream=None, query_block=0,
-                         want_normals=False):
+                         want_normals=False, waves_per_simd=0, schedule="static"):
         """Device-resident phi (raw device addresses, e.g. tensor.data_ptr())."""
         o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
-                      want_normals=want_normals)
+                      want_normals=want_normals, waves_per_simd=waves_per_simd, schedule=schedule)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit_device(self._h, C.c_void_p(phi_ptr),
@@ -195,11 +202,13 @@ class Reinit(_Handle):
         return cnt
 
     def reset_phi_device_noalloc(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
-                                 bin_scale=0.0, stream=None, query_block=0):
+                                 bin_scale=0.0, stream=None, query_block=0, waves_per_simd=0,
+                                 schedule="static"):
         """reset_phi_device with no device allocation: every workspace comes
         from the library's fixed scratch (FT_NOALLOC_SCRATCH_BYTES per GPU).
         No nearest indices, normals or interface are kept."""
-        o = make_opts(mode, blend, band, bin_scale, False, query_block=query_block)
+        o = make_opts(mode, blend, band, bin_scale, False, query_block=query_block,
+                      waves_per_simd=waves_per_simd, schedule=schedule)
         cnt = _lib.FtCounts()
         self.counts = cnt
         check(lib.ft_reinit_device_noalloc(self._h, C.c_void_p(phi_ptr),
@@ -269,9 +278,9 @@ class Reinit(_Handle):
 
     def reset_phi_batch_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
                                bin_scale=0.0, want_nearest=False, stream=None, query_block=0,
-                               want_normals=False):
+                               want_normals=False, waves_per_simd=0, schedule="static"):
         o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
-                      want_normals=want_normals)
+                      want_normals=want_normals, waves_per_simd=waves_per_simd, schedule=schedule)
         rc = lib.ft_reinit_batch(self._h, C.c_void_p(phi_ptr),
                                  C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
                                  C.c_void_p(stream) if stream else None)
@@ -293,11 +302,11 @@ class Reinit(_Handle):
 
     def reset_phi_graph_device(self, phi_ptr, ifn_ptr=None, mode="points", blend="init", band=0.0,
                                bin_scale=0.0, want_nearest=False, stream=None, query_block=0,
-                               want_normals=False):
+                               want_normals=False, waves_per_simd=0, schedule="static"):
         """reset_phi_batch_device through one captured HIP graph per call
         (captured on the first call for these buffers and options)."""
         o = make_opts(mode, blend, band, bin_scale, want_nearest, query_block=query_block,
-                      want_normals=want_normals)
+                      want_normals=want_normals, waves_per_simd=waves_per_simd, schedule=schedule)
         rc = lib.ft_reinit_graph(self._h, C.c_void_p(phi_ptr),
                                  C.c_void_p(ifn_ptr) if ifn_ptr else None, C.byref(o), None,
                                  C.c_void_p(stream) if stream else None)

@@ -80,8 +80,17 @@ typedef struct ft_opts {
     int32_t want_normals;/* 1: also the level-set normals of phi_ifn, as
                             LevelSet::resetPhi_ ends (LevelSet.cpp:451-453,
                             Assembler::grad_nodes, Assembler.cpp:11-71)        */
-    int32_t reserved[4];
+    int32_t waves_per_simd; /* launch bound of the triangle-mode query: 5..8
+                            waves per SIMD (register budget against
+                            occupancy); 0 = 8                                  */
+    int32_t schedule;    /* FT_SCHED_STATIC: one tile per wave, deep tiles on
+                            the first workgroups; FT_SCHED_QUEUE: persistent
+                            waves taking tiles from per-XCD counters          */
+    int32_t reserved[2];
 } ft_opts;
+
+#define FT_SCHED_STATIC    0
+#define FT_SCHED_QUEUE     1
 
 typedef struct ft_counts {
     int64_t n_ifp;       /* P: crossing points                                 */

@@ -346,3 +346,41 @@ def test_released_mesh_id_reused_by_a_new_mesh():
     s.close()
     b.close()
     check(lib.ft_mesh_release(a2))
+
+
+@pytest.mark.parametrize("mode", ["triangles", "points"])
+def test_launch_bounds_and_work_queue_give_the_same_bits(mode):
+    """Every launch bound (5..8 waves per SIMD) and the work-queue schedule
+    give the default launch's results bit for bit: a single jittered field
+    (deep tiles first) and a batch through the captured graph, replayed."""
+    import torch
+    m, phi, h = workload("spheres_96")
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    ref = r.reset_phi(phi, mode=mode, blend="update", want_nearest=True)
+    ref_near = r.nearest()
+    for wps in (5, 6, 7, 8):
+        for sch in ("static", "queue"):
+            out, ifn, cnt = r.reset_phi(phi, mode=mode, blend="update", want_nearest=True,
+                                        waves_per_simd=wps, schedule=sch, query_block=64 if wps % 2 else 256)
+            assert np.array_equal(out.view(np.uint64), ref[0].view(np.uint64)), (wps, sch)
+            assert np.array_equal(ifn.view(np.uint64), ref[1].view(np.uint64)), (wps, sch)
+            assert np.array_equal(r.nearest(), ref_near), (wps, sch)
+    r.close()
+    sizes = [20, 33, 27, 41, 18]
+    b = fa.Reinit(0)
+    b.set_kuhn_batch(sizes, jitter=0.2, seed=4)
+    mb = b.mesh()
+    phib = fa.phi_field("spheres", mb.xyz, seed=5)
+    d0 = torch.from_numpy(phib).cuda()
+    outs = []
+    for sch, reps in (("static", 1), ("queue", 3)):
+        for _ in range(reps):
+            d = d0.clone()
+            torch.cuda.synchronize()
+            b.reset_phi_graph_device(d.data_ptr(), None, mode=mode, schedule=sch, waves_per_simd=6)
+            torch.cuda.synchronize()
+            outs.append(d.cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
+    b.close()

@@ -129,11 +129,13 @@ def test_autotune_picks_fastest_and_caches():
     clock = [0.0]
     calls = []
     cost = {(1.0, 64): 3.0, (1.0, 256): 2.0, (1.5, 64): 1.0, (1.5, 256): 4.0}
+    wcost = {5: 0.5, 6: 0.2, 7: 0.4, 8: 0.3}
 
     class FakeHandle:
-        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block, stream=None):
-            calls.append((bin_scale, query_block))
-            clock[0] += cost[(bin_scale, query_block)]
+        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block, waves_per_simd,
+                                   schedule, stream=None):
+            calls.append((bin_scale, query_block, waves_per_simd, schedule))
+            clock[0] += cost[(bin_scale, query_block)] + wcost[waves_per_simd] + (0.05 if schedule == "queue" else 0)
 
     restores = []
     orig = autotune.time.perf_counter
@@ -141,13 +143,17 @@ def test_autotune_picks_fastest_and_caches():
     try:
         args = dict(sync=lambda: None, copy=lambda d, s, n: restores.append((d, s, n)),
                     signature=("test", 1), bin_scales=(1.0, 1.5), query_blocks=(64, 256), reps=2)
-        bs, qb, table = autotune.tune(FakeHandle(), 11, 22, 33, **args)
-        assert (bs, qb) == (1.5, 64)
-        assert [(b, q) for b, q, _ in table] == [(1.0, 64), (1.0, 256), (1.5, 64), (1.5, 256)]
-        assert [ms for _, _, ms in table] == [3000.0, 2000.0, 1000.0, 4000.0]
-        assert len(calls) == 4 * 3 and restores == [(11, 22, 33)] * 12   # input restored each run
-        assert autotune.tune(FakeHandle(), 11, 22, 33, **args)[:2] == (1.5, 64)
-        assert len(calls) == 12                                        # cached per signature
+        best, table = autotune.tune(FakeHandle(), 11, 22, 33, **args)
+        assert (best["bin_scale"], best["query_block"]) == (1.5, 64)
+        assert best["waves_per_simd"] == 6 and best["schedule"] == "static"
+        # stage 1: bin x block at 8 waves; stage 2: waves 5..7 (8 already run); stage 3: the queue
+        assert [(b, q, w, s) for b, q, w, s, _ in table] == [
+            (1.0, 64, 8, "static"), (1.0, 256, 8, "static"), (1.5, 64, 8, "static"), (1.5, 256, 8, "static"),
+            (1.5, 64, 5, "static"), (1.5, 64, 6, "static"), (1.5, 64, 7, "static"), (1.5, 64, 6, "queue")]
+        assert abs(table[0][4] - 3300.0) < 1e-6 and abs(best["ms"] - 1200.0) < 1e-6
+        assert len(calls) == 8 * 3 and restores == [(11, 22, 33)] * 24   # input restored each run
+        assert autotune.tune(FakeHandle(), 11, 22, 33, **args)[0] == best
+        assert len(calls) == 24                                        # cached per signature
     finally:
         autotune.time.perf_counter = orig
         autotune._cache.pop(("triangles", 0.0, ("test", 1)), None)
@@ -155,18 +161,22 @@ def test_autotune_picks_fastest_and_caches():
 
 def test_autotune_default_candidates_follow_the_mode():
     """Without bin_scales the tuner tries the neighbourhood of each mode's
-    measured optimum (points 2.0-2.5, triangles 1.25 node spacings)."""
+    measured optimum (points 2.0-2.5, triangles 1.25 node spacings), and
+    launch bounds only in triangles mode."""
     from fluidtissue_amd import autotune
     tried = []
 
     class FakeHandle:
-        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block, stream=None):
-            tried.append((mode, bin_scale))
+        def reset_phi_batch_device(self, phi, ifn, mode, band, bin_scale, query_block, waves_per_simd,
+                                   schedule, stream=None):
+            tried.append((mode, bin_scale, waves_per_simd))
 
     for mode in ("points", "triangles"):
         autotune.tune(FakeHandle(), 1, 2, 3, mode=mode, reps=1, sync=lambda: None,
                       copy=lambda d, s, n: None, query_blocks=(64,))
-    pts = sorted({b for m, b in tried if m == "points"})
-    tri = sorted({b for m, b in tried if m == "triangles"})
+    pts = sorted({b for m, b, _ in tried if m == "points"})
+    tri = sorted({b for m, b, _ in tried if m == "triangles"})
     assert pts == sorted(autotune.MODE_BIN_SCALES["points"]) and 2.0 in pts and 2.5 in pts
     assert tri == sorted(autotune.MODE_BIN_SCALES["triangles"]) and 1.25 in tri
+    assert {w for m, _, w in tried if m == "points"} == {8}
+    assert {w for m, _, w in tried if m == "triangles"} == {5, 6, 7, 8}
