@@ -40,6 +40,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -101,6 +102,61 @@ __device__ __forceinline__ int64_t cnt_leaves(const uint64_t* sc) { return (int6
 __device__ __forceinline__ bool halted(const uint64_t* sc)
 {
     return sc[S_CAP] != 0ull || sc[S_ERR] != ~0ull;
+}
+
+// Connectivity of the resident fields as the marching-tets kernels see it:
+// one concatenated array per kind (global indices), or, for a batch of
+// resident meshes, each field's own arrays read in place (field-local
+// indices, to which the field's node or edge offset is added), so that a
+// batch is assembled without copying its tets, t2e and edges.
+struct MeshView {
+    const int4* tets;                // concatenated (null: per field)
+    const int2* t2e3;                // t2e rows as 3 int2
+    const int2* edges;
+    const int4* const* f_tets;       // per field
+    const int2* const* f_t2e3;
+    const int2* const* f_edges;
+    const int64_t* node_off;         // device offsets, nf + 1 each
+    const int64_t* tet_off;
+    const int64_t* edge_off;
+    int nf;
+};
+
+// Field of tet t (t < T) and of edge e: binary search over the offsets.
+__device__ __forceinline__ int mv_tet_field(const MeshView& m, int64_t t)
+{
+    return ftd::field_of(m.tet_off, m.nf, t);
+}
+__device__ __forceinline__ int mv_edge_field(const MeshView& m, int64_t e)
+{
+    return ftd::field_of(m.edge_off, m.nf, e);
+}
+// Tet t of field k (k = its field) with global node ids.
+__device__ __forceinline__ int4 mv_tet(const MeshView& m, int64_t t, int k)
+{
+    if (m.tets) return m.tets[t];
+    int4 q = m.f_tets[k][t - m.tet_off[k]];
+    const int32_t a = (int32_t)m.node_off[k];
+    q.x += a; q.y += a; q.z += a; q.w += a;
+    return q;
+}
+// Row j (0..2) of tet t's t2e, as two global edge ids.
+__device__ __forceinline__ int2 mv_t2e(const MeshView& m, int64_t t, int j, int k)
+{
+    if (m.t2e3) return m.t2e3[3 * t + j];
+    int2 r = m.f_t2e3[k][3 * (t - m.tet_off[k]) + j];
+    const int32_t a = (int32_t)m.edge_off[k];
+    r.x += a; r.y += a;
+    return r;
+}
+// Edge e of field k with global node ids.
+__device__ __forceinline__ int2 mv_edge(const MeshView& m, int64_t e, int k)
+{
+    if (m.edges) return m.edges[e];
+    int2 ab = m.f_edges[k][e - m.edge_off[k]];
+    const int32_t a = (int32_t)m.node_off[k];
+    ab.x += a; ab.y += a;
+    return ab;
 }
 
 // ============================================================================
@@ -308,17 +364,30 @@ __global__ __launch_bounds__(kBS) void k_node_signs(const double* phi, int64_t n
 // two triangles (t2bits, a subset), and blk[group] = (crossing tets << 32) |
 // triangles.
 constexpr int kClassifyGroups = 2;
-__global__ __launch_bounds__(kBS) void k_classify(const int4* tets, int64_t T,
+__global__ __launch_bounds__(kBS) void k_classify(MeshView mv, int64_t T,
                                                   const uint64_t* sbits, uint64_t* tbits,
                                                   uint64_t* t2bits, uint64_t* blk)
 {
     __shared__ uint64_t part[kClassifyGroups][kBS / 64];
+    __shared__ int kf0;
     int4 q[kClassifyGroups];
     int64_t t[kClassifyGroups];
+    // per-field arrays: the block's first field by binary search, then each
+    // thread steps forward over the few field ends inside the block
+    int k = 0;
+    if (!mv.tets) {
+        if (threadIdx.x == 0) kf0 = mv_tet_field(mv, std::min<int64_t>(T - 1, (int64_t)blockIdx.x * kClassifyGroups * kBS));
+        __syncthreads();
+        k = kf0;
+    }
 #pragma unroll
     for (int g = 0; g < kClassifyGroups; g++) {
         t[g] = ((int64_t)blockIdx.x * kClassifyGroups + g) * kBS + threadIdx.x;
-        if (t[g] < T) q[g] = tets[t[g]];
+        if (t[g] < T) {
+            if (!mv.tets)
+                while (t[g] >= mv.tet_off[k + 1]) k++;
+            q[g] = mv_tet(mv, t[g], k);
+        }
     }
 #pragma unroll
     for (int g = 0; g < kClassifyGroups; g++) {
@@ -401,8 +470,8 @@ __device__ __forceinline__ void flag_error(unsigned long long* err, int code, in
 // check the count (3 or 4, and consistent with the node signs used for the
 // triangle offsets: 4 exactly for the tets of the two-triangle bitmap t2bits),
 // mark them in the edge bitmap.
-__global__ __launch_bounds__(kBS) void k_xtet(const uint64_t* t2bits, const int2* t2e3,
-                                              const int2* edges, const uint64_t* sbits,
+__global__ __launch_bounds__(kBS) void k_xtet(const uint64_t* t2bits, MeshView mv,
+                                              const uint64_t* sbits,
                                               const int32_t* xtet, uint64_t* sc,
                                               unsigned long long* ebits, int4* xedges)
 {
@@ -411,13 +480,14 @@ __global__ __launch_bounds__(kBS) void k_xtet(const uint64_t* t2bits, const int2
     if (x >= cnt_xtets(sc)) return;
     unsigned long long* err = (unsigned long long*)(sc + S_ERR);
     const int64_t t = xtet[x];
-    const int2 r0 = t2e3[3 * t], r1 = t2e3[3 * t + 1], r2 = t2e3[3 * t + 2];
+    const int k = mv.t2e3 ? 0 : mv_tet_field(mv, t);
+    const int2 r0 = mv_t2e(mv, t, 0, k), r1 = mv_t2e(mv, t, 1, k), r2 = mv_t2e(mv, t, 2, k);
     const int te[6] = {r0.x, r0.y, r1.x, r1.y, r2.x, r2.y};
     int ce[4] = {-1, -1, -1, -1};
     int n = 0;
 #pragma unroll
     for (int j = 0; j < 6; j++) {
-        const int2 ab = edges[te[j]];
+        const int2 ab = mv_edge(mv, te[j], k);          // a tet's edges are its field's
         if (ftd::get_bit(sbits, ab.x) != ftd::get_bit(sbits, ab.y)) {
             if (n < 4) ce[n] = te[j];
             n++;
@@ -464,7 +534,7 @@ __device__ __forceinline__ int nth_set_bit(uint64_t w, uint32_t n)
 // interpolation, no fma; marks both edge nodes as retained
 // (LevelSet.cpp:443-447).
 __global__ __launch_bounds__(kBS) void k_points(const uint64_t* ebits, const uint32_t* eprefix,
-                                                int64_t nwords, const int2* edges,
+                                                int64_t nwords, MeshView mv,
                                                 const float* xyz, const double* phi,
                                                 float* ifp, int32_t* ifp2e,
                                                 unsigned long long* retain, const uint64_t* sc)
@@ -492,7 +562,7 @@ __global__ __launch_bounds__(kBS) void k_points(const uint64_t* ebits, const uin
         if (k >= total) continue;
         const int64_t e = (w0 + L) * 64 + nth_set_bit(wl, k - ex);
         const uint32_t r = r0 + k;
-        const int2 ab = edges[e];
+        const int2 ab = mv_edge(mv, e, mv.edges ? 0 : mv_edge_field(mv, e));
         const double pa = phi[ab.x], pb = phi[ab.y];
         double t = 0.5;
         const double div = pa - pb;
@@ -538,7 +608,7 @@ __device__ __forceinline__ void emit_tri(int32_t n0, int32_t n1, int32_t n2, int
     tri_norm[f] = sqrtf(sq);
 }
 
-__global__ __launch_bounds__(kBS) void k_tris(const int4* tets, const float* xyz,
+__global__ __launch_bounds__(kBS) void k_tris(MeshView mv, const float* xyz,
                                               const double* phi, const int32_t* xtet,
                                               const int32_t* xtri_off, const int4* xedges,
                                               const uint64_t* ebits,
@@ -554,7 +624,7 @@ __global__ __launch_bounds__(kBS) void k_tris(const int4* tets, const float* xyz
     if_elems[x] = (int32_t)t;
     const int4 ce = xedges[x];
     if (ce.x < 0) return;                                  // flagged by k_xtet
-    const int4 q = tets[t];
+    const int4 q = mv_tet(mv, t, mv.tets ? 0 : mv_tet_field(mv, t));
     const int nd[4] = {q.x, q.y, q.z, q.w};
     int k = 0;
     while (k < 4 && fabs(phi[nd[k]]) <= (double)FLT_MIN) k++;
@@ -2534,18 +2604,40 @@ __global__ __launch_bounds__(kBS) void k_kuhn_tets(KuhnField f, int4* tets, int3
 
 // Localised copies of index arrays (local -> global): mesh uploads and batch
 // assembly from resident meshes.  Negative entries (slot padding) stay.
-__global__ __launch_bounds__(kBS) void k_add_offset_pos(const int32_t* src, int64_t n, int64_t add,
-                                                        int32_t* dst)
-{
-    const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (i < n) dst[i] = src[i] < 0 ? src[i] : (int32_t)(src[i] + add);
-}
-
 __global__ __launch_bounds__(kBS) void k_add_offset(const int32_t* src, int64_t n, int64_t add,
                                                     int32_t* dst)
 {
     const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (i < n) dst[i] = (int32_t)(src[i] + add);
+}
+
+// Batch assembly: the concatenation of per-field pieces into one array in
+// one launch.  Piece p fills dst[off[p], off[p+1]) from src[p] (element
+// e - off[p]) plus add[p], or with the constant add[p] when src[p] is null;
+// with keep_neg, negative source values (padding slots) are kept as they
+// are.  A block copies kGatherChunk consecutive elements; it finds its first
+// piece by binary search and its threads step to the next piece as they
+// cross a boundary.  32-bit words: float arrays copy bit for bit (add 0).
+constexpr int kGatherChunk = kBS * 8;
+__global__ __launch_bounds__(kBS) void k_gather(const int32_t* const* src, const int64_t* off,
+                                                const int64_t* add, int np, int keep_neg, int32_t* dst)
+{
+    __shared__ int p0;
+    const int64_t total = off[np];
+    const int64_t c0 = (int64_t)blockIdx.x * kGatherChunk;
+    if (c0 >= total) return;
+    if (threadIdx.x == 0) p0 = ftd::field_of(off, np, c0);
+    __syncthreads();
+    int p = p0;
+    for (int j = 0; j < kGatherChunk / kBS; j++) {
+        const int64_t e = c0 + (int64_t)j * kBS + threadIdx.x;
+        if (e >= total) break;
+        while (e >= off[p + 1]) p++;
+        const int32_t* sp = src[p];
+        if (!sp) { dst[e] = (int32_t)add[p]; continue; }
+        const int32_t v = sp[e - off[p]];
+        dst[e] = (keep_neg && v < 0) ? v : (int32_t)(v + add[p]);
+    }
 }
 
 // ============================================================================
@@ -2597,6 +2689,14 @@ struct ft_ctx {
     float h_est = 0.0f;
     DevBuf xyz, tets, edges, t2e, perm, sx, sy, sz, tile_field;
     DevBuf tile_sched;               // per query tile: deep flag / scan, then the launch order
+    DevBuf gather_tab;               // piece tables of the batch assembly (ft_set_batch_meshes)
+    // A batch of resident meshes reads their tets, t2e and edges in place
+    // (virt): field_tab holds the per-field pointers, field_meshes keeps the
+    // meshes alive while the batch refers to them.  Operations that need the
+    // concatenated arrays copy them once (materialize_conn).
+    bool virt = false;
+    DevBuf field_tab;
+    std::vector<std::shared_ptr<ft_ctx>> field_meshes;
     DevBuf d_node_off, d_tet_off, d_edge_off, d_slot_off;
     // per reinit
     DevBuf phi_tmp, phi_ifn_tmp;
@@ -2653,10 +2753,13 @@ struct ft_ctx {
 
 namespace {
 
+void drop_graph(ft_ctx* c);
+int materialize_conn(ft_ctx* h);
+
 std::vector<DevBuf*> all_buffers(ft_ctx* h)
 {
     return {&h->xyz, &h->tets, &h->edges, &h->t2e, &h->perm, &h->sx, &h->sy, &h->sz,
-            &h->tile_field, &h->tile_sched, &h->d_node_off, &h->d_tet_off, &h->d_edge_off, &h->d_slot_off,
+            &h->tile_field, &h->tile_sched, &h->gather_tab, &h->field_tab, &h->d_node_off, &h->d_tet_off, &h->d_edge_off, &h->d_slot_off,
             &h->fcounts, &h->fclamp, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots,
             &h->phi_tmp, &h->phi_ifn_tmp, &h->sbits, &h->retain, &h->tbits, &h->blk,
             &h->xtet, &h->xtri_off, &h->xedges, &h->ebits, &h->eprefix, &h->ifp,
@@ -2906,6 +3009,8 @@ int set_fields(ft_ctx* c, int nf, const int64_t* node_off, const int64_t* tet_of
                const int64_t* edge_off)
 {
     c->batch_meshes.clear();
+    c->virt = false;
+    c->field_meshes.clear();
     c->layout_gen++;
     c->nf = nf;
     c->node_off.assign(node_off, node_off + nf + 1);
@@ -3150,6 +3255,7 @@ int stage_error(uint64_t ev, ft_counts* cnt)
 // captured graph: it synchronises).
 int build_node_tets(ft_ctx* c, hipStream_t s)
 {
+    FT_TRY(materialize_conn(c));
     if (c->n2t_layout == c->layout_gen) return FT_OK;
     const int64_t N = c->N, T = c->T;
     FT_TRY(ensure(c->n2t_off, sizeof(uint64_t) * (N + 1)));
@@ -3189,6 +3295,98 @@ struct SearchSpec {
     bool work_queue = false;         // persistent query over tile counters
 };
 
+// Copies of the resident meshes ms into the handle's concatenated arrays,
+// one gather launch per array over all fields: the node coordinates and
+// query slots (perm with padding kept at -1, sx, sy, sz as bit copies, the
+// tile -> field map as constants), or with conn the tets, t2e and edges
+// (offset to global ids).  Their piece tables go up in one copy.
+int gather_meshes(ft_ctx* h, const std::vector<std::shared_ptr<ft_ctx>>& ms, hipStream_t s, bool conn)
+{
+    enum { G_XYZ, G_PERM, G_SX, G_SY, G_SZ, G_TILE, G_TET, G_T2E, G_EDGE, G_N };
+    const int g0 = conn ? G_TET : G_XYZ, g1 = conn ? G_N : G_TET;
+    const int nf = (int)ms.size();
+    const size_t tab = (size_t)(3 * nf + 1);             // src pointers (nf), offsets (nf+1), adds (nf)
+    std::vector<int64_t> host((size_t)(g1 - g0) * tab);
+    for (int g = g0; g < g1; g++) {
+        int64_t* t = host.data() + (g - g0) * tab;
+        const int32_t** srcp = reinterpret_cast<const int32_t**>(t);
+        int64_t* off = t + nf;
+        int64_t* add = t + 2 * nf + 1;
+        off[0] = 0;
+        for (int k = 0; k < nf; k++) {
+            const ft_ctx* m = ms[k].get();
+            int64_t cnt = 0, a = 0;
+            const void* src = nullptr;
+            switch (g) {
+            case G_XYZ: src = m->xyz.p; cnt = 3 * m->N; break;
+            case G_PERM: src = m->perm.p; cnt = m->n_slots; a = h->node_off[k]; break;
+            case G_SX: src = m->sx.p; cnt = m->n_slots; break;
+            case G_SY: src = m->sy.p; cnt = m->n_slots; break;
+            case G_SZ: src = m->sz.p; cnt = m->n_slots; break;
+            case G_TILE: src = nullptr; cnt = m->n_slots / 64; a = k; break;
+            case G_TET: src = m->tets.p; cnt = 4 * m->T; a = h->node_off[k]; break;
+            case G_T2E: src = m->t2e.p; cnt = 6 * m->T; a = h->edge_off[k]; break;
+            default: src = m->edges.p; cnt = 2 * m->E; a = h->node_off[k]; break;
+            }
+            srcp[k] = static_cast<const int32_t*>(src);
+            off[k + 1] = off[k] + cnt;
+            add[k] = a;
+        }
+    }
+    FT_TRY(ensure(h->gather_tab, sizeof(int64_t) * host.size()));
+    FT_HIP(hipMemcpyAsync(h->gather_tab.p, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice, s));
+    void* dsts[G_N] = {h->xyz.p, h->perm.p, h->sx.p, h->sy.p, h->sz.p, h->tile_field.p, h->tets.p, h->t2e.p,
+                       h->edges.p};
+    for (int g = g0; g < g1; g++) {
+        const int64_t* t = h->gather_tab.as<int64_t>() + (g - g0) * tab;
+        const int64_t total = host[(g - g0) * tab + 2 * nf];
+        if (total == 0) continue;
+        hipLaunchKernelGGL(k_gather, dim3(grid1d(total, kGatherChunk)), dim3(kBS), 0, s,
+                           reinterpret_cast<const int32_t* const*>(t), t + nf, t + 2 * nf + 1, nf,
+                           g == G_PERM ? 1 : 0, static_cast<int32_t*>(dsts[g]));
+    }
+    FT_TRY(check_launch());
+    // the host tables must outlive their (possibly staged) copy
+    FT_HIP(hipStreamSynchronize(s));
+    return FT_OK;
+}
+
+// The concatenated tets, t2e and edges of a batch of resident meshes, for the
+// operations that index them globally (normals, curvature, surface tension,
+// advection, ft_get_mesh); a no-op for any other handle.
+int materialize_conn(ft_ctx* h)
+{
+    if (!h->virt) return FT_OK;
+    drop_graph(h);                   // captured with the per-field view
+    FT_TRY(ensure(h->tets, 16 * std::max<int64_t>(1, h->T)));
+    FT_TRY(ensure(h->edges, 8 * std::max<int64_t>(1, h->E)));
+    FT_TRY(ensure(h->t2e, 24 * std::max<int64_t>(1, h->T)));
+    FT_TRY(gather_meshes(h, h->field_meshes, h->stream, true));
+    h->virt = false;
+    h->field_meshes.clear();
+    return FT_OK;
+}
+
+MeshView mesh_view(ft_ctx* c)
+{
+    MeshView m{};
+    m.nf = c->nf;
+    m.node_off = c->d_node_off.as<int64_t>();
+    m.tet_off = c->d_tet_off.as<int64_t>();
+    m.edge_off = c->d_edge_off.as<int64_t>();
+    if (!c->virt) {
+        m.tets = c->tets.as<int4>();
+        m.t2e3 = c->t2e.as<int2>();
+        m.edges = c->edges.as<int2>();
+    } else {
+        const void* const* tab = static_cast<const void* const*>(c->field_tab.p);
+        m.f_tets = reinterpret_cast<const int4* const*>(tab);
+        m.f_t2e3 = reinterpret_cast<const int2* const*>(tab + c->nf);
+        m.f_edges = reinterpret_cast<const int2* const*>(tab + 2 * c->nf);
+    }
+    return m;
+}
+
 // Enqueues the marching tets of one reinit on stream s.  eager: reads the
 // stage totals back after the two counting scans (sizing the buffers exactly,
 // and stopping early on a failed stage).  !eager (graph capture): no host
@@ -3216,7 +3414,7 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
     FT_TRY(fill<uint64_t>(s, sc + S_EDGES, 1, 0ull));   // the edge scan writes the low 32 bits
     if (T > 0)
         hipLaunchKernelGGL(k_classify, dim3((unsigned)((nblk + kClassifyGroups - 1) / kClassifyGroups)),
-                           dim3(kBS), 0, s, c->tets.as<int4>(), T, sbits, c->tbits.as<uint64_t>(),
+                           dim3(kBS), 0, s, mesh_view(c), T, sbits, c->tbits.as<uint64_t>(),
                            c->tbits.as<uint64_t>() + nw_tets, c->blk.as<uint64_t>());
     FT_TRY(scan<uint64_t, uint64_t>(c, s, c->blk.as<uint64_t>(), c->blk.as<uint64_t>(), nblk,
                                     sc + S_BLK, LoadSame{}));
@@ -3236,7 +3434,7 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
                            c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), sc);
     if (cp.X > 0)
         hipLaunchKernelGGL(k_xtet, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tbits.as<uint64_t>() + nw_tets,
-                           c->t2e.as<int2>(), c->edges.as<int2>(), sbits, c->xtet.as<int32_t>(), sc,
+                           mesh_view(c), sbits, c->xtet.as<int32_t>(), sc,
                            c->ebits.as<unsigned long long>(), c->xedges.as<int4>());
     FT_TRY(scan<uint64_t, uint32_t>(c, s, c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(),
                                     nw_edges, reinterpret_cast<uint32_t*>(sc + S_EDGES), LoadPopc{}));
@@ -3252,11 +3450,11 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
     FT_TRY(ensure_stage(c, 2, cp, mode, want_nearest));
     if (nw_edges > 0)
         hipLaunchKernelGGL(k_points, dim3(grid1d(nw_edges)), dim3(kBS), 0, s, c->ebits.as<uint64_t>(),
-                           c->eprefix.as<uint32_t>(), nw_edges, c->edges.as<int2>(), c->xyz.as<float>(),
+                           c->eprefix.as<uint32_t>(), nw_edges, mesh_view(c), c->xyz.as<float>(),
                            phi, c->ifp.as<float>(), c->ifp2e.as<int32_t>(),
                            c->retain.as<unsigned long long>(), sc);
     if (cp.X > 0)
-        hipLaunchKernelGGL(k_tris, dim3(grid1d(cp.X)), dim3(kBS), 0, s, c->tets.as<int4>(), c->xyz.as<float>(),
+        hipLaunchKernelGGL(k_tris, dim3(grid1d(cp.X)), dim3(kBS), 0, s, mesh_view(c), c->xyz.as<float>(),
                            phi, c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), c->xedges.as<int4>(),
                            c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(), c->ifp.as<float>(),
                            c->tris.as<int4>(), c->tri_norm.as<float>(), c->if_elems.as<int32_t>(), sc);
@@ -3407,7 +3605,11 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         q.stats = c->stats.as<unsigned long long>();
     }
     q.overflow = (unsigned*)(sc + S_OVF);
-    q.probe = !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
+    // The greedy probe seeds each full-domain search with a candidate (256^3:
+    // query 11.3 -> 10.4 ms).  In a narrow band the band itself bounds the
+    // search and most tiles end clamped, for which the probe is wasted work
+    // (128^3 star, 6h band: 0.696 -> 0.675 ms without it).
+    q.probe = sp.band <= 0.0f && !(getenv("FT_PROBE") && !strcmp(getenv("FT_PROBE"), "0"));
     q.sc = sc;
     q.xcd_chunk = kXcdChunk;
     if (timed) (void)hipEventRecord(c->ev[2], s);
@@ -4084,6 +4286,7 @@ int ft_batch_info(ft_handle h, int32_t* n_fields, int64_t* node_off, int64_t* te
 int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges, int32_t* t2e)
 {
     FT_TRY(check_handle(h));
+    if (tets || edges || t2e) FT_TRY(materialize_conn(h));
     hipStream_t s = h->stream;
     if (xyz && h->N) FT_HIP(hipMemcpyAsync(xyz, h->xyz.p, 12 * h->N, hipMemcpyDeviceToHost, s));
     if (tets && h->T) FT_HIP(hipMemcpyAsync(tets, h->tets.p, 16 * h->T, hipMemcpyDeviceToHost, s));
@@ -4095,12 +4298,15 @@ int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges, int32_t*
 
 // ---- resident meshes --------------------------------------------------------
 // A resident mesh is a one-field handle kept only for its mesh and its query
-// slots; batches are assembled from resident meshes by device copies.
+// slots.  A batch of resident meshes reads their tets, t2e and edges in place
+// and copies only their node coordinates and slots; it holds a reference to
+// each, so a mesh released while a batch still refers to it is freed when
+// that batch moves on (its id is free at once).
 namespace {
 // Ids of released meshes are reused; the serial is not (it grows with every
 // mesh created in the process), so (id, serial) names one mesh for good.
 struct Resident {
-    ft_ctx* c = nullptr;
+    std::shared_ptr<ft_ctx> c;
     int64_t bytes = 0;
     uint64_t serial = 0;
 };
@@ -4114,7 +4320,7 @@ int resident_finish(ft_ctx* c, int32_t* id)
     DevBuf* tmp[] = {&c->bin_count, &c->bin_prefix, &c->pcode, &c->scan_bsum};
     for (DevBuf* b : tmp) release(*b);
     Resident r;
-    r.c = c;
+    r.c = std::shared_ptr<ft_ctx>(c, [](ft_ctx* p) { ft_destroy(p); });
     r.bytes = (int64_t)(c->xyz.cap + c->tets.cap + c->edges.cap + c->t2e.cap + c->perm.cap + c->sx.cap +
                         c->sy.cap + c->sz.cap + c->tile_field.cap + c->tile_sched.cap);
     std::lock_guard<std::mutex> lk(g_res_mu);
@@ -4126,7 +4332,7 @@ int resident_finish(ft_ctx* c, int32_t* id)
     return FT_OK;
 }
 
-ft_ctx* resident_get(int32_t id, uint64_t* serial = nullptr)
+std::shared_ptr<ft_ctx> resident_get(int32_t id, uint64_t* serial = nullptr)
 {
     std::lock_guard<std::mutex> lk(g_res_mu);
     if (id < 0 || id >= (int32_t)g_res.size()) return nullptr;
@@ -4164,7 +4370,7 @@ int ft_mesh_info(int32_t id, int64_t* n_nodes, int64_t* n_tets, int64_t* n_edges
     std::lock_guard<std::mutex> lk(g_res_mu);
     if (id < 0 || id >= (int32_t)g_res.size() || !g_res[id].c)
         return ft::set_error(FT_ERR_ARG, "no resident mesh %d", id);
-    const ft_ctx* c = g_res[id].c;
+    const ft_ctx* c = g_res[id].c.get();
     if (n_nodes) *n_nodes = c->N;
     if (n_tets) *n_tets = c->T;
     if (n_edges) *n_edges = c->E;
@@ -4174,22 +4380,22 @@ int ft_mesh_info(int32_t id, int64_t* n_nodes, int64_t* n_tets, int64_t* n_edges
 
 int ft_mesh_release(int32_t id)
 {
-    ft_ctx* c = nullptr;
+    std::shared_ptr<ft_ctx> c;
     {
         std::lock_guard<std::mutex> lk(g_res_mu);
         if (id < 0 || id >= (int32_t)g_res.size() || !g_res[id].c)
             return ft::set_error(FT_ERR_ARG, "no resident mesh %d", id);
-        c = g_res[id].c;
+        c.swap(g_res[id].c);
         g_res[id] = Resident{};
     }
-    return ft_destroy(c);
+    return FT_OK;                     // destroyed here unless a batch still refers to it
 }
 
 int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
 {
     FT_TRY(check_handle(h));
     if (n_fields < 1 || !mesh_ids) return ft::set_error(FT_ERR_ARG, "ft_set_batch_meshes: bad arguments");
-    std::vector<ft_ctx*> ms(n_fields);
+    std::vector<std::shared_ptr<ft_ctx>> ms(n_fields);
     std::vector<uint64_t> serials(n_fields);
     for (int k = 0; k < n_fields; k++) {
         ms[k] = resident_get(mesh_ids[k], &serials[k]);
@@ -4197,7 +4403,7 @@ int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
         if (ms[k]->device != h->device)
             return ft::set_error(FT_ERR_ARG, "resident mesh %d lives on another device", mesh_ids[k]);
     }
-    // already assembled from these very meshes: an id alone is not enough, a
+    // already set from these very meshes: an id alone is not enough, a
     // released id may since name a new mesh of other dimensions
     if (h->has_conn && h->batch_meshes == serials) return FT_OK;
     std::vector<int64_t> no(n_fields + 1, 0), to(n_fields + 1, 0), eo(n_fields + 1, 0);
@@ -4210,43 +4416,30 @@ int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
         return ft::set_error(FT_ERR_ARG, "batch exceeds int32 indices");
     h->has_conn = false;
     h->have_interface = false;
-    h->batch_meshes.clear();
     FT_TRY(set_fields(h, n_fields, no.data(), to.data(), eo.data()));
     hipStream_t s = h->stream;
     FT_TRY(ensure(h->xyz, 12 * no[n_fields]));
-    FT_TRY(ensure(h->tets, 16 * std::max<int64_t>(1, to[n_fields])));
-    FT_TRY(ensure(h->edges, 8 * std::max<int64_t>(1, eo[n_fields])));
-    FT_TRY(ensure(h->t2e, 24 * std::max<int64_t>(1, to[n_fields])));
     FT_TRY(ensure(h->perm, 4 * std::max<int64_t>(1, h->n_slots)));
     FT_TRY(ensure(h->sx, 4 * std::max<int64_t>(1, h->n_slots)));
     FT_TRY(ensure(h->sy, 4 * std::max<int64_t>(1, h->n_slots)));
     FT_TRY(ensure(h->sz, 4 * std::max<int64_t>(1, h->n_slots)));
     FT_TRY(ensure(h->tile_field, 4 * std::max<int64_t>(1, h->n_slots / 64)));
     FT_TRY(ensure(h->tile_sched, 8 * std::max<int64_t>(1, h->n_slots / 64)));
+    // per-field connectivity pointers: tets, t2e, edges of each mesh
+    std::vector<const void*> tab(3 * (size_t)n_fields);
     for (int k = 0; k < n_fields; k++) {
-        const ft_ctx* m = ms[k];
-        const int64_t so = h->slot_off[k];
-        FT_HIP(hipMemcpyAsync(h->xyz.as<float>() + 3 * no[k], m->xyz.p, 12 * m->N, hipMemcpyDeviceToDevice, s));
-        if (m->T) {
-            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(4 * m->T)), dim3(kBS), 0, s, m->tets.as<int32_t>(),
-                               4 * m->T, no[k], h->tets.as<int32_t>() + 4 * to[k]);
-            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(6 * m->T)), dim3(kBS), 0, s, m->t2e.as<int32_t>(),
-                               6 * m->T, eo[k], h->t2e.as<int32_t>() + 6 * to[k]);
-        }
-        if (m->E)
-            hipLaunchKernelGGL(k_add_offset, dim3(grid1d(2 * m->E)), dim3(kBS), 0, s, m->edges.as<int32_t>(),
-                               2 * m->E, no[k], h->edges.as<int32_t>() + 2 * eo[k]);
-        hipLaunchKernelGGL(k_add_offset_pos, dim3(grid1d(m->n_slots)), dim3(kBS), 0, s,
-                           m->perm.as<int32_t>(), m->n_slots, no[k], h->perm.as<int32_t>() + so);
-        FT_HIP(hipMemcpyAsync(h->sx.as<float>() + so, m->sx.p, 4 * m->n_slots, hipMemcpyDeviceToDevice, s));
-        FT_HIP(hipMemcpyAsync(h->sy.as<float>() + so, m->sy.p, 4 * m->n_slots, hipMemcpyDeviceToDevice, s));
-        FT_HIP(hipMemcpyAsync(h->sz.as<float>() + so, m->sz.p, 4 * m->n_slots, hipMemcpyDeviceToDevice, s));
-        FT_TRY(fill<int32_t>(s, h->tile_field.as<int32_t>() + so / 64, m->n_slots / 64, (int32_t)k));
-        h->fh[k] = m->fh[0];
+        tab[k] = ms[k]->tets.p;
+        tab[n_fields + k] = ms[k]->t2e.p;
+        tab[2 * n_fields + k] = ms[k]->edges.p;
     }
+    FT_TRY(ensure(h->field_tab, sizeof(void*) * tab.size()));
+    FT_HIP(hipMemcpyAsync(h->field_tab.p, tab.data(), sizeof(void*) * tab.size(), hipMemcpyHostToDevice, s));
+    FT_TRY(gather_meshes(h, ms, s, false));
+    for (int k = 0; k < n_fields; k++) h->fh[k] = ms[k]->fh[0];
     h->h_est = h->fh[0].h;
-    FT_TRY(check_launch());
     FT_HIP(hipStreamSynchronize(s));
+    h->field_meshes = std::move(ms);
+    h->virt = true;
     h->batch_meshes = serials;
     h->has_conn = true;
     return FT_OK;
@@ -4317,6 +4510,7 @@ int ft_normals_device(ft_handle h, const float** normals)
 // into the handle's k_if, on stream s.
 int enqueue_curvature(ft_ctx* h, hipStream_t s)
 {
+    FT_TRY(materialize_conn(h));
     if (!h->have_normals || !h->have_interface || !h->last_phi)
         return ft::set_error(FT_ERR_STATE, "curvature needs a reinit with want_normals first");
     const int64_t N = h->N, T = h->T, P = h->last.n_ifp;
@@ -4359,6 +4553,7 @@ int ft_interface_curvature(ft_handle h, float* k_if, void* stream)
 // Flow::surfaceTension_ (src/growth/Flow.cpp:307-341) on device arrays.
 int surface_tension(ft_ctx* h, float st, int32_t rho, double* fx, double* fy, double* fz, hipStream_t s)
 {
+    FT_TRY(materialize_conn(h));
     FT_TRY(enqueue_curvature(h, s));
     const int64_t N = h->N, P = h->last.n_ifp;
     if (N == 0 || P == 0) return FT_OK;
@@ -4440,6 +4635,7 @@ FemEll fem_ell(ft_ctx* h)
 // LevelSet.cpp:35-39), once per mesh layout.
 int fem_setup(ft_ctx* h, hipStream_t s)
 {
+    FT_TRY(materialize_conn(h));
     if (h->fem_layout == h->layout_gen) return FT_OK;
     if (!h->has_conn) return ft::set_error(FT_ERR_STATE, "advection needs a mesh (ft_set_mesh)");
     FT_TRY(build_node_tets(h, s));

@@ -21,6 +21,7 @@ A batch gives every job exactly the result the job would get alone (the
 batched kernels keep each field's data apart); tests check it bit for bit.
 """
 import collections
+import contextlib
 import ctypes as C
 import threading
 import time
@@ -255,7 +256,7 @@ class Server:
 
     def __init__(self, device=0, lanes=2, budget_bytes=200e9, max_fields=64, max_nodes=64_000_000,
                  mode="triangles", blend="init", band=0.0, bin_scale=0.0, query_block=0,
-                 want_ifn=True):
+                 want_ifn=True, exclusive_kernels=True):
         self.device = device
         self.want_ifn = bool(want_ifn)
         self.cache = MeshCache(device, budget_bytes)
@@ -266,6 +267,7 @@ class Server:
                        query_block=query_block)
         self.queue = collections.deque()
         self.cv = threading.Condition(threading.RLock())
+        self.gpu_turn = threading.Lock() if exclusive_kernels else contextlib.nullcontext()
         self.stop = False
         self.inflight_bytes = 0
         self.batches = []              # (n_jobs, n_nodes, seconds) per batch
@@ -330,9 +332,13 @@ class Server:
                     for j in jobs:
                         d_phi[off:off + j.n].copy_(j.phi, non_blocking=True)
                         off += j.n
-                counts = h.reset_phi_batch_device(
-                    d_phi.data_ptr(), d_ifn.data_ptr() if d_ifn is not None else None,
-                    stream=stream.cuda_stream, **self.kw)
+                # one lane's kernels at a time (two batches' kernels side by
+                # side only compete for the CUs and their caches); the other
+                # lanes' copies overlap them
+                with self.gpu_turn:
+                    counts = h.reset_phi_batch_device(
+                        d_phi.data_ptr(), d_ifn.data_ptr() if d_ifn is not None else None,
+                        stream=stream.cuda_stream, **self.kw)
                 with torch.cuda.stream(stream):
                     off = 0
                     for j in jobs:
