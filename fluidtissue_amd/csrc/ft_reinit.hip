@@ -1637,8 +1637,9 @@ __global__ __launch_bounds__(BS, WPS) void k_query_wave(QueryArgs a)
     if (halted(a.sc)) return;                         // an earlier stage failed: phi untouched
     const int w = threadIdx.x >> 6;
     if (a.queue) {
+        // with no tile list (narrow band): the tiles in slot order, none deep
         const int64_t n_tiles = a.n_slots / 64;
-        const uint32_t K = (uint32_t)ftd::cload(a.n_deep, 0);
+        const uint32_t K = a.tile_list ? (uint32_t)ftd::cload(a.n_deep, 0) : 0u;
         const uint32_t nn = (uint32_t)(n_tiles - K);
         const uint32_t chunk = a.xcd_chunk;
         const unsigned xcd = blockIdx.x % kXcds;
@@ -1658,7 +1659,7 @@ __global__ __launch_bounds__(BS, WPS) void k_query_wave(QueryArgs a)
                 if (t < nn) r = (int64_t)K + (int64_t)t;
             }
             if (r >= 0) {
-                search_tile<MODE>(a, (int64_t)a.tile_list[r] * 64, wstack[w]);
+                search_tile<MODE>(a, (a.tile_list ? (int64_t)a.tile_list[r] : r) * 64, wstack[w]);
                 continue;
             }
             if (++src > kXcds) break;                  // every counter exhausted
@@ -3636,7 +3637,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         // with a tile list: up to 7 padding workgroups after the deep ones,
         // one more for the split of the remainder
         dim3 g(grid1d(c->n_slots, qb) + (q.tile_list ? 8u : 0u)), b(qb);
-        if (sp.work_queue && q.tile_list) {
+        if (sp.work_queue) {
             // persistent: as many waves as the device holds at this launch
             // bound (4 SIMDs per CU), whole workgroups, at least one per XCD
             const int64_t waves = (int64_t)c->cu_count * 4 * wps;
