@@ -477,20 +477,28 @@ def host_info():
     return info
 
 
-def accuracy_check(r, m, phi, h, mode, band, stream, launch, n_sample=2000, seed=3):
-    """One more reinit (untimed) with the nearest triangle kept: the interface
-    against the oracle's marching tets (bit-exact) and a seeded node sample's
-    distances against the oracle's FP64 brute force over all triangles."""
+def gpu_sample(r, phi, mode, band, stream, launch):
+    """One more reinit (untimed) keeping the nearest primitive of every node:
+    the GPU side of the accuracy check (phi_ifn, nearest index, interface)."""
     import torch
-    import oracle
-    t0 = time.perf_counter()
     d = torch.from_numpy(phi).cuda()
     r.reset_phi_device(d.data_ptr(), None, mode=mode, blend="init", band=band, want_nearest=True,
                        stream=stream, **launch)
     torch.cuda.synchronize()
-    ifn = d.cpu().numpy()
-    near = r.nearest()
-    itf = r.interface()
+    return dict(ifn=d.cpu().numpy(), near=r.nearest(), itf=r.interface())
+
+
+def accuracy_check(m, phi, h, mode, band, gpu, n_sample=2000, seed=3):
+    """The oracle as the checker of the GPU sample (run in the CPU leg): the
+    interface against the oracle's marching tets (bit-exact), every sign,
+    and a seeded node sample's distances against the oracle's FP64 brute
+    force over all primitives; in triangle mode also against a second,
+    independent checker (oracle/tri_qp.py, long double) on the triangles the
+    GPU and the oracle chose."""
+    import oracle
+    from oracle import tri_qp
+    t0 = time.perf_counter()
+    ifn, near, itf = gpu["ifn"], gpu["near"], gpu["itf"]
     ifp, ifp2e = oracle.find_interface_nodes(m.edges, m.xyz, phi)
     tris, elems, area = oracle.reconstruct_interface(m.tets, m.t2e, m.edges, m.xyz, phi, ifp, ifp2e)
     exact = bool(np.array_equal(itf.ifp.view(np.uint32), ifp.view(np.uint32)) and
@@ -504,10 +512,8 @@ def accuracy_check(r, m, phi, h, mode, band, stream, launch, n_sample=2000, seed
     if mode == "points":
         d64, j64 = oracle.point_distance(m.xyz, ifp, nodes)
         d64 = d64.astype(np.float64)
-        prims = ifp
     else:
         d64, j64 = oracle.triangle_distance(m.xyz, ifp, tris, nodes)
-        prims = tris
     inb = np.ones(len(nodes), bool)
     clamped_ok = True
     if band > 0:
@@ -523,7 +529,7 @@ def accuracy_check(r, m, phi, h, mode, band, stream, launch, n_sample=2000, seed
             dg = float(np.sqrt(np.sum((m.xyz[nodes[k]].astype(np.float64) -
                                        ifp[near[nodes[k]]].astype(np.float64)) ** 2)))
         else:
-            dg = oracle.node_triangle_distance(nodes[k], m.xyz, ifp, prims, near[nodes[k]])
+            dg = oracle.node_triangle_distance(nodes[k], m.xyz, ifp, tris, near[nodes[k]])
         if abs(dg - d64[k]) <= eps:
             ties += 1
         else:
@@ -534,10 +540,23 @@ def accuracy_check(r, m, phi, h, mode, band, stream, launch, n_sample=2000, seed
            "rms_err_h": float(np.sqrt((err ** 2).mean()) / h) if err.size else 0.0,
            "tolerance_h": 1e-5, "eps_ties": ties, "index_mismatches": mism,
            "signs_bit_exact_all_nodes": signs, "interface_bit_exact": exact,
-           "reference": "FP64 brute force over the oracle's interface (all primitives)",
-           "seconds": round(time.perf_counter() - t0, 1)}
+           "reference": "FP64 brute force over the oracle's interface (all primitives)"}
+    if mode == "triangles":
+        sel = nodes[inb]
+        fg = near[sel]
+        ok = fg >= 0
+        q_gpu = tri_qp.node_distance(m.xyz, ifp, tris, sel[ok], fg[ok])
+        q_ora = tri_qp.node_distance(m.xyz, ifp, tris, sel, j64[inb])
+        e_ind = np.abs(got[inb][ok].astype(np.longdouble) - q_gpu).astype(np.float64)
+        out["independent"] = {
+            "max_err_h": float(e_ind.max() / h) if e_ind.size else 0.0,
+            "rms_err_h": float(np.sqrt((e_ind ** 2).mean()) / h) if e_ind.size else 0.0,
+            "oracle_vs_checker_max_h": float(np.max(np.abs(q_ora.astype(np.float64) - d64[inb])) / h),
+            "reference": "the GPU's distance against its chosen triangle's distance from a second "
+                         "checker (constrained quadratic minimization, long double)"}
     if band > 0:
         out["outside_band_clamped_exactly"] = clamped_ok
+    out["seconds"] = round(time.perf_counter() - t0, 1)
     return out
 
 
@@ -789,12 +808,11 @@ def main():
     t_query = ph[2] * 1e-3
     peak = 8000.0
 
-    accuracy = None
-    if not args.no_accuracy:
-        accuracy = accuracy_check(r, m, phi, h, args.mode, band, stream,
-                                  dict(bin_scale=args.bin_scale, query_block=args.query_block,
-                                       waves_per_simd=args.waves_per_simd, schedule=args.schedule))
-        log(f"accuracy: {accuracy}")
+    sample = None
+    if not args.no_accuracy and not args.no_cpu:
+        sample = gpu_sample(r, phi, args.mode, band, stream,
+                            dict(bin_scale=args.bin_scale, query_block=args.query_block,
+                                 waves_per_simd=args.waves_per_simd, schedule=args.schedule))
 
     # the drop-in path with host phi (ft_reinit on the caller's own arrays, as
     # LevelSet::resetPhi_ would call it: H2D of phi, the reinit, D2H of phi and
@@ -827,13 +845,18 @@ def main():
         counters = live_counters(args, N, ph[2], os.path.join(ROOT, "gpurun_out", "bench_counters"))
         log(f"counters: { {k: v for k, v in counters.items() if k != 'commands'} }")
 
-    cpu = None
+    # The CPU leg: the oracle on the host, as the reference CPU path's timed
+    # baseline and as the checker of the GPU sample above.
+    cpu = accuracy = None
     hinfo = host_info()
     if not args.no_cpu:
         cpu = cpu_baseline(m, phi, hinfo["threads_used"])
         log(f"cpu baseline: {cpu['value']:.3f} Mnodes/s ({cpu['sample']})")
         cpu.pop("seconds")
         cpu["host"] = hinfo
+        if sample is not None:
+            accuracy = accuracy_check(m, phi, h, args.mode, band, sample)
+            log(f"accuracy: {accuracy}")
 
     served = None
     if not args.no_served:

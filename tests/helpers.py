@@ -53,10 +53,19 @@ def assert_interface_bit_exact(got, ref):
 
 def triangle_errors(name, d_gpu, f_gpu, nodes, h, ref=None, eps_tie=None):
     """Compares unsigned GPU distances / nearest triangles on `nodes` with the
-    FP64 brute-force oracle.  Returns (max_err, rms_err, n_ties, n_mismatch)."""
+    FP64 brute-force oracle.  Returns (max_err, rms_err, n_ties, n_mismatch).
+    Each GPU distance is also checked, within 1e-5 h, against its own
+    triangle's distance from the second checker (oracle/tri_qp.py: the
+    constrained quadratic minimization in long double), so the error figure
+    does not rest on the one geometric formula the device settle shares with
+    the oracle."""
+    from oracle import tri_qp
     m, phi, _ = workload(name)
     itf = ref or oracle_interface(name)
     d64, f64 = oracle.triangle_distance(m.xyz, itf["ifp"], itf["tris"], nodes)
+    ok = f_gpu >= 0
+    q = tri_qp.node_distance(m.xyz, itf["ifp"], itf["tris"], nodes[ok], f_gpu[ok])
+    assert np.all(np.abs(d_gpu[ok].astype(np.longdouble) - q) <= 1e-5 * h)
     err = np.abs(d_gpu.astype(np.float64) - d64)
     eps_tie = 1e-5 * h if eps_tie is None else eps_tie
     ties = mismatch = 0

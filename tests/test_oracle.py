@@ -72,6 +72,39 @@ def test_tri_dist_degenerate():
         assert math.sqrt(oracle.point_triangle_dist2(x, a, a, a)) == pytest.approx(np.linalg.norm(x - a))
 
 
+# The second checker (oracle/tri_qp.py): the same distances by minimizing the
+# quadratic over the triangle's parameters, in long double.  Pinned by the
+# same closed forms and against the geometric oracle on random, thin and
+# collinear triangles; a wrong coefficient in any side's quadratic or in the
+# stationary point fails the comparison.
+@pytest.mark.parametrize("x,d2", [
+    ((0.2, 0.3, 0.7), 0.49), ((-1, -1, 0), 2.0), ((2, 0, 1), 2.0), ((0, 3, 0), 4.0),
+    ((0.5, -1, 0), 1.0), ((1, 1, 0), 0.5), ((-1, 0.5, 2), 5.0), ((0.25, 0.25, 0), 0.0),
+])
+def test_tri_qp_closed_form(x, d2):
+    from oracle import tri_qp
+    got = float(tri_qp.dist2(np.array([x], float), *(v[None] for v in TRI))[0])
+    assert got == pytest.approx(d2, rel=1e-15, abs=1e-15)
+
+
+def test_tri_qp_agrees_with_the_geometric_oracle():
+    from oracle import tri_qp
+    rng = np.random.default_rng(4)
+    n = 4000
+    a, b, c = rng.normal(size=(3, n, 3))
+    thin = rng.random(n) < 0.25                 # a quarter nearly collinear
+    c[thin] = a[thin] + rng.uniform(0.1, 0.9, size=(thin.sum(), 1)) * (b[thin] - a[thin]) + \
+        1e-9 * rng.normal(size=(thin.sum(), 3))
+    x = rng.normal(size=(n, 3)) * 2
+    q = tri_qp.dist2(x, a, b, c)
+    g = np.array([oracle.point_triangle_dist2(x[k], a[k], b[k], c[k]) for k in range(n)])
+    err = np.abs(np.sqrt(q.astype(np.float64)) - np.sqrt(g))
+    assert err.max() < 1e-12
+    # vertex permutations give the same distance
+    q2 = tri_qp.dist2(x, c, a, b)
+    assert np.max(np.abs(np.sqrt(q2.astype(np.float64)) - np.sqrt(q.astype(np.float64)))) < 1e-12
+
+
 # ---------------------------------------------------------------------------
 # findInterfaceNodes_ (LevelSet.cpp:261-300)
 # ---------------------------------------------------------------------------
