@@ -1941,11 +1941,9 @@ __device__ __forceinline__ int64_t ell_at(const FemEll& e, int64_t i, int k)
 // G[j] = invBt grad(phi_j) (the constant gradients (-1,-1,-1), e1, e2, e3,
 // Tetrahedron.hpp:36-50; sums left to right, as in k_tet_grad).  False for a
 // singular map, whose tet the reference leaves with an unset inverse.
-__device__ __forceinline__ bool fem_frame(const float* xyz, const int qi[4], double G[4][3], double& detB)
+// x: the tet's four nodes, in its local order.
+__device__ __forceinline__ bool fem_frame(const double x[4][3], double G[4][3], double& detB)
 {
-    double x[4][3];
-    for (int j = 0; j < 4; j++)
-        for (int k = 0; k < 3; k++) x[j][k] = (double)xyz[3 * (int64_t)qi[j] + k];
     double B[3][3], C[3][3];
     for (int r = 0; r < 3; r++)
         for (int c = 0; c < 3; c++) B[r][c] = x[c + 1][r] - x[0][r];
@@ -2048,30 +2046,65 @@ __global__ __launch_bounds__(kFemSlice) void k_fem_rowfill(const uint64_t* n2t_o
 // + V3 (uz v) with V_c(q) = sum_j V_c[t(j)] phi_j(q) accumulated from 0 in
 // j order (vel: the velocity interleaved per node).
 //
-// kFemLanes lanes share a row and take its tets in rounds: in each round lane
-// L computes the four contributions of tet kFemLanes r + L into LDS, then the
+// kFemLanes lanes share a row and take its tets in rounds.  First the row's
+// columns are gathered once into LDS: each column node's coordinates (and,
+// for the advection form, its velocity), so that a tet's nodes are read
+// there through its column map (fmap: the row slot of each of its four
+// nodes) instead of through the tet list.  Then in each round lane L
+// computes the four contributions of tet kFemLanes r + L into LDS, and the
 // row's first lane adds that round's tets into the row accumulator in tet
 // order, so every entry is summed exactly as the reference's std::map does.
 // A workgroup is one wave of kFemRowsPerWave rows.
 constexpr int kFemLanes = 4;
 constexpr int kFemRowsPerWave = 64 / kFemLanes;
+constexpr int kFemQb = 4 * kFemLanes + 1;            // round buffer per row, padded against bank conflicts
+constexpr int kFemQm = kFemLanes + 1;                // round column maps per row, padded likewise
 
-// LDS: kFemRowsPerWave accumulators of `stride` doubles, then as many round
-// buffers of 4 kFemLanes doubles.
-__host__ __device__ inline size_t fem_lds_bytes(int stride)
+// LDS per wave, in this order: kFemRowsPerWave accumulators of `stride`
+// doubles; the round buffers (kFemQb doubles per row); for the advection
+// form the column velocities (3 doubles per column slot); the column
+// coordinates (3 floats per slot); the round maps (kFemQm words per row);
+// the row's own slot (one word per row).  The velocity region is free after
+// the rounds: k_fem_cn reuses it.
+__host__ __device__ inline size_t fem_lds_bytes(int stride, int form)
 {
-    return sizeof(double) * (size_t)kFemRowsPerWave * ((size_t)stride + 4 * kFemLanes);
+    const size_t R = kFemRowsPerWave, S = (size_t)stride;
+    return 8 * R * (S + kFemQb) + (form == 1 ? 24 * R * S : 0) + 12 * R * S + 4 * R * kFemQm + 4 * R;
 }
 
 template <int FORM>
-__device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const uint32_t* fmap,
-                         const int4* tets, const float* xyz, const double* vel, int64_t N, const FemQuad& qd,
-                         const FemEll& e, int stride, double* lds, int64_t& row, double*& acc, int& len)
+struct FemLds {
+    double* acc;       // the row's accumulator
+    double* qb;        // the row's round buffer
+    double* vel;       // the row's column velocities (FORM 1)
+    float* xyz;        // the row's column coordinates
+    uint32_t* qm;      // the row's round maps
+    int* own;          // the row's own slot
+    __device__ FemLds(double* lds, int stride, int rl)
+    {
+        const size_t R = kFemRowsPerWave, S = (size_t)stride;
+        acc = lds + rl * S;
+        qb = lds + R * S + rl * kFemQb;
+        double* v0 = lds + R * (S + kFemQb);
+        vel = v0 + rl * 3 * S;
+        float* x0 = reinterpret_cast<float*>(v0 + (FORM == 1 ? 3 * R * S : 0));
+        xyz = x0 + rl * 3 * S;
+        uint32_t* m0 = reinterpret_cast<uint32_t*>(x0 + 3 * R * S);
+        qm = m0 + rl * kFemQm;
+        own = reinterpret_cast<int*>(m0 + R * kFemQm) + rl;
+    }
+};
+
+template <int FORM>
+__device__ void fem_rows(const uint64_t* __restrict__ n2t_off, const uint32_t* __restrict__ fmap,
+                         const float* __restrict__ xyz, const double* __restrict__ vel, int64_t N,
+                         const FemQuad& qd, const FemEll& e, int stride, double* lds, int64_t& row, double*& acc,
+                         int& len)
 {
     const int lane = threadIdx.x % kFemLanes, rl = threadIdx.x / kFemLanes;
     const int64_t i = (int64_t)blockIdx.x * kFemRowsPerWave + rl;
-    double* qb = lds + (size_t)kFemRowsPerWave * stride + (size_t)rl * 4 * kFemLanes;
-    acc = lds + (size_t)rl * stride;
+    const FemLds<FORM> L(lds, stride, rl);
+    acc = L.acc;
     row = i;
     len = 0;
     uint64_t a0 = 0;
@@ -2079,23 +2112,38 @@ __device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const 
     if (i < N) {
         a0 = n2t_off[i];
         na = (int)(n2t_off[i + 1] - a0);
-        if (lane == 0) {
-            len = e.len[i];
-            for (int k = 0; k < len; k++) acc[k] = 0.0;
+        len = e.len[i];
+        for (int k = lane; k < len; k += kFemLanes) {
+            const int32_t c = e.cols[ell_at(e, i, k)];
+            for (int d = 0; d < 3; d++) L.xyz[3 * k + d] = xyz[3 * (int64_t)c + d];
+            if (FORM == 1)
+                for (int d = 0; d < 3; d++) L.vel[3 * k + d] = vel[3 * (int64_t)c + d];
+            if (c == (int32_t)i) *L.own = k;
+            acc[k] = 0.0;
         }
     }
     int na_max = na;                                   // rounds are uniform over the wave
     for (int o = 32; o >= 1; o >>= 1) na_max = max(na_max, __shfl_xor(na_max, o));
+    uint32_t mn = lane < na ? fmap[a0 + lane] : 0u;    // maps one round ahead
+    __syncthreads();
+    const int own = i < N ? *L.own : 0;
     for (int r0 = 0; r0 < na_max; r0 += kFemLanes) {
         const int a = r0 + lane;
+        const uint32_t mp = mn;
+        if (a + kFemLanes < na) mn = fmap[a0 + a + kFemLanes];
         if (a < na) {
-            const int4 q = tets[n2t_ids[a0 + a]];
-            const int qi[4] = {q.x, q.y, q.z, q.w};
+            L.qm[lane] = mp;
+            int sl[4];
+            double x[4][3];
+            for (int j = 0; j < 4; j++) {
+                sl[j] = (int)((mp >> (8 * j)) & 0xffu);
+                for (int d = 0; d < 3; d++) x[j][d] = (double)L.xyz[3 * sl[j] + d];
+            }
             double G[4][3], detB;
-            if (!fem_frame(xyz, qi, G, detB)) {
-                for (int j = 0; j < 4; j++) qb[4 * lane + j] = 0.0;   // no contribution
+            if (!fem_frame(x, G, detB)) {
+                for (int j = 0; j < 4; j++) L.qb[4 * lane + j] = 0.0;   // no contribution
             } else {
-                const int li = qi[0] == (int)i ? 0 : qi[1] == (int)i ? 1 : qi[2] == (int)i ? 2 : 3;
+                const int li = sl[0] == own ? 0 : sl[1] == own ? 1 : sl[2] == own ? 2 : 3;
                 // quadrature points in the order 0, 2, 1, 3: s02 = f0 w + f2 w and
                 // s13 = f1 w + f3 w per basis function, then (s02 + s13) |det B|
                 const double w = qd.w;
@@ -2106,7 +2154,7 @@ __device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const 
                     double V0 = 0.0, V1 = 0.0, V2 = 0.0;
                     if (FORM == 1) {
                         for (int j = 0; j < 4; j++) {
-                            const double* vj = vel + 3 * (int64_t)qi[j];
+                            const double* vj = L.vel + 3 * sl[j];
                             const double pj = qd.phi[4 * j + q4];
                             V0 = V0 + vj[0] * pj;
                             V1 = V1 + vj[1] * pj;
@@ -2123,7 +2171,7 @@ __device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const 
                         else s13[j] = s13[j] + pw;
                     }
                 }
-                for (int j = 0; j < 4; j++) qb[4 * lane + j] = (s02[j] + s13[j]) * detB;
+                for (int j = 0; j < 4; j++) L.qb[4 * lane + j] = (s02[j] + s13[j]) * detB;
             }
         }
         __syncthreads();
@@ -2132,12 +2180,12 @@ __device__ void fem_rows(const uint64_t* n2t_off, const int32_t* n2t_ids, const 
             // independent, so they are issued together; tets stay in order
             const int nt = min(kFemLanes, na - r0);
             for (int t = 0; t < nt; t++) {
-                const uint32_t pos = fmap[a0 + r0 + t];
+                const uint32_t pos = L.qm[t];
                 int c[4];
                 double s4[4];
                 for (int j = 0; j < 4; j++) {
                     c[j] = (int)((pos >> (8 * j)) & 0xffu);
-                    s4[j] = acc[c[j]] + qb[4 * t + j];
+                    s4[j] = acc[c[j]] + L.qb[4 * t + j];
                 }
                 for (int j = 0; j < 4; j++) acc[c[j]] = s4[j];
             }
@@ -2158,30 +2206,28 @@ __global__ __launch_bounds__(kBS) void k_vel_interleave(const double* flow, int6
 }
 
 // The mass matrix, once per mesh (LevelSet::init, LevelSet.cpp:35-39).
-__global__ __launch_bounds__(64) void k_fem_mass(const uint64_t* n2t_off, const int32_t* n2t_ids,
-                                                 const uint32_t* fmap, const int4* tets, const float* xyz,
-                                                 int64_t N, FemQuad qd, FemEll e, int stride, double* mvals)
+__global__ __launch_bounds__(64) void k_fem_mass(const uint64_t* n2t_off, const uint32_t* fmap,
+                                                 const float* xyz, int64_t N, FemQuad qd, FemEll e, int stride, double* mvals)
 {
     extern __shared__ double fem_lds[];
     int64_t i;
     double* acc;
     int len;
-    fem_rows<0>(n2t_off, n2t_ids, fmap, tets, xyz, nullptr, N, qd, e, stride, fem_lds, i, acc, len);
-    for (int k = 0; k < len; k++) mvals[ell_at(e, i, k)] = acc[k];
+    fem_rows<0>(n2t_off, fmap, xyz, nullptr, N, qd, e, stride, fem_lds, i, acc, len);
+    for (int k = threadIdx.x % kFemLanes; k < len; k += kFemLanes) mvals[ell_at(e, i, k)] = acc[k];
 }
 
 // The advection matrix alone (ft_fem_assemble), into vals in ELL layout.
-__global__ __launch_bounds__(64) void k_fem_adv(const uint64_t* n2t_off, const int32_t* n2t_ids,
-                                                const uint32_t* fmap, const int4* tets, const float* xyz,
-                                                const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
+__global__ __launch_bounds__(64) void k_fem_adv(const uint64_t* n2t_off, const uint32_t* fmap,
+                                                const float* xyz, const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
                                                 double* vals)
 {
     extern __shared__ double fem_lds[];
     int64_t i;
     double* acc;
     int len;
-    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, fem_lds, i, acc, len);
-    for (int k = 0; k < len; k++) vals[ell_at(e, i, k)] = acc[k];
+    fem_rows<1>(n2t_off, fmap, xyz, vel, N, qd, e, stride, fem_lds, i, acc, len);
+    for (int k = threadIdx.x % kFemLanes; k < len; k += kFemLanes) vals[ell_at(e, i, k)] = acc[k];
 }
 
 // ELL rows to CSR at the row offsets off (a scan of the lengths).
@@ -2223,32 +2269,59 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* part)
 // residual from the guess phi), each summed in column order; r0 = r, x = phi,
 // p = v = 0.
 // At least 5 waves per SIMD: 94 VGPRs, no spills (measured faster than 118 at 4).
-__global__ __launch_bounds__(64, 5) void k_fem_cn(const uint64_t* n2t_off, const int32_t* n2t_ids,
-                                               const uint32_t* fmap, const int4* tets, const float* xyz,
-                                               const double* vel, int64_t N, FemQuad qd, FemEll e, int stride,
-                                               const double* mvals, double s, const double* phi,
-                                               double* lvals, double* invdiag, double* b, double* x, double* r,
-                                               double* r0, double* p, double* v)
+__global__ __launch_bounds__(64, 3) void k_fem_cn(const uint64_t* __restrict__ n2t_off,
+                                               const uint32_t* __restrict__ fmap, const float* __restrict__ xyz,
+                                               const double* __restrict__ vel, int64_t N, FemQuad qd, FemEll e,
+                                               int stride, const double* __restrict__ mvals, double s,
+                                               const double* __restrict__ phi, double* __restrict__ lvals,
+                                               double* __restrict__ invdiag, double* __restrict__ b,
+                                               double* __restrict__ x, double* __restrict__ r,
+                                               double* __restrict__ r0, double* __restrict__ p,
+                                               double* __restrict__ v)
 {
     extern __shared__ double fem_lds[];
     int64_t i;
     double* A;
     int len;
-    fem_rows<1>(n2t_off, n2t_ids, fmap, tets, xyz, vel, N, qd, e, stride, fem_lds, i, A, len);
-    if (i < N && threadIdx.x % kFemLanes == 0) {
-        double bi = 0.0, lx = 0.0, id = 1.0;
+    fem_rows<1>(n2t_off, fmap, xyz, vel, N, qd, e, stride, fem_lds, i, A, len);
+    // the row's entries split over its lanes: L = M + s A into lvals, the
+    // products R phi_c (in place of A) and L phi_c into LDS; then the first
+    // lane sums both in column order, as a sequential row loop would
+    const int lane = threadIdx.x % kFemLanes;
+    double* LP = FemLds<1>(fem_lds, stride, threadIdx.x / kFemLanes).vel;   // free after the rounds
+    double id = 0.0;
+    int found = 0;
+    for (int k = lane; k < len; k += kFemLanes) {
+        const int64_t at = ell_at(e, i, k);
+        const int32_t c = e.cols[at];
+        const double m = mvals[at], sa = s * A[k];
+        const double L = m + sa, R = m - sa;
+        const double pc = phi[c];
+        lvals[at] = L;
+        A[k] = R * pc;
+        LP[k] = L * pc;
+        if (c == (int32_t)i && L != 0.0) {
+            id = 1.0 / L;
+            found = 1;
+        }
+    }
+    for (int o = 1; o < kFemLanes; o <<= 1) {          // the diagonal's lane to all
+        const double vo = __shfl_xor(id, o);
+        const int fo = __shfl_xor(found, o);
+        if (!found && fo) {
+            id = vo;
+            found = 1;
+        }
+    }
+    __syncthreads();
+    if (i < N && lane == 0) {
+        double bi = 0.0, lx = 0.0;
         for (int k = 0; k < len; k++) {
-            const int64_t at = ell_at(e, i, k);
-            const int32_t c = e.cols[at];
-            const double m = mvals[at], sa = s * A[k];
-            const double L = m + sa, R = m - sa;
-            lvals[at] = L;
-            bi += R * phi[c];
-            lx += L * phi[c];
-            if (c == (int32_t)i && L != 0.0) id = 1.0 / L;
+            bi += A[k];
+            lx += LP[k];
         }
         const double ri = bi - lx;
-        invdiag[i] = id;
+        invdiag[i] = found ? id : 1.0;
         b[i] = bi;
         x[i] = phi[i];
         r[i] = ri;
@@ -4624,7 +4697,7 @@ FemQuad fem_quad()
     return q;
 }
 
-size_t fem_lds(const ft_ctx* h) { return fem_lds_bytes(h->fem_stride); }
+size_t fem_lds(const ft_ctx* h, int form) { return fem_lds_bytes(h->fem_stride, form); }
 unsigned fem_grid(int64_t N) { return (unsigned)((N + kFemRowsPerWave - 1) / kFemRowsPerWave); }
 
 FemEll fem_ell(ft_ctx* h)
@@ -4673,9 +4746,8 @@ int fem_setup(ft_ctx* h, hipStream_t s)
         hipLaunchKernelGGL(k_fem_rowfill, dim3((unsigned)ns), dim3(kFemSlice), 0, s, h->n2t_off.as<uint64_t>(),
                            h->n2t_ids.as<int32_t>(), h->tets.as<int4>(), N, e, w, h->fem_cols.as<int32_t>(),
                            h->fem_map.as<uint32_t>());
-        hipLaunchKernelGGL(k_fem_mass, dim3(fem_grid(N)), dim3(64), fem_lds(h), s,
-                           h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(),
-                           h->tets.as<int4>(), h->xyz.as<float>(), N, fem_quad(), e, h->fem_stride,
+        hipLaunchKernelGGL(k_fem_mass, dim3(fem_grid(N)), dim3(64), fem_lds(h, 0), s,
+                           h->n2t_off.as<uint64_t>(), h->fem_map.as<uint32_t>(), h->xyz.as<float>(), N, fem_quad(), e, h->fem_stride,
                            h->fem_m.as<double>());
     }
     FT_TRY(check_launch());
@@ -4727,9 +4799,8 @@ int advect(ft_ctx* h, double* d_phi, const double* d_flow, double dt, double tol
     double* lv = h->fem_l.as<double>();
     FT_TRY(ensure(h->adv_vel3, 24 * (size_t)N));
     hipLaunchKernelGGL(k_vel_interleave, dim3(grid1d(N)), dim3(kBS), 0, s, d_flow, N, h->adv_vel3.as<double>());
-    hipLaunchKernelGGL(k_fem_cn, dim3(fem_grid(N)), dim3(64), fem_lds(h), s, h->n2t_off.as<uint64_t>(),
-                       h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(), h->tets.as<int4>(),
-                       h->xyz.as<float>(), h->adv_vel3.as<double>(), N, fem_quad(), e, h->fem_stride,
+    hipLaunchKernelGGL(k_fem_cn, dim3(fem_grid(N)), dim3(64), fem_lds(h, 1), s, h->n2t_off.as<uint64_t>(),
+                       h->fem_map.as<uint32_t>(), h->xyz.as<float>(), h->adv_vel3.as<double>(), N, fem_quad(), e, h->fem_stride,
                        h->fem_m.as<double>(), 0.5 * dt, d_phi, lv, invdiag, b, x, r, r0, p, v);
     hipLaunchKernelGGL(k_dots2, dim3((unsigned)nb), dim3(kBcgBS), 0, s, N, r, r, b, b, part);
     FT_TRY(check_launch());
@@ -4830,9 +4901,8 @@ int ft_fem_assemble(ft_handle h, int32_t form, const double* vel, int64_t* row_o
         FT_TRY(ensure(h->adv_vel3, 24 * (size_t)N));
         hipLaunchKernelGGL(k_vel_interleave, dim3(grid1d(N)), dim3(kBS), 0, s, h->adv_flow.as<double>(), N,
                            h->adv_vel3.as<double>());
-        hipLaunchKernelGGL(k_fem_adv, dim3(fem_grid(N)), dim3(64), fem_lds(h), s, h->n2t_off.as<uint64_t>(),
-                           h->n2t_ids.as<int32_t>(), h->fem_map.as<uint32_t>(), h->tets.as<int4>(),
-                           h->xyz.as<float>(), h->adv_vel3.as<double>(), N, fem_quad(), e, h->fem_stride,
+        hipLaunchKernelGGL(k_fem_adv, dim3(fem_grid(N)), dim3(64), fem_lds(h, 1), s, h->n2t_off.as<uint64_t>(),
+                           h->fem_map.as<uint32_t>(), h->xyz.as<float>(), h->adv_vel3.as<double>(), N, fem_quad(), e, h->fem_stride,
                            h->fem_l.as<double>());
         src = h->fem_l.as<double>();
     }
