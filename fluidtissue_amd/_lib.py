@@ -35,6 +35,9 @@ FT_PHI_ELLIPSOIDS = 4
 
 FT_NOALLOC_SCRATCH_BYTES = 128 << 20
 
+FT_OBJ_INTERFACE = 0x02
+FT_OBJ_BOUNDARY = 0x04
+
 
 class FtOpts(C.Structure):
     _fields_ = [
@@ -137,6 +140,7 @@ _SIGS = {
     "ft_surface_tension_device": (C.c_int, [_H, C.c_float, C.c_int32, _P, _P, _P, _P]),
     "ft_get_counts": (C.c_int, [_H, _P]),
     "ft_write_interface_obj": (C.c_int, [_H, C.c_char_p, C.c_int32]),
+    "ft_write_obj": (C.c_int, [_H, C.c_char_p, C.c_int32, C.c_int32, _P, C.c_int64]),
     "ft_normals_device": (C.c_int, [_H, C.POINTER(C.c_void_p)]),
     "ft_set_timing": (C.c_int, [_H, C.c_int32]),
     "ft_get_timing": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

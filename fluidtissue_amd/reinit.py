@@ -410,9 +410,22 @@ class Reinit(_Handle):
                                      _ptr(f[2])))
         return tuple(f)
 
-    def write_obj(self, fname, unit_normals=True):
-        """fname.obj / fname.mtl of the last reinit's interface (cfish::Write_Wavefront_obj)."""
-        check(lib.ft_write_interface_obj(self._h, str(fname).encode(), 1 if unit_normals else 0))
+    def write_obj(self, fname, unit_normals=True, boundary=None, interface=True):
+        """fname.obj / fname.mtl (cfish::Write_Wavefront_obj): the last
+        reinit's interface when `interface`, then, when `boundary` is given
+        (n x 3 node ids: the mesh's exterior boundary triangles), the domain
+        boundary section."""
+        if boundary is None:
+            if not interface:
+                raise ValueError("write_obj: nothing to write")
+            check(lib.ft_write_interface_obj(self._h, str(fname).encode(), 1 if unit_normals else 0))
+            return
+        b = np.ascontiguousarray(boundary, np.int32)
+        if b.ndim != 2 or b.shape[1] != 3:
+            raise ValueError("boundary: an n x 3 array of node ids")
+        parts = _lib.FT_OBJ_BOUNDARY | (_lib.FT_OBJ_INTERFACE if interface else 0)
+        check(lib.ft_write_obj(self._h, str(fname).encode(), parts, 1 if unit_normals else 0, _ptr(b),
+                               len(b)))
 
     def nearest(self):
         out = np.empty(self.n, np.int32)

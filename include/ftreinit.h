@@ -249,6 +249,21 @@ int ft_get_counts(ft_handle h, ft_counts* total);
  * as src/growth/WriteData.cpp:48-82), text identical to its iostream output. */
 int ft_write_interface_obj(ft_handle h, const char* fname, int32_t unit_normals);
 
+/* The parts of cfish::Write_Wavefront_obj (src/leveque/WriteData.hpp:14,
+ * the reference's INTERFACE and BOUNDARY flags). */
+#define FT_OBJ_INTERFACE 0x02
+#define FT_OBJ_BOUNDARY 0x04
+
+/* Wavefront OBJ with the parts named in `parts` (src/leveque/WriteData.cpp:
+ * 9-87): the interface as ft_write_interface_obj, then the domain boundary:
+ * the nodes of b_tris (n_btris x 3 node ids of the resident mesh, the
+ * reference's mesh.b_tris()) in increasing id, and the triangles in the
+ * order given, indexed after the interface's points.  The boundary needs a
+ * single resident mesh; the interface part needs a reinit first.  Then the
+ * materials file, as ft_write_interface_obj. */
+int ft_write_obj(ft_handle h, const char* fname, int32_t parts, int32_t unit_normals,
+                 const int32_t* b_tris, int64_t n_btris);
+
 /* Captures and launches of ft_reinit_graph on h so far (nullable). */
 int ft_graph_stats(ft_handle h, int64_t* captures, int64_t* launches);
 

@@ -108,6 +108,34 @@ def expected_interface_obj(ifp, tris, fname, unit_normals=True):
     return "\n".join(lines) + "\n"
 
 
+def boundary_faces(tets):
+    """The faces of a tet mesh that belong to one tet only (the exterior
+    boundary), each as its tet lists them with the opposite node left out,
+    in tet order."""
+    tets = np.asarray(tets)[:, :4]
+    faces = np.concatenate([tets[:, [1, 2, 3]], tets[:, [0, 2, 3]], tets[:, [0, 1, 3]], tets[:, [0, 1, 2]]])
+    order = np.concatenate([np.arange(len(tets)) * 4 + k for k in range(4)])
+    key = np.sort(faces, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    once = cnt[inv.ravel()] == 1
+    sel = np.nonzero(once)[0]
+    return faces[sel[np.argsort(order[sel], kind="stable")]].astype(np.int32)
+
+
+def expected_boundary_obj(xyz, b_tris, offset):
+    """The BOUNDARY text of cfish::Write_Wavefront_obj (src/leveque/
+    WriteData.cpp:63-87): the boundary nodes in increasing id, then the
+    triangles in the given order, indices from offset."""
+    xyz = np.asarray(xyz, np.float32)
+    nodes = sorted(set(int(v) for v in np.asarray(b_tris).ravel()))
+    pos = {v: k for k, v in enumerate(nodes)}
+    lines = ["#", "# domain boundary", "#", ""]
+    lines += [f"v {float(xyz[v, 0]):g} {float(xyz[v, 1]):g} {float(xyz[v, 2]):g}" for v in nodes]
+    lines += ["", "g boundary", "usemtl 1"]
+    lines += [f"f {offset + pos[int(a)]} {offset + pos[int(b)]} {offset + pos[int(c)]}" for a, b, c in b_tris]
+    return "\n".join(lines) + "\n"
+
+
 EXPECTED_MTL = ("newmtl 0\nKa 0 0 0\nKd 0.6 0.6 0.6\nd 1\nKs 0 0 0\nNs 0 interface\n\n"
                 "newmtl 1\nKa 0 0 0\nKd 0.0 0.0 1.0\nd 0.2\nKs 0 0 0\nNs 0 boundary\n"
                 "newmtl 2\nKa 0 0 0\nKd 1.0 0.0 0.0\nd 1\nKs 0 0 0\nNs 0 knots\n")

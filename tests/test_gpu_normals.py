@@ -186,3 +186,34 @@ def test_interface_obj_matches_reference_text(tmp_path, unit):
     ref = oracle_interface("sphere_wave_32")
     assert open(base + ".obj").read() == expected_interface_obj(ref["ifp"], ref["tris"], base, unit)
     assert open(base + ".mtl").read() == EXPECTED_MTL
+
+
+def test_obj_boundary_section_matches_reference_text(tmp_path):
+    """INTERFACE | BOUNDARY: the interface text, then the boundary nodes in
+    increasing id and the given triangles indexed after the interface's
+    points (WriteData.cpp:63-87); BOUNDARY alone indexes from 1; bad node
+    ids are refused."""
+    from tests.helpers import (EXPECTED_MTL, boundary_faces, expected_boundary_obj, expected_interface_obj,
+                               oracle_interface)
+    m, phi, h = workload("sphere_wave_32")
+    bt = boundary_faces(m.tets)
+    assert len(bt) == 12 * 31 * 31                     # two triangles per boundary square
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    base = str(tmp_path / "only_boundary")
+    r.write_obj(base, boundary=bt, interface=False)
+    head = f"mtllib {base}.mtl\n\n"
+    assert open(base + ".obj").read() == head + expected_boundary_obj(m.xyz, bt, 1)
+    assert open(base + ".mtl").read() == EXPECTED_MTL
+    r.reset_phi(phi, mode="triangles")
+    base = str(tmp_path / "both")
+    r.write_obj(base, boundary=bt)
+    ref = oracle_interface("sphere_wave_32")
+    want = expected_interface_obj(ref["ifp"], ref["tris"], base) + expected_boundary_obj(m.xyz, bt,
+                                                                                       1 + len(ref["ifp"]))
+    assert open(base + ".obj").read() == want
+    bad = bt.copy()
+    bad[5, 1] = m.xyz.shape[0]
+    with pytest.raises(fa._lib.FtError):
+        r.write_obj(str(tmp_path / "bad"), boundary=bad)
+    r.close()
