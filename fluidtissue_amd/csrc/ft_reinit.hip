@@ -4376,6 +4376,8 @@ int ft_get_mesh(ft_handle h, float* xyz, int32_t* tets, int32_t* edges, int32_t*
 // and copies only their node coordinates and slots; it holds a reference to
 // each, so a mesh released while a batch still refers to it is freed when
 // that batch moves on (its id is free at once).
+}  // extern "C" (the registry below is C++)
+
 namespace {
 // Ids of released meshes are reused; the serial is not (it grows with every
 // mesh created in the process), so (id, serial) names one mesh for good.
@@ -4414,6 +4416,8 @@ std::shared_ptr<ft_ctx> resident_get(int32_t id, uint64_t* serial = nullptr)
     return g_res[id].c;
 }
 }  // namespace
+
+extern "C" {
 
 int ft_mesh_create_kuhn(int32_t device, int32_t nx, int32_t ny, int32_t nz, double jitter,
                         uint64_t seed, int32_t* id)
