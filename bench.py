@@ -488,7 +488,7 @@ def gpu_sample(r, phi, mode, band, stream, launch):
     return dict(ifn=d.cpu().numpy(), near=r.nearest(), itf=r.interface())
 
 
-def accuracy_check(m, phi, h, mode, band, gpu, n_sample=2000, seed=3):
+def accuracy_check(m, phi, h, mode, band, gpu, n_sample=2000, seed=3, nodes=None):
     """The oracle as the checker of the GPU sample (run in the CPU leg): the
     interface against the oracle's marching tets (bit-exact), every sign,
     and a seeded node sample's distances against the oracle's FP64 brute
@@ -505,8 +505,10 @@ def accuracy_check(m, phi, h, mode, band, gpu, n_sample=2000, seed=3):
                  np.array_equal(itf.ifp2e, ifp2e) and np.array_equal(itf.tris, tris) and
                  np.array_equal(itf.if_elems, elems) and
                  np.array_equal(itf.ifp_area.view(np.uint32), area.view(np.uint32)))
-    rng = np.random.default_rng(seed)
-    nodes = rng.choice(m.n_nodes, n_sample, replace=False)
+    if nodes is None:
+        rng = np.random.default_rng(seed)
+        nodes = rng.choice(m.n_nodes, n_sample, replace=False)
+    n_sample = len(nodes)
     got = np.abs(ifn[nodes])
     signs = bool(np.array_equal(np.signbit(ifn), phi < 0.0))
     if mode == "points":
@@ -558,6 +560,92 @@ def accuracy_check(m, phi, h, mode, band, gpu, n_sample=2000, seed=3):
         out["outside_band_clamped_exactly"] = clamped_ok
     out["seconds"] = round(time.perf_counter() - t0, 1)
     return out
+
+
+def batched_fields(k_count, seed=7, lo=24, hi=96):
+    """The first k_count fields of the batched workload: sizes N ~ U{lo..hi}
+    (seeded as run_batched draws them) and each field's ellipsoids."""
+    import fluidtissue_amd as fa
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, size=512)[:k_count]
+    out = []
+    for k, n in enumerate(sizes):
+        m = fa.kuhn_mesh(int(n))
+        out.append((int(n), m, fa.phi_field("ellipsoids", m.xyz, seed=1000 + k)))
+    return out
+
+
+def config_samples(mode):
+    """GPU side of the per-config accuracy (no oracle here): one reinit of
+    each of the other configs' fields, keeping the nearest primitive.
+      sphere_wave_32: the 32^3 field, full domain, every node checked later;
+      star_128 band: the 6h band, 1500 nodes the GPU puts inside the band
+        and 500 anywhere;
+      batched: 4 fields of the 512-field batch, reinitialised as one batch
+        through the captured graph and each also alone; the batch's phi must
+        equal the solo runs' bit for bit, and the solo runs carry the
+        interface and the nearest index;
+      served sizes: a 96^3 jittered field of the 64 spheres (the serving
+        stream's sizes are 32^3-192^3 from these generators)."""
+    import torch
+    import fluidtissue_amd as fa
+    out = {}
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream      # the stream torch copies phi on
+
+    def solo(m, phi, band):
+        r = fa.Reinit(0)
+        r.set_mesh(m)
+        g = gpu_sample(r, phi, mode, band, st, {})
+        r.close()
+        return g
+
+    m = fa.kuhn_mesh(32)
+    phi = fa.phi_field("sphere_wave", m.xyz)
+    out["sphere_wave_32"] = dict(m=m, phi=phi, h=1.0 / 31, band=0.0, gpu=solo(m, phi, 0.0), all_nodes=True)
+    m = fa.kuhn_mesh(128)
+    phi = fa.phi_field("star", m.xyz)
+    h = 1.0 / 127
+    g = solo(m, phi, 6.0 * h)
+    rng = np.random.default_rng(11)
+    inside = np.nonzero(np.abs(g["ifn"]) < 6.0 * h)[0]
+    nodes = np.unique(np.concatenate([rng.choice(inside, min(1500, len(inside)), replace=False),
+                                      rng.choice(m.n_nodes, 500, replace=False)]))
+    out["star_128_band6h"] = dict(m=m, phi=phi, h=h, band=6.0 * h, gpu=g, nodes=nodes)
+    fields = batched_fields(4)
+    b = fa.Reinit(0)
+    b.set_batch([m for _, m, _ in fields])
+    d = torch.from_numpy(np.concatenate([p for _, _, p in fields])).to(dev)
+    b.reset_phi_graph_device(d.data_ptr(), None, mode=mode, stream=st)
+    torch.cuda.synchronize()
+    batch_phi = d.cpu().numpy()
+    b.close()
+    off = 0
+    for k, (n, m, phi) in enumerate(fields):
+        g = solo(m, phi, 0.0)
+        same = bool(np.array_equal(batch_phi[off:off + m.n_nodes].view(np.uint64), g["ifn"].view(np.uint64)))
+        off += m.n_nodes
+        out[f"batched_field{k}_{n}^3"] = dict(m=m, phi=phi, h=1.0 / (n - 1), band=0.0, gpu=g,
+                                             batch_equals_solo=same, n_sample=1000)
+    m = fa.kuhn_mesh(96, jitter=0.2, seed=1)
+    phi = fa.phi_field("spheres", m.xyz, seed=0)
+    out["served_size_spheres_96"] = dict(m=m, phi=phi, h=1.0 / 95, band=0.0, gpu=solo(m, phi, 0.0))
+    return out
+
+
+def config_accuracy(samples, mode):
+    """CPU leg: the oracle checks each config's GPU sample."""
+    res = {}
+    for name, c in samples.items():
+        m = c["m"]
+        nodes = np.arange(m.n_nodes) if c.get("all_nodes") else c.get("nodes")
+        a = accuracy_check(m, c["phi"], c["h"], mode, c["band"], c["gpu"], n_sample=c.get("n_sample", 2000),
+                           nodes=nodes)
+        a["nodes_total"] = int(m.n_nodes)
+        if "batch_equals_solo" in c:
+            a["batch_equals_solo_bitwise"] = c["batch_equals_solo"]
+        res[name] = a
+    return res
 
 
 def _read_counters(dirpath):
@@ -814,6 +902,12 @@ def main():
                             dict(bin_scale=args.bin_scale, query_block=args.query_block,
                                  waves_per_simd=args.waves_per_simd, schedule=args.schedule))
 
+    cfg_samples = None
+    if not args.no_accuracy and not args.no_cpu and args.workload == "spheres_256":
+        t_c = time.perf_counter()
+        cfg_samples = config_samples(args.mode)
+        log(f"per-config GPU samples: {len(cfg_samples)} ({time.perf_counter() - t_c:.1f} s)")
+
     # the drop-in path with host phi (ft_reinit on the caller's own arrays, as
     # LevelSet::resetPhi_ would call it: H2D of phi, the reinit, D2H of phi and
     # phi_ifn); reported beside the device-resident rate, never as `value`
@@ -847,7 +941,7 @@ def main():
 
     # The CPU leg: the oracle on the host, as the reference CPU path's timed
     # baseline and as the checker of the GPU sample above.
-    cpu = accuracy = None
+    cpu = accuracy = accuracy_cfg = None
     hinfo = host_info()
     if not args.no_cpu:
         cpu = cpu_baseline(m, phi, hinfo["threads_used"])
@@ -857,6 +951,14 @@ def main():
         if sample is not None:
             accuracy = accuracy_check(m, phi, h, args.mode, band, sample)
             log(f"accuracy: {accuracy}")
+        if cfg_samples is not None:
+            t_c = time.perf_counter()
+            accuracy_cfg = config_accuracy(cfg_samples, args.mode)
+            for k, v in accuracy_cfg.items():
+                log(f"accuracy {k}: max {v['max_err_h']:.3g} h, rms {v['rms_err_h']:.3g} h, "
+                    f"mismatches {v['index_mismatches']}, ties {v['eps_ties']}, "
+                    f"interface exact {v['interface_bit_exact']}, signs exact {v['signs_bit_exact_all_nodes']}")
+            log(f"per-config accuracy: {time.perf_counter() - t_c:.1f} s")
 
     served = None
     if not args.no_served:
@@ -904,6 +1006,7 @@ def main():
                           "achieved_GBps": b_path / (ms * 1e-3) / 1e9,
                           "frac": b_path / (ms * 1e-3) / 1e9 / peak},
         "accuracy": accuracy,
+        "accuracy_by_config": accuracy_cfg,
         "counters": counters,
         "served": served,
         "cpu_baseline": cpu,
