@@ -936,7 +936,11 @@ def main():
 
     counters = None
     if not args.no_counters:
-        counters = live_counters(args, N, ph[2], os.path.join(ROOT, "gpurun_out", "bench_counters"))
+        # raw counter files: FT_BENCH_PROFILE_DIR if set, else a temporary
+        # folder (the JSON line carries what they say)
+        import tempfile
+        out_root = os.environ.get("FT_BENCH_PROFILE_DIR") or tempfile.mkdtemp(prefix="ft_bench_counters_")
+        counters = live_counters(args, N, ph[2], out_root)
         log(f"counters: { {k: v for k, v in counters.items() if k != 'commands'} }")
 
     # The CPU leg: the oracle on the host, as the reference CPU path's timed
