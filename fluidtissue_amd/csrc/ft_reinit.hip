@@ -2753,6 +2753,13 @@ struct ft_ctx {
     int device = 0;
     int cu_count = 256;              // compute units of the device
     hipStream_t stream = nullptr;
+    // A second stream for the stages the query does not read (the interface's
+    // point-to-triangle lists and areas) and the query's tile schedule: they
+    // run beside the index build.  side_fork/side_join order it against the
+    // calling stream; side_open: forked and not joined yet.
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork_ev = nullptr, side_join_ev = nullptr;
+    bool side_open = false;
     // fields and mesh (field k: nodes [node_off[k], node_off[k+1]), ...)
     int nf = 0;
     std::vector<int64_t> node_off, tet_off, edge_off, slot_off;
@@ -2809,6 +2816,7 @@ struct ft_ctx {
     int64_t spec_hits = 0;
     // scan scratch and small scalars
     DevBuf scan_bsum, scalars;
+    DevBuf scan_side;                // scan scratch of the side stream
     uint64_t* h_scalars = nullptr;   // pinned mirror
     ft_counts last{};
     std::vector<ft_counts> field_counts;
@@ -2841,7 +2849,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
             &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
-            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scalars,
+            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scan_side, &h->scalars,
             &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
             &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f,
             &h->fem_len, &h->fem_w, &h->fem_soff, &h->fem_cols, &h->fem_m, &h->fem_l, &h->fem_flag, &h->fem_map,
@@ -2911,12 +2919,14 @@ int fill(hipStream_t s, T* p, int64_t n, T v)
 }
 
 // Exclusive scan of n elements; the total lands in *d_total (device).
+// ws: the block-sum scratch (default: the handle's; each stream needs its own).
 template <class TI, class TO, class Ld>
-int scan(ft_ctx* c, hipStream_t s, const TI* in, TO* out, int64_t n, TO* d_total, Ld ld)
+int scan(ft_ctx* c, hipStream_t s, const TI* in, TO* out, int64_t n, TO* d_total, Ld ld, DevBuf* ws = nullptr)
 {
+    DevBuf& w = ws ? *ws : c->scan_bsum;
     const int64_t nb = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
-    FT_TRY(ensure(c->scan_bsum, sizeof(TO) * (size_t)nb));
-    TO* bs = c->scan_bsum.as<TO>();
+    FT_TRY(ensure(w, sizeof(TO) * (size_t)nb));
+    TO* bs = w.as<TO>();
     if (n > 0) {
         hipLaunchKernelGGL((k_scan_reduce<TI, TO, Ld>), dim3((unsigned)nb), dim3(kBS), 0, s, in, n, bs, ld);
     } else {
@@ -3280,6 +3290,8 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
         FT_TRY(ensure(c->p2t_off, 4 * (cp.P + 1)));
         FT_TRY(ensure(c->p2t_cursor, 4 * at_least1(cp.P)));
         FT_TRY(ensure(c->p2t, 4 * at_least1(3 * cp.F)));
+        // the side stream's scans: the point lists, the tile schedule
+        FT_TRY(ensure(c->scan_side, sizeof(uint64_t) * std::max(scan_blocks(cp.P), scan_blocks(c->n_slots / 64))));
     } else {
         const int64_t n = mode == 0 ? cp.P : cp.F;
         const int64_t G = c->idx_G;
@@ -3467,6 +3479,33 @@ MeshView mesh_view(ft_ctx* c)
 // reads and no allocation; the buffers were sized for `cp` beforehand and
 // k_check_caps guards them.  counts_only: stops once the totals are known
 // (phi is not written).
+// Work enqueued on c->side from now on runs after everything enqueued on s
+// so far.
+int side_fork(ft_ctx* c, hipStream_t s)
+{
+    FT_HIP(hipEventRecord(c->side_fork_ev, s));
+    FT_HIP(hipStreamWaitEvent(c->side, c->side_fork_ev, 0));
+    c->side_open = true;
+    return FT_OK;
+}
+
+// Work enqueued on s from now on runs after everything enqueued on c->side.
+int side_join(ft_ctx* c, hipStream_t s)
+{
+    if (!c->side_open) return FT_OK;
+    c->side_open = false;
+    FT_HIP(hipEventRecord(c->side_join_ev, c->side));
+    FT_HIP(hipStreamWaitEvent(s, c->side_join_ev, 0));
+    return FT_OK;
+}
+
+// Joins the side stream into s when it leaves scope (early returns included).
+struct SideJoin {
+    ft_ctx* c;
+    hipStream_t s;
+    ~SideJoin() { (void)side_join(c, s); }
+};
+
 int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_nearest,
                      bool want_normals, Caps& cp, bool eager, bool counts_only, ft_counts* cnt,
                      bool timed)
@@ -3532,20 +3571,26 @@ int enqueue_marching(ft_ctx* c, hipStream_t s, double* phi, int mode, bool want_
                            phi, c->xtet.as<int32_t>(), c->xtri_off.as<int32_t>(), c->xedges.as<int4>(),
                            c->ebits.as<uint64_t>(), c->eprefix.as<uint32_t>(), c->ifp.as<float>(),
                            c->tris.as<int4>(), c->tri_norm.as<float>(), c->if_elems.as<int32_t>(), sc);
-    // p2t and areas; entries past the point count stay empty
-    uint32_t* deg = c->p2t_cursor.as<uint32_t>();
-    FT_TRY(fill<uint32_t>(s, deg, cp.P, 0u));
-    if (cp.F > 0)
-        hipLaunchKernelGGL(k_p2t_count, dim3(grid1d(cp.F)), dim3(kBS), 0, s, c->tris.as<int4>(), sc, deg);
-    FT_TRY(scan<uint32_t, uint32_t>(c, s, deg, c->p2t_off.as<uint32_t>(), cp.P,
-                                    c->p2t_off.as<uint32_t>() + cp.P, LoadSame{}));
-    FT_TRY(fill<uint32_t>(s, deg, cp.P, 0u));
-    if (cp.F > 0)
-        hipLaunchKernelGGL(k_p2t_fill, dim3(grid1d(cp.F)), dim3(kBS), 0, s, c->tris.as<int4>(), sc,
-                           c->p2t_off.as<uint32_t>(), deg, c->p2t.as<int32_t>());
-    if (cp.P > 0)
-        hipLaunchKernelGGL(k_point_area, dim3(grid1d(cp.P)), dim3(kBS), 0, s, c->p2t_off.as<uint32_t>(),
-                           sc, c->p2t.as<int32_t>(), c->tri_norm.as<float>(), c->ifp_area.as<float>());
+    // p2t and areas; entries past the point count stay empty.  The query
+    // reads neither: they run on the side stream, beside the index build,
+    // and the query's stream joins it before the query.
+    FT_TRY(side_fork(c, s));
+    {
+        hipStream_t ss = c->side;
+        uint32_t* deg = c->p2t_cursor.as<uint32_t>();
+        FT_TRY(fill<uint32_t>(ss, deg, cp.P, 0u));
+        if (cp.F > 0)
+            hipLaunchKernelGGL(k_p2t_count, dim3(grid1d(cp.F)), dim3(kBS), 0, ss, c->tris.as<int4>(), sc, deg);
+        FT_TRY(scan<uint32_t, uint32_t>(c, ss, deg, c->p2t_off.as<uint32_t>(), cp.P,
+                                        c->p2t_off.as<uint32_t>() + cp.P, LoadSame{}, &c->scan_side));
+        FT_TRY(fill<uint32_t>(ss, deg, cp.P, 0u));
+        if (cp.F > 0)
+            hipLaunchKernelGGL(k_p2t_fill, dim3(grid1d(cp.F)), dim3(kBS), 0, ss, c->tris.as<int4>(), sc,
+                               c->p2t_off.as<uint32_t>(), deg, c->p2t.as<int32_t>());
+        if (cp.P > 0)
+            hipLaunchKernelGGL(k_point_area, dim3(grid1d(cp.P)), dim3(kBS), 0, ss, c->p2t_off.as<uint32_t>(),
+                               sc, c->p2t.as<int32_t>(), c->tri_norm.as<float>(), c->ifp_area.as<float>());
+    }
     if (c->nf > 1)
         hipLaunchKernelGGL(k_field_counts, dim3((c->nf + 63) / 64), dim3(64), 0, s, c->nf,
                            c->d_tet_off.as<int64_t>(), c->d_edge_off.as<int64_t>(), T, E,
@@ -3601,6 +3646,24 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     if (c->idx_layout != c->layout_gen)
         return ft::set_error(FT_ERR_STATE, "index grids not set up for this field layout");
     FT_TRY(ensure_stage(c, 3, cp, mode, sp.want_nearest));
+    // Deep tiles first, in a full-domain search (in a narrow band they end
+    // early, clamped).  The schedule reads only phi and the slots: it is
+    // built on the side stream while the index is built here.
+    const int64_t n_tiles = c->n_slots / 64;
+    int32_t* tile_list = nullptr;
+    if (sp.band <= 0.0f && n_tiles > 0) {
+        FT_TRY(side_fork(c, s));
+        hipStream_t ss = c->side;
+        uint32_t* deep = c->tile_sched.as<uint32_t>();
+        tile_list = reinterpret_cast<int32_t*>(deep + n_tiles);
+        FT_TRY(fill<uint64_t>(ss, sc + S_DEEP, 1, 0ull));   // the scan writes the low 32 bits
+        hipLaunchKernelGGL(k_tile_deep, dim3(grid1d(64 * n_tiles)), dim3(kBS), 0, ss, n_tiles,
+                           c->perm.as<int32_t>(), sp.phi_old, c->tile_field.as<int32_t>(),
+                           c->d_grids.as<Grid>(), deep, sc);
+        FT_TRY(scan<uint32_t, uint32_t>(c, ss, deep, deep, n_tiles, reinterpret_cast<uint32_t*>(sc + S_DEEP),
+                                        LoadSame{}, &c->scan_side));
+        hipLaunchKernelGGL(k_tile_list, dim3(grid1d(n_tiles)), dim3(kBS), 0, ss, n_tiles, deep, sc, tile_list);
+    }
     const int64_t G = c->idx_G;
     const int64_t L_cap = std::min(G, n_cap);
     IndexGeom ig{c->d_edge_off.as<int64_t>(), c->d_cnt_off.as<int64_t>(), c->d_grids.as<Grid>(), nf,
@@ -3687,20 +3750,9 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     q.sc = sc;
     q.xcd_chunk = kXcdChunk;
     if (timed) (void)hipEventRecord(c->ev[2], s);
-    // Deep tiles first, in a full-domain search (in a narrow band they end
-    // early, clamped).
-    const int64_t n_tiles = c->n_slots / 64;
-    if (sp.band <= 0.0f && n_tiles > 0) {
-        uint32_t* deep = c->tile_sched.as<uint32_t>();
-        int32_t* list = reinterpret_cast<int32_t*>(deep + n_tiles);
-        FT_TRY(fill<uint64_t>(s, sc + S_DEEP, 1, 0ull));   // the scan writes the low 32 bits
-        hipLaunchKernelGGL(k_tile_deep, dim3(grid1d(64 * n_tiles)), dim3(kBS), 0, s, n_tiles,
-                           c->perm.as<int32_t>(), sp.phi_old, c->tile_field.as<int32_t>(),
-                           c->d_grids.as<Grid>(), deep, sc);
-        FT_TRY(scan<uint32_t, uint32_t>(c, s, deep, deep, n_tiles, reinterpret_cast<uint32_t*>(sc + S_DEEP),
-                                        LoadSame{}));
-        hipLaunchKernelGGL(k_tile_list, dim3(grid1d(n_tiles)), dim3(kBS), 0, s, n_tiles, deep, sc, list);
-        q.tile_list = list;
+    FT_TRY(side_join(c, s));                          // the schedule and the point lists
+    if (tile_list) {
+        q.tile_list = tile_list;
         q.n_deep = sc + S_DEEP;
     }
     if (c->n_slots > 0) {
@@ -3858,6 +3910,7 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     cnt.bad_tet = -1;
     Caps cp;
     const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
+    SideJoin join{c, s};
     int rc = index_setup(c, s, o->bin_scale, sp.mode);
     if (rc == FT_OK && sp.want_normals) rc = build_node_tets(c, s);
     // Speculative pass: the capacities of the last call on these fields, no
@@ -3941,6 +3994,8 @@ int capture_reinit(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o
     int rc = enqueue_marching(c, c->stream, d_phi, sp.mode, sp.want_nearest, sp.want_normals, cc, false,
                               false, &unused, false);
     if (rc == FT_OK) rc = enqueue_search(c, c->stream, sp, cc, false, false);
+    const int rj = side_join(c, c->stream);          // the capture must end joined
+    if (rc == FT_OK) rc = rj;
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc != FT_OK) {
@@ -4163,9 +4218,22 @@ int ft_create(int32_t device, ft_handle* out)
         delete c;
         return ft::set_error(FT_ERR_HIP, "hipStreamCreate failed");
     }
+    // the side stream is ordered by events alone (default priority, as the other)
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_join_ev, hipEventDisableTiming) != hipSuccess) {
+        if (c->side_fork_ev) (void)hipEventDestroy(c->side_fork_ev);
+        if (c->side) (void)hipStreamDestroy(c->side);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ft::set_error(FT_ERR_HIP, "side stream set-up failed");
+    }
     if (ensure(c->scalars, 8 * S_N) != FT_OK ||
         hipHostMalloc((void**)&c->h_scalars, 8 * S_N, hipHostMallocDefault) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->side);
+        (void)hipEventDestroy(c->side_fork_ev);
+        (void)hipEventDestroy(c->side_join_ev);
         release(c->scalars);
         delete c;
         return ft::set_error(FT_ERR_NOMEM, "scalar buffers");
@@ -4178,6 +4246,7 @@ int ft_destroy(ft_handle h)
 {
     if (!h) return FT_OK;
     (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->side);
     (void)hipStreamSynchronize(h->stream);
     drop_graph(h);
     for (DevBuf* b : all_buffers(h)) release(*b);
@@ -4189,6 +4258,9 @@ int ft_destroy(ft_handle h)
     }
     for (int k = 0; k < 5; k++)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    (void)hipEventDestroy(h->side_fork_ev);
+    (void)hipEventDestroy(h->side_join_ev);
+    (void)hipStreamDestroy(h->side);
     (void)hipStreamDestroy(h->stream);
     delete h;
     return FT_OK;
@@ -4963,6 +5035,7 @@ int ft_reset_phi_points(ft_handle h, const float* ifp, int64_t n_ifp, double* ph
     FT_TRY(fill<uint64_t>(s, sc + S_ERR, 1, ~0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_CAP, 1, 0ull));
     FT_TRY(fill<uint64_t>(s, sc + S_OVF, 1, 0ull));
+    SideJoin join{h, s};
     FT_TRY(enqueue_search(h, s, sp, cp, true, false));
     ft_counts cnt{};
     FT_TRY(collect(h, s, true, &cnt));
