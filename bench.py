@@ -26,6 +26,8 @@ WORKLOADS = {
     "star_128": dict(n=128, jitter=0.0, phi="star", band_h=6.0, desc="128^3 Kuhn, star surface"),
     "spheres_256": dict(n=256, jitter=0.2, phi="spheres",
                  desc="256^3 Kuhn, interior nodes jittered +-0.2h, 64 spheres"),
+    "spheres_256_band6h": dict(n=256, jitter=0.2, phi="spheres", band_h=6.0,
+                               desc="256^3 Kuhn, interior nodes jittered +-0.2h, 64 spheres, 6h band"),
     "spheres_128": dict(n=128, jitter=0.2, phi="spheres",
                      desc="128^3 Kuhn, interior nodes jittered +-0.2h, 64 spheres (the 256^3 field, coarser)"),
 }
