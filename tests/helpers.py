@@ -28,6 +28,13 @@ def workload(name):
     if name == "ellipsoids":
         m = fa.kuhn_mesh(40)
         return m, fa.phi_field("ellipsoids", m.xyz, seed=5), 1.0 / 39
+    if name == "reference_domain_40":
+        # the reference's own domain layout (x = -dim + i h, dim = 2: the
+        # cube [-2, 2]^3, TetraMesh.cpp:199-211), 40^3 nodes jittered +-0.2h,
+        # the ellipsoids field of the unit cube carried along
+        m = fa.kuhn_mesh(40, origin=(-2.0, -2.0, -2.0), spacing=4.0 / 39, jitter=0.2, seed=3)
+        unit = ((m.xyz.astype(np.float64) + 2.0) / 4.0).astype(np.float32)
+        return m, fa.phi_field("ellipsoids", unit, seed=5), 4.0 / 39
     raise KeyError(name)
 
 
