@@ -28,6 +28,10 @@ def workload(name):
     if name == "ellipsoids":
         m = fa.kuhn_mesh(40)
         return m, fa.phi_field("ellipsoids", m.xyz, seed=5), 1.0 / 39
+    if name == "slab_48x32x16":
+        # a non-cubic grid (the bin grid is cubic over the longest side)
+        m = fa.kuhn_mesh((48, 32, 16), spacing=1.0 / 47, jitter=0.2, seed=4)
+        return m, fa.phi_field("spheres", m.xyz, seed=2), 1.0 / 47
     if name == "reference_domain_40":
         # the reference's own domain layout (x = -dim + i h, dim = 2: the
         # cube [-2, 2]^3, TetraMesh.cpp:199-211), 40^3 nodes jittered +-0.2h,
