@@ -28,6 +28,21 @@ def workload(name):
     if name == "ellipsoids":
         m = fa.kuhn_mesh(40)
         return m, fa.phi_field("ellipsoids", m.xyz, seed=5), 1.0 / 39
+    if name == "shuffled_32":
+        # the 32^3 field with node ids and tet order permuted at random: no
+        # spatial order left in either list
+        m = fa.kuhn_mesh(32, jitter=0.2, seed=6)
+        phi = fa.phi_field("sphere_wave", m.xyz)
+        rng = np.random.default_rng(8)
+        new_of = rng.permutation(m.n_nodes).astype(np.int32)      # old id -> new id
+        xyz = np.empty_like(m.xyz)
+        xyz[new_of] = m.xyz
+        phi2 = np.empty_like(phi)
+        phi2[new_of] = phi
+        order = rng.permutation(len(m.tets))
+        m = fa.Mesh(xyz=xyz, tets=np.ascontiguousarray(new_of[m.tets[order]]),
+                    edges=np.ascontiguousarray(new_of[m.edges]), t2e=np.ascontiguousarray(m.t2e[order]))
+        return m, phi2, 1.0 / 31
     if name == "slab_48x32x16":
         # a non-cubic grid (the bin grid is cubic over the longest side)
         m = fa.kuhn_mesh((48, 32, 16), spacing=1.0 / 47, jitter=0.2, seed=4)

@@ -32,7 +32,7 @@ def _run(rein, name, **kw):
 
 
 @pytest.mark.parametrize("name", ["sphere_wave_32", "star_64", "spheres_96", "ellipsoids", "reference_domain_40",
-                                  "slab_48x32x16"])
+                                  "slab_48x32x16", "shuffled_32"])
 def test_marching_tets_bit_exact(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name)
     ref = oracle_interface(name)
@@ -47,7 +47,8 @@ def test_marching_tets_bit_exact_star_128(rein):
     assert_interface_bit_exact(rein.interface(), oracle_interface("star_128"))
 
 
-@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids", "reference_domain_40", "slab_48x32x16"])
+@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids", "reference_domain_40", "slab_48x32x16",
+                                  "shuffled_32"])
 def test_points_mode_bit_exact_full(rein, name):
     """Reference semantics (EuclideanDistance.cu:14-35,108-117) on every node."""
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="points", blend="init", want_nearest=True)
@@ -97,7 +98,8 @@ def test_points_mode_bit_exact_sampled(rein, name):
     assert np.array_equal(rein.nearest()[nodes], j)
 
 
-@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids", "reference_domain_40", "slab_48x32x16"])
+@pytest.mark.parametrize("name", ["sphere_wave_32", "ellipsoids", "reference_domain_40", "slab_48x32x16",
+                                  "shuffled_32"])
 def test_triangle_mode_full(rein, name):
     m, phi, h, out, ifn, cnt = _run(rein, name, mode="triangles", want_nearest=True)
     nodes = np.arange(m.n_nodes)
