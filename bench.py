@@ -662,6 +662,10 @@ def _read_counters(dirpath):
                 d = rows.setdefault(key, {})
                 name = row.get("Counter_Name", "")
                 d[name] = d.get(name, 0.0) + float(row.get("Counter_Value", 0.0))
+                try:                            # the dispatch's own duration, ns
+                    d["duration_ns"] = float(row["End_Timestamp"]) - float(row["Start_Timestamp"])
+                except (KeyError, ValueError):
+                    pass
     return rows
 
 
@@ -727,8 +731,13 @@ def live_counters(args, N, q_ms, out_root):
         res["salu_per_wave"] = (last(got["sq"], qk, "SQ_INSTS_SALU") or 0.0) / waves
         res["smem_per_wave"] = (last(got["sq"], qk, "SQ_INSTS_SMEM") or 0.0) / waves
         # each wave64 VALU instruction issues over 4 cycles of a 16-lane SIMD;
-        # 256 CUs x 4 SIMDs at 2.4 GHz, over the kernel's HIP-event time
-        res["valu_issue_frac"] = valu * 4.0 / (1024 * 2.4e9 * q_ms * 1e-3)
+        # 256 CUs x 4 SIMDs at 2.4 GHz, over the profiled dispatch's own
+        # duration (its timestamps in the same counter pass)
+        dur = last(got["sq"], qk, "duration_ns")
+        if dur:
+            res["query_dispatch_ms_profiled"] = dur * 1e-6
+            res["valu_issue_frac"] = valu * 4.0 / (1024 * 2.4e9 * dur * 1e-9)
+
     return res
 
 
