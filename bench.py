@@ -169,7 +169,7 @@ def run_batched(args):
 
     rng = np.random.default_rng(7)
     nfields = args.fields
-    sizes = rng.integers(24, 97, size=nfields)
+    sizes = rng.integers(args.min_n, args.max_n + 1, size=nfields)
     t0 = time.perf_counter()
     ids = {}
     for n in sorted(set(int(x) for x in sizes)):
@@ -257,12 +257,12 @@ def run_batched(args):
     for mid in ids.values():
         lib.ft_mesh_release(mid)
     return {
-        "metric": "Mnodes/s batched reinit (512 fields, one captured graph) vs serial plain calls",
+        "metric": f"Mnodes/s batched reinit ({nfields} fields, one captured graph) vs serial plain calls",
         "value": N / (ms_batched * 1e-3) / 1e6, "unit": "Mnodes/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batched,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 distance search; f64 phi and crossing points", "data": "synthetic",
-        "config": {"workload": f"batched: {nfields} fields N~U{{24..96}}, 1-16 ellipsoids, "
+        "config": {"workload": f"batched: {nfields} fields N~U{{{args.min_n}..{args.max_n}}}, 1-16 ellipsoids, "
                                f"{args.mode} distance", "nodes": N, "fields": nfields},
         "serial_plain": {"ms": ms_serial, "Mnodes_per_s": N / (ms_serial * 1e-3) / 1e6},
         "eager_batched": {"ms": ms_eager, "Mnodes_per_s": N / (ms_eager * 1e-3) / 1e6},
@@ -791,6 +791,8 @@ def main():
     ap.add_argument("--workload", default="spheres_256", choices=sorted(WORKLOADS) + ["batched", "serving", "advect"])
     ap.add_argument("--advect-n", type=int, default=256)
     ap.add_argument("--fields", type=int, default=512)
+    ap.add_argument("--min-n", type=int, default=24, help="batched: smallest field side")
+    ap.add_argument("--max-n", type=int, default=96, help="batched: largest field side")
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--rates", type=float, nargs="+", default=[50.0, 200.0, 500.0])
     ap.add_argument("--lanes", type=int, default=2, help="serving: worker lanes (handles and streams)")
