@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kBS) void k_compact(const uint64_t* tbits, const ui
 #pragma unroll
     for (int o = 1; o < 4; o <<= 1) {
         const uint64_t t = ftd::shfl_up_u64(inc, o, 4);
-        if ((threadIdx.x & 3) >= o) inc += t;
+        if ((int)(threadIdx.x & 3) >= o) inc += t;
     }
     if (w >= nwords || word == 0) return;
     const uint64_t base = blk_prefix[w >> 2] + (inc - mine);
@@ -3853,7 +3853,7 @@ void print_query_stats(ft_ctx* c, int mode)
     }
 }
 
-SearchSpec reinit_search_spec(ft_ctx* c, double* phi, double* phi_ifn, const ft_opts* o)
+SearchSpec reinit_search_spec(double* phi, double* phi_ifn, const ft_opts* o)
 {
     SearchSpec sp;
     sp.mode = o->mode == FT_DIST_POINTS ? 0 : 1;
@@ -3909,7 +3909,7 @@ int reinit_device(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o,
     ft_counts cnt{};
     cnt.bad_tet = -1;
     Caps cp;
-    const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
+    const SearchSpec sp = reinit_search_spec(d_phi, d_phi_ifn, o);
     SideJoin join{c, s};
     int rc = index_setup(c, s, o->bin_scale, sp.mode);
     if (rc == FT_OK && sp.want_normals) rc = build_node_tets(c, s);
@@ -3984,7 +3984,7 @@ void drop_graph(ft_ctx* c)
 int capture_reinit(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o, const Caps& cp)
 {
     drop_graph(c);
-    const SearchSpec sp = reinit_search_spec(c, d_phi, d_phi_ifn, o);
+    const SearchSpec sp = reinit_search_spec(d_phi, d_phi_ifn, o);
     if (sp.want_normals) FT_TRY(build_node_tets(c, c->stream));
     for (int st = 0; st < 4; st++) FT_TRY(ensure_stage(c, st, cp, sp.mode, sp.want_nearest, sp.want_normals));
     FT_HIP(hipStreamSynchronize(c->stream));
