@@ -4120,7 +4120,8 @@ std::vector<DevBuf*> call_buffers(ft_ctx* h)
             &h->nearest, &h->fcounts, &h->fclamp, &h->normals, &h->grad_t, &h->phi_ifn_int,
             &h->bin_count, &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent, &h->bnodes, &h->bflags,
-            &h->stats, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots, &h->scan_bsum};
+            &h->stats, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots, &h->scan_bsum,
+            &h->scan_side};
 }
 
 // reinit_device with every per-call buffer taken from the device's scratch:
@@ -4159,6 +4160,19 @@ int reinit_noalloc(ft_ctx* c, double* d_phi, double* d_phi_ifn, const ft_opts* o
     c->idx_G = idx_G;
     c->idx_code_bits = idx_bits;
     c->have_interface = c->have_nearest = c->have_normals = false;
+    // Every buffer a reinit sizes must be in call_buffers; one that is not
+    // would keep pointing into the shared scratch after the call.  Such a
+    // buffer is dropped (not freed) and the call reports it.
+    const char* lo = static_cast<const char*>(base);
+    bool escaped = false;
+    for (DevBuf* b : all_buffers(c)) {
+        const char* p = static_cast<const char*>(b->p);
+        if (p && p >= lo && p < lo + kScratchBytes) {
+            *b = DevBuf{};
+            escaped = true;
+        }
+    }
+    if (escaped) return ft::set_error(FT_ERR_INTERNAL, "a per-call buffer was left in the noalloc scratch");
     if (rc == FT_OK && es != hipSuccess)
         return ft::set_error(FT_ERR_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(es));
     return rc;

@@ -41,6 +41,9 @@ def _pair(r, phi, mode, blend, stream):
 
 @pytest.mark.parametrize("mode", ["triangles", "points"])
 def test_noalloc_equals_reinit_device(mode):
+    """Each handle's first call is the allocation-free one: every buffer the
+    reinit sizes then comes from the scratch, and the library refuses the
+    call (FT_ERR_INTERNAL) if one of them were left pointing into it."""
     import torch
     stream = torch.cuda.current_stream().cuda_stream
     big, small = fa.kuhn_mesh(48, jitter=0.2, seed=1), fa.kuhn_mesh(30)
