@@ -1,5 +1,6 @@
 """GPU parity: the HIP reinit through the C ABI against the CPU oracle."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -345,6 +346,48 @@ def test_cpp_drop_in_link(tmp_path):
     assert "no HIP device" not in got["stderr"]
     for col in got["phi_cols"]:
         assert np.array_equal(col.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("mode", ["points", "triangles"])
+def test_cpp_level2_integration(tmp_path, mode):
+    """A C++ program drives LevelSet::resetPhi_'s Level-2 replacement through
+    the C ABI alone (mesh once, reinit with the init blend, interface and OBJ
+    read back, reinit with the update blend on the result), as documented for
+    integrators.  Its interface is the oracle's bit for bit; points mode is
+    bitwise the oracle's reinit for both blends; triangle mode is within
+    1e-5 h of the FP64 brute force, with the blends and retained values
+    bitwise the oracle's epilogue of those distances."""
+    from tests.drop_in import run_level2
+    from tests.helpers import expected_interface_obj
+    n = 24
+    r = run_level2(tmp_path, n, mode)
+    h = 1.0 / (n - 1)
+    xyz, tets, edges, t2e = r["xyz"], r["tets"], r["edges"], r["t2e"]
+    ref = oracle.reinit(xyz, tets, edges, t2e, r["phi0"], blend=0)
+    assert r["P"] == len(ref["ifp"]) > 0
+    assert np.array_equal(r["ifp"].view(np.uint32), ref["ifp"].view(np.uint32))
+    assert np.array_equal(r["ifp2e"], ref["ifp2e"]) and np.array_equal(r["tris"], ref["tris"])
+    assert np.array_equal(r["elems"], ref["if_elems"])
+    assert np.array_equal(r["area"].view(np.uint32), ref["ifp_area"].view(np.uint32))
+    off, p2t = oracle.p2t_lists(ref["tris"], len(ref["ifp"]))
+    assert np.array_equal(r["p2t_off"], off) and np.array_equal(r["p2t"], p2t)
+    base = os.path.join(r["dir"], "shape")
+    assert open(base + ".obj").read() == expected_interface_obj(ref["ifp"], ref["tris"], base, True)
+    if mode == "points":
+        assert np.array_equal(r["ifn1"].view(np.uint64), ref["phi_ifn"].view(np.uint64))
+        assert np.array_equal(r["phi1"].view(np.uint64), ref["phi"].view(np.uint64))
+        ref2 = oracle.reinit(xyz, tets, edges, t2e, r["phi1"], blend=1)
+        assert np.array_equal(r["ifn2"].view(np.uint64), ref2["phi_ifn"].view(np.uint64))
+        assert np.array_equal(r["phi2"].view(np.uint64), ref2["phi"].view(np.uint64))
+    else:
+        d64, _ = oracle.triangle_distance(xyz, ref["ifp"], ref["tris"])
+        assert np.max(np.abs(np.abs(r["ifn1"]) - d64)) <= 1e-5 * h
+        assert np.array_equal(np.signbit(r["ifn1"]), r["phi0"] < 0.0)
+        _, out1 = oracle.finish_reset(r["phi0"], r["ifn1"], ref["ifp2e"], edges, 0)
+        assert np.array_equal(r["phi1"].view(np.uint64), out1.view(np.uint64))
+        ifp1, ifp2e1 = oracle.find_interface_nodes(edges, xyz, r["phi1"])
+        _, out2 = oracle.finish_reset(r["phi1"], r["ifn2"], ifp2e1, edges, 1)
+        assert np.array_equal(r["phi2"].view(np.uint64), out2.view(np.uint64))
 
 
 def test_repeated_calls_outgrowing_capacities():
