@@ -1722,11 +1722,41 @@ __global__ __launch_bounds__(kBS) void k_n2t_fill(const int4* tets, int64_t T, c
     for (int k = 0; k < 4; k++) ids[off[v[k]] + atomicAdd(&cursor[v[k]], 1u)] = (int32_t)t;
 }
 
+// Each node's list sorted to increasing tet id: lists of up to kN2tReg ids in
+// registers by a bitonic network (padding sorts last), longer ones in place by
+// insertion.
+constexpr int kN2tReg = 32;
 __global__ __launch_bounds__(kBS) void k_n2t_sort(const uint64_t* off, int64_t N, int32_t* ids)
 {
     const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (i >= N) return;
     const uint64_t b = off[i], e = off[i + 1];
+    const int n = (int)(e - b);
+    if (n <= kN2tReg) {
+        int32_t v[kN2tReg];
+#pragma unroll
+        for (int a = 0; a < kN2tReg; a++) v[a] = a < n ? ids[b + a] : INT32_MAX;
+#pragma unroll
+        for (int k = 2; k <= kN2tReg; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (int a = 0; a < kN2tReg; a++) {
+                    const int l = a ^ j;
+                    if (l > a) {
+                        const int32_t x = v[a], y = v[l];
+                        const bool sw = (a & k) == 0 ? x > y : x < y;
+                        v[a] = sw ? y : x;
+                        v[l] = sw ? x : y;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kN2tReg; a++)
+            if (a < n) ids[b + a] = v[a];
+        return;
+    }
     for (uint64_t a = b + 1; a < e; a++) {
         const int32_t key = ids[a];
         uint64_t j = a;
