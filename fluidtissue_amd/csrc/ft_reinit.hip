@@ -1701,25 +1701,85 @@ __global__ __launch_bounds__(BS, WPS) void k_query_wave(QueryArgs a)
 // ============================================================================
 // Node -> tets lists (Mesh3D::p2t, TetraMesh.cpp:84-86, 115-131): counts,
 // scan, scatter, then each list sorted to increasing tet id.
+// Node -> tets counts and fills with the references of a workgroup's tets
+// gathered first in an LDS table keyed by node (consecutive tets share most
+// of their nodes): one memory atomic per distinct node of the workgroup
+// instead of one per reference.  A reference that finds no slot within
+// kN2tProbe probes goes to memory on its own.  The order inside a node's
+// list is arbitrary here; k_n2t_sort makes it the tet order.
+constexpr int kN2tHash = 1024;            // LDS slots per workgroup (power of 2)
+constexpr int kN2tProbe = 16;
+__device__ __forceinline__ int n2t_slot(int32_t* key, int32_t v)
+{
+    uint32_t h = ((uint32_t)v * 2654435761u) >> (32 - 10);
+    for (int p = 0; p < kN2tProbe; p++) {
+        const int32_t prev = atomicCAS(&key[h], -1, v);
+        if (prev == -1 || prev == v) return (int)h;
+        h = (h + 1) & (kN2tHash - 1);
+    }
+    return -1;
+}
+
 __global__ __launch_bounds__(kBS) void k_n2t_count(const int4* tets, int64_t T, uint32_t* cnt)
 {
+    static_assert(kN2tHash == 1 << 10, "the slot hash takes 10 bits");
+    __shared__ int32_t key[kN2tHash];
+    __shared__ uint32_t num[kN2tHash];
+    for (int k = threadIdx.x; k < kN2tHash; k += kBS) {
+        key[k] = -1;
+        num[k] = 0u;
+    }
+    __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (t >= T) return;
-    const int4 q = tets[t];
-    atomicAdd(&cnt[q.x], 1u);
-    atomicAdd(&cnt[q.y], 1u);
-    atomicAdd(&cnt[q.z], 1u);
-    atomicAdd(&cnt[q.w], 1u);
+    if (t < T) {
+        const int4 q = tets[t];
+        const int32_t v[4] = {q.x, q.y, q.z, q.w};
+        for (int j = 0; j < 4; j++) {
+            const int sl = n2t_slot(key, v[j]);
+            if (sl >= 0) atomicAdd(&num[sl], 1u);
+            else atomicAdd(&cnt[v[j]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kN2tHash; k += kBS)
+        if (key[k] >= 0) atomicAdd(&cnt[key[k]], num[k]);
 }
 
 __global__ __launch_bounds__(kBS) void k_n2t_fill(const int4* tets, int64_t T, const uint64_t* off,
                                                   uint32_t* cursor, int32_t* ids)
 {
+    __shared__ int32_t key[kN2tHash];
+    __shared__ uint32_t num[kN2tHash];
+    __shared__ uint32_t base[kN2tHash];
+    for (int k = threadIdx.x; k < kN2tHash; k += kBS) {
+        key[k] = -1;
+        num[k] = 0u;
+    }
+    __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
-    if (t >= T) return;
-    const int4 q = tets[t];
-    const int v[4] = {q.x, q.y, q.z, q.w};
-    for (int k = 0; k < 4; k++) ids[off[v[k]] + atomicAdd(&cursor[v[k]], 1u)] = (int32_t)t;
+    int32_t v[4] = {-1, -1, -1, -1};
+    int sl[4] = {-1, -1, -1, -1};
+    if (t < T) {
+        const int4 q = tets[t];
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        for (int j = 0; j < 4; j++) {
+            sl[j] = n2t_slot(key, v[j]);
+            if (sl[j] >= 0) atomicAdd(&num[sl[j]], 1u);
+        }
+    }
+    __syncthreads();
+    // one reservation per distinct node of the workgroup
+    for (int k = threadIdx.x; k < kN2tHash; k += kBS) {
+        if (key[k] >= 0) base[k] = atomicAdd(&cursor[key[k]], num[k]);
+        num[k] = 0u;                                    // now the slot's local cursor
+    }
+    __syncthreads();
+    if (t < T) {
+        for (int j = 0; j < 4; j++) {
+            const uint32_t r = sl[j] >= 0 ? base[sl[j]] + atomicAdd(&num[sl[j]], 1u) : atomicAdd(&cursor[v[j]], 1u);
+            ids[off[v[j]] + r] = (int32_t)t;
+        }
+    }
 }
 
 // Each node's list sorted to increasing tet id: lists of up to kN2tReg ids in
