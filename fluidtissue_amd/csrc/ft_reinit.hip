@@ -82,7 +82,8 @@ inline unsigned grid1d(int64_t n, int bs = kBS) { return (unsigned)((n + bs - 1)
 //   S_EDGES  crossing points (low 32 bits)
 //   S_ERR    ~0, or (code << 40) | first failing tet
 //   S_BINS   (leaves << 32) | primitives
-//   S_DEEP   query tiles scheduled first (low 32 bits)
+//   S_DEEP   query tiles scheduled first (low 32 bits), of which the very deep
+//            ones, split over several waves (high 32 bits)
 //   S_OVF    traversal stack/budget overflow of the query
 //   S_CAP    a stage total exceeded the capacity its buffers were sized for
 //   S_Q      the query's work-queue counters (9 x u32: deep tiles, then one
@@ -177,6 +178,9 @@ struct LoadPopc {  // u64 bitmap word -> popcount
 };
 struct LoadNonEmpty {  // bin count -> (non-empty << 32) | count
     __device__ uint64_t operator()(uint32_t c) const { return ((uint64_t)(c > 0) << 32) | c; }
+};
+struct LoadTileClass {  // tile class 0 / 1 deep / 2 very deep -> (very deep << 32) | deep
+    __device__ uint64_t operator()(uint32_t c) const { return ((uint64_t)(c == 2) << 32) | (uint64_t)(c >= 1); }
 };
 
 template <class TI, class TO, class Ld>
@@ -1265,6 +1269,7 @@ struct QueryArgs {
     const uint64_t* sc;              // stage totals and halt flags
     unsigned xcd_chunk;              // tiles per XCD chunk (0: dispatch order)
     uint32_t* queue;                 // work-queue schedule: deep counter, then one per XCD (null: static)
+    ulonglong2* split_out;           // best and second key per lane of each part of a split tile (null: none split)
 };
 
 // The dispatcher hands workgroup b to XCD b % 8, each XCD with its own L2.
@@ -1363,6 +1368,7 @@ struct Searcher {
 
     uint32_t n_leaves = 0, n_prims = 0, n_pairs = 0;   // statistics (FT_QUERY_STATS)
     uint64_t skey;   // mode 1: second best squared distance and its triangle
+    int parts = 1, part = 0;   // a split tile's part: only leaves l with l % parts == part are tested
 
     // One triangle for all lanes: a wave-uniform early out when the bounding
     // rectangle of its frame rules it out for every lane, else the exact FP32
@@ -1377,6 +1383,7 @@ struct Searcher {
     // Leaf test at a wave-uniform leaf: records read through scalar loads.
     __device__ void test_leaf_uniform(int l, uint64_t admit = ~0ull)
     {
+        if (parts > 1 && l % parts != part) return;
         const uint32_t b = ftd::cload(leaf_start, l), e = b + ftd::cload(leaf_cnt, l);
         n_leaves++;
         n_prims += e - b;
@@ -1501,33 +1508,58 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
 // about the same there, 10 h is best on the 128^3 field, 3 h and 20 h lose.
 constexpr float kDeepTileH = 10.0f;
 
-// One wave per tile: deep[t] = 1 when some node of tile t is deep.
+// The deepest tiles still set the kernel's end when they start first: at
+// 256^3 the tiles at the centres of the largest spheres (phi_old near -30 h)
+// test some 25k triangles each, and one wave's chain of dependent record
+// loads took 9.3-9.6 ms of a 9.6 ms kernel while the other tiles had ended by
+// about 9.0 ms (per-tile timestamps, a probe build).  A tile is very deep when
+// one of its nodes has phi_old <= -kSplitTileH h: it is searched by
+// kSplitParts waves, each with all 64 nodes but testing only the leaves l with
+// l % kSplitParts equal to its part, and k_split_merge combines the parts.
+// At most kSplitCap tiles per launch are split (the parts' results take
+// kSplitParts x 1 KiB per tile); the rest run whole.  The split only changes
+// which wave tests a leaf: each node's result is the same (k_split_merge).
+constexpr float kSplitTileH = 20.0f;
+constexpr int kSplitParts = 4;
+constexpr int kSplitCap = 1024;
+
+__host__ __device__ inline int64_t split_tiles_cap(int64_t n_tiles)
+{
+    return n_tiles < kSplitCap ? n_tiles : kSplitCap;
+}
+
+// One wave per tile: cls[t] = 2 when some node of tile t is very deep, 1 when
+// deep, else 0.
 __global__ __launch_bounds__(kBS) void k_tile_deep(int64_t n_tiles, const int32_t* perm, const double* phi,
                                                    const int32_t* tile_field, const Grid* grids,
-                                                   uint32_t* deep, const uint64_t* sc)
+                                                   uint32_t* cls, const uint64_t* sc)
 {
     if (halted(sc)) return;
     const int64_t t = ((int64_t)blockIdx.x * kBS + threadIdx.x) >> 6;
     if (t >= n_tiles) return;                          // whole wave
     const int32_t node = perm[t * 64 + ftd::lane_id()];
     const float h = grids[tile_field[t]].h;
-    const bool d = node >= 0 && phi[node] <= -(double)(kDeepTileH * h);
-    const uint64_t m = __ballot(d);
-    if (ftd::lane_id() == 0) deep[t] = m != 0ull;
+    const double p = node >= 0 ? phi[node] : 0.0;
+    const uint64_t m1 = __ballot(p <= -(double)(kDeepTileH * h));
+    const uint64_t m2 = __ballot(p <= -(double)(kSplitTileH * h));
+    if (ftd::lane_id() == 0) cls[t] = m2 ? 2u : (m1 ? 1u : 0u);
 }
 
-// Stable partition from the exclusive scan ex of the deep flags (K deep in
-// all): deep tiles to list[0, K), the others after them, each in tile order.
-__global__ __launch_bounds__(kBS) void k_tile_list(int64_t n_tiles, const uint32_t* ex, const uint64_t* sc,
-                                                   int32_t* list)
+// Stable partition from the exclusive scan ex of LoadTileClass(cls) (K deep
+// tiles, K2 of them very deep): the very deep tiles to list[0, K2), the other
+// deep ones to list[K2, K), the rest after them, each in tile order.
+__global__ __launch_bounds__(kBS) void k_tile_list(int64_t n_tiles, const uint32_t* cls, const uint64_t* ex,
+                                                   const uint64_t* sc, int32_t* list)
 {
     if (halted(sc)) return;
     const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (t >= n_tiles) return;
-    const uint32_t K = (uint32_t)sc[S_DEEP];
-    const uint32_t e = ex[t];
-    const uint32_t next = t + 1 < n_tiles ? ex[t + 1] : K;
-    list[next != e ? (int64_t)e : (int64_t)K + (t - e)] = (int32_t)t;
+    const uint64_t tot = sc[S_DEEP];
+    const int64_t K = (int64_t)(uint32_t)tot, K2 = (int64_t)(tot >> 32);
+    const int64_t e1 = (int64_t)(uint32_t)ex[t], e2 = (int64_t)(ex[t] >> 32);
+    const uint32_t c = cls[t];
+    const int64_t at = c == 2 ? e2 : (c == 1 ? K2 + (e1 - e2) : K + (t - e1));
+    list[at] = (int32_t)t;
 }
 
 // Wave-coherent search: the 64 lanes of a wave hold 64 consecutive slots of
@@ -1536,8 +1568,10 @@ __global__ __launch_bounds__(kBS) void k_tile_list(int64_t n_tiles, const uint32
 // keeps its own best.  All control flow is wave-uniform (ballots), node and
 // primitive records are read at wave-uniform addresses through scalar loads,
 // the stack is one per wave in LDS.
+// split >= 0: part split % kSplitParts of split tile split / kSplitParts,
+// whose best and second keys go to a.split_out instead of the epilogue.
 template <int MODE>
-__device__ __forceinline__ void search_tile(const QueryArgs& a, int64_t base, int32_t* stack)
+__device__ __forceinline__ void search_tile(const QueryArgs& a, int64_t base, int32_t* stack, int split = -1)
 {
     const int64_t slot = base + (threadIdx.x & 63);
     const int32_t node = a.perm[slot];                 // -1: padding lane
@@ -1551,6 +1585,10 @@ __device__ __forceinline__ void search_tile(const QueryArgs& a, int64_t base, in
     S.prec = a.prec; S.trec = a.trec; S.pidx = a.pidx;
     S.leaf_start = a.leaf_start; S.leaf_cnt = a.leaf_cnt;
     S.ifp = a.ifp; S.tris = a.tris;
+    if (split >= 0) {
+        S.parts = kSplitParts;
+        S.part = __builtin_amdgcn_readfirstlane(split % kSplitParts);
+    }
     uint32_t n_nodes = 0;
     if (root != INT32_MIN && root < 0) {
         S.test_leaf_uniform(~root);
@@ -1608,6 +1646,10 @@ __device__ __forceinline__ void search_tile(const QueryArgs& a, int64_t base, in
             cur = __builtin_amdgcn_readfirstlane(next);
         }
         if (budget <= 0) atomicOr(a.overflow, 2u);
+    }
+    if (split >= 0) {                                  // full domain: nothing is clamped
+        a.split_out[(int64_t)split * 64 + (threadIdx.x & 63)] = make_ulonglong2(S.bkey, S.skey);
+        return;
     }
     const bool clamped = node >= 0 && a.band > 0.0f && S.bj() < 0;
     const uint64_t cm = __ballot(clamped);
@@ -1668,17 +1710,29 @@ __global__ __launch_bounds__(BS, WPS) void k_query_wave(QueryArgs a)
     }
     int64_t base;
     if (a.tile_list) {
-        // deep tiles on the first Kb workgroups (Kb a multiple of the 8 XCDs),
-        // the rest on the workgroups after them in XCD-chunked order
+        // deep tiles on the first Kb workgroups (Kb a multiple of the 8 XCDs):
+        // first kSplitParts waves for each split tile, then one per other deep
+        // tile; the rest on the workgroups after them in XCD-chunked order
         constexpr unsigned wpb = BS / 64;
         const int64_t n_tiles = a.n_slots / 64;
-        const unsigned K = (unsigned)ftd::cload(a.n_deep, 0);
-        const unsigned Kb = ((K + wpb - 1) / wpb + 7u) & ~7u;
+        const uint64_t nd = ftd::cload(a.n_deep, 0);
+        const unsigned K = (unsigned)nd;
+        // (explicit compares: min() of two 64-bit integers resolves to a
+        // floating-point overload here and rounds them)
+        const uint64_t cap = (uint64_t)split_tiles_cap(n_tiles);
+        const unsigned K2 = a.split_out ? (unsigned)((nd >> 32) < cap ? (nd >> 32) : cap) : 0u;
+        const unsigned D = K2 * kSplitParts + (K - K2);          // deep waves
+        const unsigned Kb = ((D + wpb - 1) / wpb + 7u) & ~7u;
         const unsigned b = blockIdx.x;
         int64_t r;
         if (b < Kb) {
-            r = (int64_t)b * wpb + w;
-            if (r >= K) return;                        // whole wave: padding of the deep workgroups
+            const unsigned rr = b * wpb + w;
+            if (rr >= D) return;                       // whole wave: padding of the deep workgroups
+            if (rr < K2 * kSplitParts) {
+                search_tile<MODE>(a, (int64_t)a.tile_list[rr / kSplitParts] * 64, wstack[w], (int)rr);
+                return;
+            }
+            r = (int64_t)K2 + (rr - K2 * kSplitParts);
         } else {
             const int64_t nn = n_tiles - K;
             const unsigned nbn = (unsigned)((nn + wpb - 1) / wpb);
@@ -1693,6 +1747,43 @@ __global__ __launch_bounds__(BS, WPS) void k_query_wave(QueryArgs a)
         if (base >= a.n_slots) return;                // whole wave out of range
     }
     search_tile<MODE>(a, base, wstack[w]);
+}
+
+// The parts of each split tile combined, one wave per tile: per lane the
+// least key and the least key other than it over the parts' best and second
+// keys (a part's two differ, and a triangle is tested by one part only, the
+// one owning its leaf), then the node's epilogue.  Every triangle that can be
+// the best or a second within the FP64 settle's margin is tested by its part:
+// a part prunes with its own best, which is no better than the tile's.
+template <int MODE>
+__global__ __launch_bounds__(kBS) void k_split_merge(QueryArgs a)
+{
+    if (halted(a.sc)) return;
+    const int64_t n_tiles = a.n_slots / 64;
+    const int64_t k = ((int64_t)blockIdx.x * kBS + threadIdx.x) >> 6;
+    const int64_t cap = split_tiles_cap(n_tiles), nd2 = (int64_t)(ftd::cload(a.n_deep, 0) >> 32);
+    const int64_t K2 = nd2 < cap ? nd2 : cap;
+    if (k >= K2) return;                               // whole wave
+    const int lane = threadIdx.x & 63;
+    const int64_t slot = (int64_t)a.tile_list[k] * 64 + lane;
+    const int32_t node = a.perm[slot];
+    const ulonglong2 v = a.split_out[k * kSplitParts * 64 + lane];
+    uint64_t best = v.x, second = v.y;
+    for (int p = 1; p < kSplitParts; p++) {
+        const ulonglong2 u = a.split_out[(k * kSplitParts + p) * 64 + lane];
+        // explicit compares: min() of two 64-bit keys would round them
+        const uint64_t nb = u.x < best ? u.x : best;
+        const uint64_t k1 = best != nb ? best : second, k2 = u.x != nb ? u.x : u.y;
+        second = k2 < k1 ? k2 : k1;
+        best = nb;
+    }
+    if (node < 0) return;
+    Searcher<MODE> S;
+    S.x = a.sx[slot]; S.y = a.sy[slot]; S.z = a.sz[slot];
+    S.bkey = best;
+    S.skey = second;
+    S.ifp = a.ifp; S.tris = a.tris;
+    finish_node<MODE>(a, S, node);
 }
 
 // ============================================================================
@@ -2859,7 +2950,8 @@ struct ft_ctx {
     bool has_conn = false;
     float h_est = 0.0f;
     DevBuf xyz, tets, edges, t2e, perm, sx, sy, sz, tile_field;
-    DevBuf tile_sched;               // per query tile: deep flag / scan, then the launch order
+    DevBuf tile_sched;               // per query tile: class, launch order, class scan
+    DevBuf deep_part;                // the parts' keys of the split tiles
     DevBuf gather_tab;               // piece tables of the batch assembly (ft_set_batch_meshes)
     // A batch of resident meshes reads their tets, t2e and edges in place
     // (virt): field_tab holds the per-field pointers, field_meshes keeps the
@@ -2939,7 +3031,7 @@ std::vector<DevBuf*> all_buffers(ft_ctx* h)
             &h->p2t_cursor, &h->p2t, &h->ifp_area, &h->nearest, &h->bin_count,
             &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent,
-            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scan_side, &h->scalars,
+            &h->bnodes, &h->bflags, &h->stats, &h->scan_bsum, &h->scan_side, &h->deep_part, &h->scalars,
             &h->n2t_off, &h->n2t_ids, &h->grad_t, &h->normals, &h->phi_ifn_int,
             &h->gsplit, &h->kmesh, &h->k_if, &h->st_cnt, &h->st_off, &h->st_ids, &h->st_f,
             &h->fem_len, &h->fem_w, &h->fem_soff, &h->fem_cols, &h->fem_m, &h->fem_l, &h->fem_flag, &h->fem_map,
@@ -3238,8 +3330,74 @@ void boxes_from_host(ft_ctx* c, const float* xyz)
     c->h_est = c->nf ? c->fh[0].h : 0.0f;
 }
 
+// Grid of the node sort.  A query wave takes 64 consecutive slots, so the
+// tiles are compact only if the bins follow the nodes.  When a field's node
+// count and box fit a structured grid -- n = nx ny nz nodes with nx - 1,
+// ny - 1, nz - 1 spacings s along the box's edges, as a box mesher's (or
+// the Kuhn generator's) grid, jittered by less than s / 2 -- the bins are
+// kf s wide and start s / 2 before the box: every bin then holds the
+// kf^3 nodes of one block, and 64 consecutive slots in Morton order are an
+// aligned 4 x 4 x 4 block of nodes (kf = 1 or 2).  Bins of the estimated
+// spacing from the box corner cut those blocks where jittered nodes cross a
+// bin edge; at 256^3 aligned blocks made the query 4.6% faster.  Any other
+// field keeps the grid of about one node per bin.  The order only affects
+// speed: each node's result is the same in any slot.
+Grid node_sort_grid(const FieldHost& f, int64_t n, int max_lg, int* lg_out)
+{
+    const Grid fallback = make_grid(f, f.h, max_lg, lg_out);
+    double ext[3], ext_max = 0.0;
+    for (int k = 0; k < 3; k++) {
+        ext[k] = (double)f.hi[k] - f.lo[k];
+        if (!(ext[k] >= 0.0)) return fallback;
+        ext_max = std::max(ext_max, ext[k]);
+    }
+    if (n < 8 || !(ext_max > 0.0)) return fallback;
+    // prod_k (ext_k / s + 1) decreases in s: bisection for the s that gives n
+    auto count = [&](double s) {
+        double p = 1.0;
+        for (int k = 0; k < 3; k++) p *= ext[k] / s + 1.0;
+        return p;
+    };
+    double lo_s = ext_max / (double)n, hi_s = ext_max;   // count(lo_s) > n >= 8 >= count(hi_s)
+    if (!(count(hi_s) <= (double)n)) return fallback;
+    for (int it = 0; it < 200; it++) {
+        const double mid = 0.5 * (lo_s + hi_s);
+        (count(mid) > (double)n ? lo_s : hi_s) = mid;
+    }
+    const double s0 = 0.5 * (lo_s + hi_s);
+    int64_t nk[3], prod = 1, cells = 1;
+    double s_sum = 0.0;
+    int s_cnt = 0;
+    for (int k = 0; k < 3; k++) {
+        const double q = ext[k] / s0 + 1.0;
+        nk[k] = std::llround(q);
+        if (nk[k] < 1 || std::fabs(q - (double)nk[k]) > 0.05) return fallback;
+        prod *= nk[k];
+        cells = std::max(cells, nk[k]);
+        if (nk[k] > 1) { s_sum += ext[k] / (double)(nk[k] - 1); s_cnt++; }
+    }
+    if (prod != n || s_cnt == 0) return fallback;
+    const double s = s_sum / s_cnt;
+    for (int k = 0; k < 3; k++)
+        if (nk[k] > 1 && std::fabs(ext[k] / (double)(nk[k] - 1) - s) > 1e-4 * s) return fallback;
+    const int64_t kf = (cells + (1ll << max_lg) - 1) >> max_lg;   // spacings per bin
+    if (kf > 2) return fallback;                                   // blocks would not be 4 x 4 x 4
+    Grid g{};
+    int lg = 0;
+    for (int k = 0; k < 3; k++) {
+        g.lo[k] = (float)((double)f.lo[k] - 0.5 * s);
+        // node i of the axis lands in bin floor((i + 1/2) / kf)
+        g.dim[k] = (int)(((nk[k] - 1) * 2 + 1) / (2 * kf)) + 1;
+        while ((1 << lg) < g.dim[k]) lg++;
+    }
+    g.inv_b = (float)(1.0 / ((double)kf * s));
+    g.h = f.h;
+    *lg_out = lg;
+    return g;
+}
+
 // Morton order of each field's nodes into the query slots (device xyz):
-// one counting sort per field on a grid of about one node per bin.
+// one counting sort per field on node_sort_grid.
 int sort_nodes(ft_ctx* c)
 {
     const int nf = c->nf;
@@ -3248,14 +3406,14 @@ int sort_nodes(ft_ctx* c)
     FT_TRY(ensure(c->sy, sizeof(float) * std::max<int64_t>(1, c->n_slots)));
     FT_TRY(ensure(c->sz, sizeof(float) * std::max<int64_t>(1, c->n_slots)));
     FT_TRY(ensure(c->tile_field, sizeof(int32_t) * std::max<int64_t>(1, c->n_slots / 64)));
-    FT_TRY(ensure(c->tile_sched, 2 * sizeof(int32_t) * std::max<int64_t>(1, c->n_slots / 64)));
+    FT_TRY(ensure(c->tile_sched, 4 * sizeof(int32_t) * std::max<int64_t>(1, c->n_slots / 64)));
     if (c->N == 0) return FT_OK;
     hipStream_t s = c->stream;
     for (int k = 0; k < nf; k++) {
         const int64_t n = c->node_off[k + 1] - c->node_off[k];
         if (n == 0) continue;
         int lg = 0;
-        const Grid g = make_grid(c->fh[k], c->fh[k].h, kMaxBinsLog2PerAxis, &lg);
+        const Grid g = node_sort_grid(c->fh[k], n, kMaxBinsLog2PerAxis, &lg);
         const int64_t G = (int64_t)1 << (3 * lg);
         FT_TRY(ensure(c->bin_count, sizeof(uint32_t) * G));
         FT_TRY(ensure(c->bin_prefix, sizeof(uint64_t) * G));
@@ -3404,6 +3562,7 @@ int ensure_stage(ft_ctx* c, int stage, const Caps& cp, int mode, bool want_neare
         const int64_t nb = std::max({scan_blocks(nblk), scan_blocks(nw_edges), scan_blocks(cp.P),
                                      scan_blocks(G)});
         FT_TRY(ensure(c->scan_bsum, sizeof(uint64_t) * nb));
+        FT_TRY(ensure(c->deep_part, sizeof(ulonglong2) * 64 * kSplitParts * at_least1(split_tiles_cap(c->n_slots / 64))));
     }
     return FT_OK;
 }
@@ -3744,15 +3903,16 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
     if (sp.band <= 0.0f && n_tiles > 0) {
         FT_TRY(side_fork(c, s));
         hipStream_t ss = c->side;
-        uint32_t* deep = c->tile_sched.as<uint32_t>();
-        tile_list = reinterpret_cast<int32_t*>(deep + n_tiles);
-        FT_TRY(fill<uint64_t>(ss, sc + S_DEEP, 1, 0ull));   // the scan writes the low 32 bits
+        // tile_sched: classes (u32), the launch order (i32), the scan (u64)
+        uint32_t* cls = c->tile_sched.as<uint32_t>();
+        tile_list = reinterpret_cast<int32_t*>(cls + n_tiles);
+        uint64_t* ex = reinterpret_cast<uint64_t*>(cls + 2 * n_tiles);
         hipLaunchKernelGGL(k_tile_deep, dim3(grid1d(64 * n_tiles)), dim3(kBS), 0, ss, n_tiles,
                            c->perm.as<int32_t>(), sp.phi_old, c->tile_field.as<int32_t>(),
-                           c->d_grids.as<Grid>(), deep, sc);
-        FT_TRY(scan<uint32_t, uint32_t>(c, ss, deep, deep, n_tiles, reinterpret_cast<uint32_t*>(sc + S_DEEP),
-                                        LoadSame{}, &c->scan_side));
-        hipLaunchKernelGGL(k_tile_list, dim3(grid1d(n_tiles)), dim3(kBS), 0, ss, n_tiles, deep, sc, tile_list);
+                           c->d_grids.as<Grid>(), cls, sc);
+        FT_TRY(scan<uint32_t, uint64_t>(c, ss, cls, ex, n_tiles, sc + S_DEEP, LoadTileClass{}, &c->scan_side));
+        hipLaunchKernelGGL(k_tile_list, dim3(grid1d(n_tiles)), dim3(kBS), 0, ss, n_tiles, cls, ex, sc,
+                           tile_list);
     }
     const int64_t G = c->idx_G;
     const int64_t L_cap = std::min(G, n_cap);
@@ -3850,8 +4010,12 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         q.xcd_chunk = qb == 64 ? kXcdChunk64 : kXcdChunk;
         const int wps = mode == 0 ? 8 : sp.waves_per_simd;
         // with a tile list: up to 7 padding workgroups after the deep ones,
-        // one more for the split of the remainder
-        dim3 g(grid1d(c->n_slots, qb) + (q.tile_list ? 8u : 0u)), b(qb);
+        // one more for the split of the remainder, and the extra waves of
+        // the split tiles (static schedule)
+        if (q.tile_list && !sp.work_queue) q.split_out = c->deep_part.as<ulonglong2>();
+        const int64_t split_waves = q.split_out ? split_tiles_cap(n_tiles) * (kSplitParts - 1) : 0;
+        dim3 g(grid1d(c->n_slots, qb) + (q.tile_list ? 8u : 0u) + (unsigned)((split_waves + qb / 64 - 1) / (qb / 64))),
+            b(qb);
         if (sp.work_queue) {
             // persistent: as many waves as the device holds at this launch
             // bound (4 SIMDs per CU), whole workgroups, at least one per XCD
@@ -3861,6 +4025,12 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
             FT_TRY(fill<uint32_t>(s, q.queue, 9, 0u));
         }
         FT_TRY(launch_query(mode, qb, wps, g, b, s, q));
+        if (q.split_out) {
+            const dim3 gm(grid1d(64 * split_tiles_cap(n_tiles)));
+            if (mode == 0) hipLaunchKernelGGL(k_split_merge<0>, gm, dim3(kBS), 0, s, q);
+            else hipLaunchKernelGGL(k_split_merge<1>, gm, dim3(kBS), 0, s, q);
+            FT_TRY(check_launch());
+        }
     }
     if (timed) (void)hipEventRecord(c->ev[3], s);
     if (sp.want_normals && c->N > 0) {
@@ -4211,7 +4381,7 @@ std::vector<DevBuf*> call_buffers(ft_ctx* h)
             &h->bin_count, &h->bin_prefix, &h->pcode, &h->prec, &h->trec, &h->pidx, &h->leaf_key,
             &h->leaf_start, &h->leaf_cnt, &h->leaf_b, &h->leaf_parent, &h->bnodes, &h->bflags,
             &h->stats, &h->d_grids, &h->d_cnt_off, &h->leaf_off, &h->roots, &h->scan_bsum,
-            &h->scan_side};
+            &h->scan_side, &h->deep_part};
 }
 
 // reinit_device with every per-call buffer taken from the device's scratch:
@@ -4678,7 +4848,7 @@ int ft_set_batch_meshes(ft_handle h, int32_t n_fields, const int32_t* mesh_ids)
     FT_TRY(ensure(h->sy, 4 * std::max<int64_t>(1, h->n_slots)));
     FT_TRY(ensure(h->sz, 4 * std::max<int64_t>(1, h->n_slots)));
     FT_TRY(ensure(h->tile_field, 4 * std::max<int64_t>(1, h->n_slots / 64)));
-    FT_TRY(ensure(h->tile_sched, 8 * std::max<int64_t>(1, h->n_slots / 64)));
+    FT_TRY(ensure(h->tile_sched, 16 * std::max<int64_t>(1, h->n_slots / 64)));
     // per-field connectivity pointers: tets, t2e, edges of each mesh
     std::vector<const void*> tab(3 * (size_t)n_fields);
     for (int k = 0; k < n_fields; k++) {

@@ -384,3 +384,30 @@ def test_launch_bounds_and_work_queue_give_the_same_bits(mode):
     for o in outs[1:]:
         assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
     b.close()
+
+
+@pytest.mark.parametrize("mode", ["triangles", "points"])
+def test_split_deep_tiles_give_the_same_bits(mode):
+    """Tiles deep inside the surface (a node with phi_old <= -20 h) are searched
+    by several waves, each testing the leaves it owns, and merged afterwards.
+    Their nodes get the whole-tile search's results bit for bit, nearest
+    indices included: the work-queue schedule searches every tile whole.  The
+    64^3 star reaches about -50 h, so it has such tiles; the triangle results
+    are also checked against the oracle on its deepest nodes."""
+    m, phi, h = workload("star_64")
+    assert phi.min() <= -25 * h
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    split = r.reset_phi(phi, mode=mode, blend="update", want_nearest=True)
+    near_split = r.nearest()
+    whole = r.reset_phi(phi, mode=mode, blend="update", want_nearest=True, schedule="queue")
+    near_whole = r.nearest()
+    assert np.array_equal(split[0].view(np.uint64), whole[0].view(np.uint64))
+    assert np.array_equal(split[1].view(np.uint64), whole[1].view(np.uint64))
+    assert np.array_equal(near_split, near_whole)
+    if mode == "triangles":
+        deep = np.argsort(phi)[:300]
+        emax, _, _, mism = triangle_errors("star_64", np.abs(split[1][deep]), near_split[deep], deep, h)
+        assert emax <= 1e-5 * h
+        assert mism == 0
+    r.close()
