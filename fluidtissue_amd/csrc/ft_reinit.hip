@@ -1509,23 +1509,30 @@ __device__ __forceinline__ void record_stats(const QueryArgs& a, const Searcher<
 constexpr float kDeepTileH = 10.0f;
 
 // The deepest tiles still set the kernel's end when they start first: at
-// 256^3 the tiles at the centres of the largest spheres (phi_old near -30 h)
+// 256^3 the tiles at the centres of the largest spheres (phi_old near -29 h)
 // test some 25k triangles each, and one wave's chain of dependent record
 // loads took 9.3-9.6 ms of a 9.6 ms kernel while the other tiles had ended by
 // about 9.0 ms (per-tile timestamps, a probe build).  A tile is very deep when
 // one of its nodes has phi_old <= -kSplitTileH h: it is searched by
 // kSplitParts waves, each with all 64 nodes but testing only the leaves l with
 // l % kSplitParts equal to its part, and k_split_merge combines the parts.
-// At most kSplitCap tiles per launch are split (the parts' results take
-// kSplitParts x 1 KiB per tile); the rest run whole.  The split only changes
-// which wave tests a leaf: each node's result is the same (k_split_merge).
-constexpr float kSplitTileH = 20.0f;
+// Each part repeats the walk of the inner tree nodes, so splitting costs work:
+// at 256^3 (query ms, this field and its mirror image across x = z) 2 parts
+// from 25 h gave 9.05 / 8.98, 4 parts 9.07-9.25 / 9.02, 4 parts from 20 h
+// 9.17 / 9.25, 8 parts 9.36, from 30 h the deepest tiles ran whole again
+// (10.9), and the unsplit query took 9.6-11.2 / 8.9-9.4 depending on the
+// order of the tiles.  Four parts keep more margin for fields deeper than
+// this one.  At most split_tiles_cap tiles per launch are split (the parts'
+// results take kSplitParts x 1 KiB per tile); the rest run whole.  The split
+// only changes which wave tests a leaf: each node's result is the same.
+constexpr float kSplitTileH = 25.0f;
 constexpr int kSplitParts = 4;
-constexpr int kSplitCap = 1024;
+constexpr int64_t kSplitCap = 1024;
 
 __host__ __device__ inline int64_t split_tiles_cap(int64_t n_tiles)
 {
-    return n_tiles < kSplitCap ? n_tiles : kSplitCap;
+    const int64_t cap = n_tiles / 64 > kSplitCap ? n_tiles / 64 : kSplitCap;
+    return n_tiles < cap ? n_tiles : cap;
 }
 
 // One wave per tile: cls[t] = 2 when some node of tile t is very deep, 1 when

@@ -388,7 +388,7 @@ def test_launch_bounds_and_work_queue_give_the_same_bits(mode):
 
 @pytest.mark.parametrize("mode", ["triangles", "points"])
 def test_split_deep_tiles_give_the_same_bits(mode):
-    """Tiles deep inside the surface (a node with phi_old <= -20 h) are searched
+    """Tiles deep inside the surface (a node with phi_old <= -25 h) are searched
     by several waves, each testing the leaves it owns, and merged afterwards.
     Their nodes get the whole-tile search's results bit for bit, nearest
     indices included: the work-queue schedule searches every tile whole.  The
