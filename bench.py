@@ -802,7 +802,7 @@ def main():
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--query-block", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
-    ap.add_argument("--schedule", default="static", choices=["static", "queue"])
+    ap.add_argument("--schedule", default="static", choices=["static", "queue", "bins"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")

@@ -27,6 +27,7 @@ FT_BLEND_INIT = 0
 FT_BLEND_UPDATE = 1
 FT_SCHED_STATIC = 0
 FT_SCHED_QUEUE = 1
+FT_SCHED_BINS = 2
 
 FT_PHI_SPHERE_WAVE = 1
 FT_PHI_STAR = 2

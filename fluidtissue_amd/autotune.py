@@ -10,7 +10,8 @@ Tunes, on the caller's own resident batch and level set:
                   against occupancy; triangles mode only);
   schedule        one tile per wave in a fixed deep-first order ("static"),
                   or persistent waves taking tiles from per-XCD counters
-                  ("queue").
+                  ("queue"); in a narrow band in triangles mode also the
+                  bins staged in LDS instead of the tree walk ("bins").
 Each candidate runs the full reinit; the fastest median wins.  The search
 is staged: bin_scale x query_block at 8 waves, static; then waves_per_simd
 at the best of those; then the schedule.  Results are cached per (mode,
@@ -72,6 +73,8 @@ def tune(handle, phi_ptr, phi_backup_ptr, nbytes, mode="triangles", reps=2, sync
 
     w0 = 8 if 8 in waves else waves[-1]
     s0 = schedules[0]
+    if band > 0.0 and mode == "triangles" and "bins" not in schedules:
+        schedules = tuple(schedules) + ("bins",)
     best = min(((run(bs, qb, w0, s0), bs, qb) for bs in bin_scales for qb in query_blocks))
     _, bs, qb = best
     best_w = min(((run(bs, qb, w, s0), w) for w in waves))[1]

@@ -88,8 +88,10 @@ inline unsigned grid1d(int64_t n, int bs = kBS) { return (unsigned)((n + bs - 1)
 //   S_CAP    a stage total exceeded the capacity its buffers were sized for
 //   S_Q      the query's work-queue counters (9 x u32: deep tiles, then one
 //            per XCD), zeroed per launch
+//   S_MARGIN the bins schedule: largest distance from a triangle's centroid to
+//            its vertices (float bits, low 32)
 enum Slot { S_BLK = 0, S_EDGES = 1, S_ERR = 2, S_P2T = 3, S_BINS = 4, S_DEEP = 5, S_OVF = 6, S_CAP = 7,
-            S_Q = 8, S_N = 13 };
+            S_Q = 8, S_MARGIN = 13, S_N = 14 };
 
 // Stage totals of the running reinit as the kernels see them.  Kernels after
 // a scan read the total from the device, so a launch sized for a capacity
@@ -1791,6 +1793,210 @@ __global__ __launch_bounds__(kBS) void k_split_merge(QueryArgs a)
     S.skey = second;
     S.ifp = a.ifp; S.tris = a.tris;
     finish_node<MODE>(a, S, node);
+}
+
+// ----------------------------------------------------------------------------
+// Narrow band without the tree (schedule FT_SCHED_BINS, triangles): each wave
+// takes its tile's box, widened by the band and by the largest distance from
+// a triangle's centroid to its vertices (the primitives are binned by
+// centroid), and walks the index bins in that box in their Morton order.  The
+// triangles of each non-empty bin are staged in LDS, 64 at a time, by the
+// wave's lanes, and every lane then tests each staged triangle (the frame
+// rectangle cull against its own bound, then the exact distance).  Each lane
+// keeps its own best two, as the tree walk does, so no cross-lane reduction
+// is needed.  Every triangle within the band of a lane has its centroid in
+// the widened box, and the lane's first bound is the band: the results are
+// the tree walk's, bit for bit.
+// ----------------------------------------------------------------------------
+struct BinQuery {
+    int64_t G;                       // bins over all fields (cnt_off[f] + code)
+    const uint64_t* prefix;          // (bins with triangles before << 32) | triangles before
+                                     // (the index's scan; the scatter consumed the counts)
+    const Grid* grids;               // per field
+    const int64_t* cnt_off;          // per field: its first bin
+};
+
+// Largest distance from a triangle's centroid (as k_prim_bin computes it) to
+// its vertices, as float bits (non-negative: integer max orders them).
+__global__ __launch_bounds__(kBS) void k_tri_margin(const uint64_t* sc, const float* ifp, const int4* tris,
+                                                    uint32_t* out)
+{
+    if (halted(sc)) return;
+    const int64_t j = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    float r = 0.0f;
+    if (j < cnt_tris(sc)) {
+        float a[3], b[3], d[3], c[3];
+        tri_vertices(tris, ifp, (int32_t)j, a, b, d);
+        for (int k = 0; k < 3; k++) c[k] = (a[k] + b[k] + d[k]) * (1.0f / 3.0f);
+        float m = 0.0f;
+        const float* v[3] = {a, b, d};
+        for (int q = 0; q < 3; q++) {
+            const float dx = v[q][0] - c[0], dy = v[q][1] - c[1], dz = v[q][2] - c[2];
+            m = fmaxf(m, sqrtf(dx * dx + dy * dy + dz * dz));
+        }
+        r = m;
+    }
+    for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o));
+    if (ftd::lane_id() == 0 && r > 0.0f) atomicMax(out, __float_as_uint(r));
+}
+
+// Orders the wave's LDS stores before its loads (a wave runs its lanes
+// together: a wave-scope fence waits for its own LDS accesses).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kBinStage = 64;        // triangles staged per pass (one per lane)
+
+template <int BS>
+__global__ __launch_bounds__(BS, 8) void k_query_bins(QueryArgs a, BinQuery bq)
+{
+    __shared__ TriRec srec[BS / 64][kBinStage];
+    __shared__ int32_t sid[BS / 64][kBinStage];
+    if (halted(a.sc)) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t base = (int64_t)xcd_block(blockIdx.x, gridDim.x, a.xcd_chunk) * BS + (threadIdx.x & ~63);
+    if (base >= a.n_slots) return;                    // whole wave out of range
+    const int64_t slot = base + lane;
+    const int32_t node = a.perm[slot];
+    const float px = a.sx[slot], py = a.sy[slot], pz = a.sz[slot];
+    const int field = __builtin_amdgcn_readfirstlane(a.tile_field[base >> 6]);
+    Searcher<1> S;
+    S.x = px; S.y = py; S.z = pz;
+    S.init_best(a.band * a.band);
+    S.ifp = a.ifp; S.tris = a.tris;
+    // the tile's box over its real nodes, widened
+    float lo[3] = {node >= 0 ? px : INFINITY, node >= 0 ? py : INFINITY, node >= 0 ? pz : INFINITY};
+    float hi[3] = {node >= 0 ? px : -INFINITY, node >= 0 ? py : -INFINITY, node >= 0 ? pz : -INFINITY};
+    for (int o = 32; o >= 1; o >>= 1)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+        }
+    const Grid g = bq.grids[field];
+    const float wide = a.band + __uint_as_float((uint32_t)ftd::cload(a.sc, S_MARGIN));
+    int b0[3], n[3];
+    for (int k = 0; k < 3; k++) {
+        // one more bin each side: the centroid's bin and this range round alike
+        // only up to the last ulp
+        // (clamped to the grid before the conversion: a wide band overflows int)
+        const float dmax = (float)g.dim[k];
+        const float f0 = fminf((lo[k] - wide - g.lo[k]) * g.inv_b, dmax);
+        const float f1 = fminf((hi[k] + wide - g.lo[k]) * g.inv_b, dmax);
+        const int i0 = f0 >= 1.0f ? (int)f0 - 1 : 0;
+        int i1 = f1 >= 0.0f ? (int)f1 + 1 : 0;
+        i1 = i1 < g.dim[k] - 1 ? i1 : g.dim[k] - 1;
+        b0[k] = __builtin_amdgcn_readfirstlane(i0 < g.dim[k] ? i0 : g.dim[k] - 1);
+        n[k] = __builtin_amdgcn_readfirstlane(i1 >= b0[k] ? i1 - b0[k] + 1 : 0);
+    }
+    // The widened range is walked in blocks of 4 x 4 x 4 bins, 64 consecutive
+    // Morton codes, whose triangle count is a difference of two prefix entries:
+    // a tile far from the surface finds every block empty in one pass.
+    const uint32_t off = (uint32_t)bq.cnt_off[field], fend = (uint32_t)bq.cnt_off[field + 1];
+    const uint32_t total = (uint32_t)cnt_tris(a.sc);
+    auto before = [&](uint32_t x) -> uint32_t { return (int64_t)x < bq.G ? (uint32_t)bq.prefix[x] : total; };
+    int B0[3], nB[3];
+    for (int k = 0; k < 3; k++) {
+        B0[k] = b0[k] >> 2;
+        nB[k] = n[k] > 0 ? ((b0[k] + n[k] - 1) >> 2) - B0[k] + 1 : 0;
+    }
+    const int64_t nblk = (int64_t)nB[0] * nB[1] * nB[2];
+    // this lane's place in a block (Morton bits: z0 y0 x0 z1 y1 x1)
+    const int lz = (lane & 1) | ((lane >> 2) & 2), ly = ((lane >> 1) & 1) | ((lane >> 3) & 2),
+              lx = ((lane >> 2) & 1) | ((lane >> 4) & 2);
+    for (int64_t q0 = 0; q0 < nblk; q0 += 64) {
+        const int64_t q = q0 + lane;
+        uint32_t first = 0, bc = 0;
+        int bxq = 0, byq = 0, bzq = 0;
+        if (q < nblk) {
+            bzq = B0[2] + (int)(q % nB[2]);
+            byq = B0[1] + (int)((q / nB[2]) % nB[1]);
+            bxq = B0[0] + (int)(q / ((int64_t)nB[2] * nB[1]));
+            first = off + (ftd::morton3((uint32_t)bxq, (uint32_t)byq, (uint32_t)bzq) << 6);
+            const uint32_t last = first + 64 < fend ? first + 64 : fend;
+            bc = first < fend ? before(last) - before(first) : 0u;
+        }
+        // non-empty blocks nearest the tile first (the nodes' bounds then
+        // tighten early and reject more of the later bins): distance from the
+        // tile's box to the block's, only to order them
+        float bd = INFINITY;
+        if (bc > 0) {
+            const float e = 4.0f / g.inv_b;
+            const int bq3[3] = {bxq, byq, bzq};
+            float d2 = 0.0f;
+            for (int k = 0; k < 3; k++) {
+                const float l0 = g.lo[k] + (float)bq3[k] * e;
+                const float gap = fmaxf(fmaxf(l0 - hi[k], lo[k] - (l0 + e)), 0.0f);
+                d2 = __fmaf_rn(gap, gap, d2);
+            }
+            bd = fminf(d2, FLT_MAX);                  // finite for every block in `blocks`
+        }
+        uint64_t blocks = __ballot(bc > 0);
+        // each pass takes out one block (a lane in `blocks` has the least key:
+        // finite, against infinity for the others), so at most 64 passes
+        for (int pass = 0; pass < 64 && blocks; pass++) {
+            // the lane holding the nearest remaining block
+            float mv = bd;
+            int ml = lane;
+            for (int o = 32; o >= 1; o >>= 1) {
+                const float ov = __shfl_xor(mv, o);
+                const int ol = __shfl_xor(ml, o);
+                if (ov < mv || (ov == mv && ol < ml)) { mv = ov; ml = ol; }
+            }
+            const int sb = __builtin_amdgcn_readfirstlane(ml);
+            blocks &= ~(1ull << sb);
+            if (lane == sb) bd = INFINITY;
+            const uint32_t c = (uint32_t)__shfl((int)first, sb) + (uint32_t)lane;
+            const int bx = 4 * __shfl(bxq, sb) + lx, by = 4 * __shfl(byq, sb) + ly, bz = 4 * __shfl(bzq, sb) + lz;
+            uint32_t cnt = 0;
+            if (bx >= b0[0] && bx < b0[0] + n[0] && by >= b0[1] && by < b0[1] + n[1] && bz >= b0[2] &&
+                bz < b0[2] + n[2]) {
+                const uint64_t pc = bq.prefix[c];
+                cnt = before(c + 1) - (uint32_t)pc;
+                if (cnt) {
+                    // the bin's leaf box against the tile's box: beyond the band
+                    // of every node, skipped (the box gap is a lower bound on
+                    // each node's distance to the leaf)
+                    const Bound& lb = a.leaf_b[(uint32_t)(pc >> 32)];
+                    float d2 = 0.0f;
+                    for (int k = 0; k < 3; k++) {
+                        const float gap = fmaxf(fmaxf(lb.lo[k] - hi[k], lo[k] - lb.hi[k]), 0.0f);
+                        d2 = __fmaf_rn(gap, gap, d2);
+                    }
+                    if (d2 > prune_bound(a.band * a.band)) cnt = 0;
+                }
+            }
+            uint64_t has = __ballot(cnt > 0);
+            while (has) {
+                const int src = __ffsll((unsigned long long)has) - 1;
+                has &= has - 1;
+                const uint32_t cc = (uint32_t)__shfl((int)c, src), nt = (uint32_t)__shfl((int)cnt, src);
+                const uint64_t pcc = bq.prefix[cc];
+                const uint32_t start = (uint32_t)pcc;
+                // each node against the bin's leaf box, with its bound so far
+                const Bound& lb = a.leaf_b[(uint32_t)(pcc >> 32)];
+                if (__ballot(ftd::box_dist2(px, py, pz, lb.lo, lb.hi) <= S.bnd) == 0ull) continue;
+                for (uint32_t s0 = 0; s0 < nt; s0 += kBinStage) {
+                    const uint32_t k = nt - s0 < (uint32_t)kBinStage ? nt - s0 : (uint32_t)kBinStage;
+                    if ((uint32_t)lane < k) {
+                        srec[w][lane] = a.trec[start + s0 + lane];
+                        sid[w][lane] = a.pidx[start + s0 + lane];
+                    }
+                    wave_lds_sync();
+                    for (uint32_t t = 0; t < k; t++) S.test_tri(srec[w][t], sid[w][t]);
+                    wave_lds_sync();                   // before the next pass overwrites them
+                }
+            }
+        }
+    }
+    const bool clamped = node >= 0 && S.bj() < 0;
+    const uint64_t cm = __ballot(clamped);
+    if (lane == 0 && cm) atomicAdd(&a.n_clamped[field], (unsigned long long)__popcll(cm));
+    if (node < 0) return;
+    finish_node<1>(a, S, node);
 }
 
 // ============================================================================
@@ -3635,6 +3841,7 @@ struct SearchSpec {
     int query_block = 0;
     int waves_per_simd = 8;          // launch bound of the query (triangles: 5..8)
     bool work_queue = false;         // persistent query over tile counters
+    bool bins = false;               // narrow band: bins staged in LDS instead of the tree walk
 };
 
 // Copies of the resident meshes ms into the handle's concatenated arrays,
@@ -4016,6 +4223,19 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
         const int qb = sp.query_block == 64 || sp.query_block == 128 ? sp.query_block : 256;
         q.xcd_chunk = qb == 64 ? kXcdChunk64 : kXcdChunk;
         const int wps = mode == 0 ? 8 : sp.waves_per_simd;
+        if (sp.bins) {
+            // narrow band from the bins, staged in LDS (no tree walk)
+            FT_TRY(fill<uint64_t>(s, sc + S_MARGIN, 1, 0ull));
+            if (n_cap > 0)
+                hipLaunchKernelGGL(k_tri_margin, dim3(grid1d(n_cap)), dim3(kBS), 0, s, sc, ifp, tris,
+                                   reinterpret_cast<uint32_t*>(sc + S_MARGIN));
+            const BinQuery bq{G, c->bin_prefix.as<uint64_t>(), c->d_grids.as<Grid>(), c->d_cnt_off.as<int64_t>()};
+            const dim3 gb(grid1d(c->n_slots, qb)), bb(qb);
+            if (qb == 64) hipLaunchKernelGGL(k_query_bins<64>, gb, bb, 0, s, q, bq);
+            else if (qb == 128) hipLaunchKernelGGL(k_query_bins<128>, gb, bb, 0, s, q, bq);
+            else hipLaunchKernelGGL(k_query_bins<256>, gb, bb, 0, s, q, bq);
+            FT_TRY(check_launch());
+        } else {
         // with a tile list: up to 7 padding workgroups after the deep ones,
         // one more for the split of the remainder, and the extra waves of
         // the split tiles (static schedule)
@@ -4037,6 +4257,7 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
             if (mode == 0) hipLaunchKernelGGL(k_split_merge<0>, gm, dim3(kBS), 0, s, q);
             else hipLaunchKernelGGL(k_split_merge<1>, gm, dim3(kBS), 0, s, q);
             FT_TRY(check_launch());
+        }
         }
     }
     if (timed) (void)hipEventRecord(c->ev[3], s);
@@ -4138,6 +4359,7 @@ SearchSpec reinit_search_spec(double* phi, double* phi_ifn, const ft_opts* o)
     sp.query_block = o->query_block;
     sp.waves_per_simd = o->waves_per_simd > 0 ? o->waves_per_simd : 8;
     sp.work_queue = o->schedule == FT_SCHED_QUEUE;
+    sp.bins = o->schedule == FT_SCHED_BINS;
     return sp;
 }
 
@@ -4156,8 +4378,10 @@ int validate_opts(const ft_opts* o)
         return ft::set_error(FT_ERR_ARG, "query_block must be 0, 64, 128 or 256");
     if (o->waves_per_simd != 0 && (o->waves_per_simd < 5 || o->waves_per_simd > 8))
         return ft::set_error(FT_ERR_ARG, "waves_per_simd must be 0 or 5..8");
-    if (o->schedule != FT_SCHED_STATIC && o->schedule != FT_SCHED_QUEUE)
+    if (o->schedule != FT_SCHED_STATIC && o->schedule != FT_SCHED_QUEUE && o->schedule != FT_SCHED_BINS)
         return ft::set_error(FT_ERR_ARG, "unknown query schedule %d", o->schedule);
+    if (o->schedule == FT_SCHED_BINS && !(o->band > 0.0 && o->mode == FT_DIST_TRIANGLES))
+        return ft::set_error(FT_ERR_ARG, "the bins schedule is for a narrow band in triangles mode");
     return FT_OK;
 }
 

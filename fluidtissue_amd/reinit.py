@@ -94,7 +94,7 @@ def phi_field(kind, xyz, seed=0):
     return phi
 
 
-SCHEDULES = {"static": _lib.FT_SCHED_STATIC, "queue": _lib.FT_SCHED_QUEUE}
+SCHEDULES = {"static": _lib.FT_SCHED_STATIC, "queue": _lib.FT_SCHED_QUEUE, "bins": _lib.FT_SCHED_BINS}
 
 
 def make_opts(mode="points", blend="init", band=0.0, bin_scale=0.0, want_nearest=False,

@@ -85,12 +85,17 @@ typedef struct ft_opts {
                             occupancy); 0 = 8                                  */
     int32_t schedule;    /* FT_SCHED_STATIC: one tile per wave, deep tiles on
                             the first workgroups; FT_SCHED_QUEUE: persistent
-                            waves taking tiles from per-XCD counters          */
+                            waves taking tiles from per-XCD counters;
+                            FT_SCHED_BINS (narrow band, triangles only): no
+                            tree walk, each tile tests the triangles of the
+                            index bins within the band, staged in LDS; the
+                            same results as the tree walk                    */
     int32_t reserved[2];
 } ft_opts;
 
 #define FT_SCHED_STATIC    0
 #define FT_SCHED_QUEUE     1
+#define FT_SCHED_BINS      2
 
 typedef struct ft_counts {
     int64_t n_ifp;       /* P: crossing points                                 */

@@ -411,3 +411,28 @@ def test_split_deep_tiles_give_the_same_bits(mode):
         assert emax <= 1e-5 * h
         assert mism == 0
     r.close()
+
+
+@pytest.mark.parametrize("name,band_h", [("star_64", 6.0), ("spheres_96", 6.0), ("spheres_96", 2.0)])
+def test_bins_schedule_equals_tree_walk(name, band_h):
+    """The narrow band without the tree (each tile tests the triangles of the
+    index bins within the band of its box, staged in LDS) gives the tree
+    walk's results bit for bit: phi, phi_ifn, nearest triangles and the
+    clamped count."""
+    m, phi, h = workload(name)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    walk = r.reset_phi(phi, mode="triangles", blend="update", band=band_h * h, want_nearest=True)
+    near_walk = r.nearest()
+    for qb in (64, 256):
+        bins = r.reset_phi(phi, mode="triangles", blend="update", band=band_h * h, want_nearest=True,
+                           schedule="bins", query_block=qb)
+        assert np.array_equal(bins[0].view(np.uint64), walk[0].view(np.uint64)), qb
+        assert np.array_equal(bins[1].view(np.uint64), walk[1].view(np.uint64)), qb
+        assert np.array_equal(r.nearest(), near_walk), qb
+        assert bins[2].n_clamped == walk[2].n_clamped > 0
+    with pytest.raises(fa._lib.FtError):
+        r.reset_phi(phi, mode="points", band=band_h * h, schedule="bins")
+    with pytest.raises(fa._lib.FtError):
+        r.reset_phi(phi, mode="triangles", schedule="bins")
+    r.close()
