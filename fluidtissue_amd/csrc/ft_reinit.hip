@@ -1817,14 +1817,17 @@ struct BinQuery {
 };
 
 // Largest distance from a triangle's centroid (as k_prim_bin computes it) to
-// its vertices, as float bits (non-negative: integer max orders them).
+// its vertices, as float bits (non-negative: integer max orders them).  A
+// grid-stride loop and one atomic per workgroup (one per wave contended).
+constexpr int kMarginBlocks = 512;
 __global__ __launch_bounds__(kBS) void k_tri_margin(const uint64_t* sc, const float* ifp, const int4* tris,
                                                     uint32_t* out)
 {
+    __shared__ float part[kBS / 64];
     if (halted(sc)) return;
-    const int64_t j = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    const int64_t F = cnt_tris(sc);
     float r = 0.0f;
-    if (j < cnt_tris(sc)) {
+    for (int64_t j = (int64_t)blockIdx.x * kBS + threadIdx.x; j < F; j += (int64_t)gridDim.x * kBS) {
         float a[3], b[3], d[3], c[3];
         tri_vertices(tris, ifp, (int32_t)j, a, b, d);
         for (int k = 0; k < 3; k++) c[k] = (a[k] + b[k] + d[k]) * (1.0f / 3.0f);
@@ -1834,10 +1837,16 @@ __global__ __launch_bounds__(kBS) void k_tri_margin(const uint64_t* sc, const fl
             const float dx = v[q][0] - c[0], dy = v[q][1] - c[1], dz = v[q][2] - c[2];
             m = fmaxf(m, sqrtf(dx * dx + dy * dy + dz * dz));
         }
-        r = m;
+        r = fmaxf(r, m);
     }
     for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o));
-    if (ftd::lane_id() == 0 && r > 0.0f) atomicMax(out, __float_as_uint(r));
+    if (ftd::lane_id() == 0) part[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = part[0];
+        for (int k = 1; k < kBS / 64; k++) m = fmaxf(m, part[k]);
+        if (m > 0.0f) atomicMax(out, __float_as_uint(m));
+    }
 }
 
 // Orders the wave's LDS stores before its loads (a wave runs its lanes
@@ -4227,8 +4236,8 @@ int enqueue_search(ft_ctx* c, hipStream_t s, const SearchSpec& sp, const Caps& c
             // narrow band from the bins, staged in LDS (no tree walk)
             FT_TRY(fill<uint64_t>(s, sc + S_MARGIN, 1, 0ull));
             if (n_cap > 0)
-                hipLaunchKernelGGL(k_tri_margin, dim3(grid1d(n_cap)), dim3(kBS), 0, s, sc, ifp, tris,
-                                   reinterpret_cast<uint32_t*>(sc + S_MARGIN));
+                hipLaunchKernelGGL(k_tri_margin, dim3((unsigned)std::min<int64_t>(grid1d(n_cap), kMarginBlocks)),
+                                   dim3(kBS), 0, s, sc, ifp, tris, reinterpret_cast<uint32_t*>(sc + S_MARGIN));
             const BinQuery bq{G, c->bin_prefix.as<uint64_t>(), c->d_grids.as<Grid>(), c->d_cnt_off.as<int64_t>()};
             const dim3 gb(grid1d(c->n_slots, qb)), bb(qb);
             if (qb == 64) hipLaunchKernelGGL(k_query_bins<64>, gb, bb, 0, s, q, bq);
