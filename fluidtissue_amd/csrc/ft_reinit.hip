@@ -2369,6 +2369,19 @@ __device__ __forceinline__ bool fem_frame(const double x[4][3], double G[4][3], 
     return true;
 }
 
+// First position in the sorted lst[0, cnt) whose entry is not below v
+// (binary search: most of a row's candidates are already in its list).
+__device__ __forceinline__ int fem_lower_bound(const int32_t* lst, int cnt, int32_t v)
+{
+    int lo = 0, hi = cnt;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (lst[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Sorted unique columns of row i (the node and every node of its tets) into
 // lst; returns the count, or -1 past kFemMaxCols.
 __device__ int fem_row_cols(const uint64_t* n2t_off, const int32_t* n2t_ids, const int4* tets, int64_t i,
@@ -2380,8 +2393,7 @@ __device__ int fem_row_cols(const uint64_t* n2t_off, const int32_t* n2t_ids, con
         const int v4[4] = {q.x, q.y, q.z, q.w};
         for (int j = 0; j < 4; j++) {
             const int32_t v = v4[j];
-            int pos = 0;
-            while (pos < cnt && lst[pos] < v) pos++;
+            const int pos = fem_lower_bound(lst, cnt, v);
             if (pos < cnt && lst[pos] == v) continue;
             if (cnt == kFemMaxCols) return -1;
             for (int k = cnt; k > pos; k--) lst[k] = lst[k - 1];
@@ -2433,9 +2445,9 @@ __global__ __launch_bounds__(kFemSlice) void k_fem_rowfill(const uint64_t* n2t_o
         const int v4[4] = {q.x, q.y, q.z, q.w};
         uint32_t m = 0;
         for (int j = 0; j < 4; j++) {
-            int k = 0;
-            while (k < cnt - 1 && l[k] != v4[j]) k++;
-            m |= (uint32_t)k << (8 * j);
+            // every node of the row's tets is in its list (cnt >= 1 here)
+            const int k = fem_lower_bound(l, cnt, v4[j]);
+            m |= (uint32_t)(k < cnt ? k : cnt - 1) << (8 * j);
         }
         fmap[a] = m;
     }
