@@ -77,6 +77,50 @@ def test_graph_replays_equal_eager(mode, blend):
 
 
 @pytest.mark.parametrize("mode", ["triangles", "points"])
+def test_graph_split_deep_tiles(mode):
+    """A batch holding a field with very deep tiles (the 64^3 star reaches about
+    -50 h, where tiles are searched by several waves and merged): the eager
+    call equals the work-queue schedule, which searches every tile whole, and
+    the captured graph replays it bit for bit."""
+    import torch
+    r = fa.Reinit(0)
+    r.set_kuhn_batch([64, 20, 33])
+    m = r.mesh()
+    no, _, _ = r.batch_info()
+    phi = np.concatenate([fa.phi_field("star", m.xyz[no[0]:no[1]]),
+                          fa.phi_field("sphere_wave", m.xyz[no[1]:no[2]]),
+                          fa.phi_field("spheres", m.xyz[no[2]:no[3]], seed=3)])
+    assert phi[:no[1]].min() <= -25.0 / 63
+    ref_out, ref_ifn, ref_cnt, ref_itf, ref_near = _eager(r, phi, mode, "update", want_nearest=True)
+    d = torch.from_numpy(phi.copy()).cuda()
+    di = torch.empty_like(d)
+    r.reset_phi_batch_device(d.data_ptr(), di.data_ptr(), mode=mode, blend="update", want_nearest=True,
+                             schedule="queue")
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy().view(np.uint64), ref_out.view(np.uint64))
+    assert np.array_equal(di.cpu().numpy().view(np.uint64), ref_ifn.view(np.uint64))
+    assert np.array_equal(r.nearest(), ref_near)
+    other = fa.Reinit(0)                       # eager work between the replays
+    other.set_kuhn_batch([24, 18])
+    other_phi = _batch_phi(other, 2)
+    src = torch.from_numpy(phi).cuda()
+    for rep in range(3):
+        d.copy_(src)
+        di.fill_(7.0)
+        cnts = r.reset_phi_graph_device(d.data_ptr(), di.data_ptr(), mode=mode, blend="update",
+                                        want_nearest=True)
+        torch.cuda.synchronize()
+        assert [(c.n_ifp, c.n_tri, c.n_if_elems) for c in cnts] == ref_cnt, rep
+        assert np.array_equal(d.cpu().numpy().view(np.uint64), ref_out.view(np.uint64)), rep
+        assert np.array_equal(di.cpu().numpy().view(np.uint64), ref_ifn.view(np.uint64)), rep
+        assert np.array_equal(r.nearest(), ref_near), rep
+        other.reset_phi(other_phi, mode="triangles")
+        torch.cuda.synchronize()
+    other.close()
+    r.close()
+
+
+@pytest.mark.parametrize("mode", ["triangles", "points"])
 def test_graph_deep_tiles_first(mode):
     """Fields with nodes deep inside the surface (a sphere of radius 0.4 on
     40^3 and 33^3 grids: nodes down to ~15h inside), so the query's schedule
