@@ -43,10 +43,12 @@ def _pair(r, phi, mode, blend, stream):
 def test_noalloc_equals_reinit_device(mode):
     """Each handle's first call is the allocation-free one: every buffer the
     reinit sizes then comes from the scratch, and the library refuses the
-    call (FT_ERR_INTERNAL) if one of them were left pointing into it."""
+    call (FT_ERR_INTERNAL) if one of them were left pointing into it.  The
+    40^3 star reaches about -28 h, so its deepest tiles are split over
+    several waves, whose parts also live in the scratch."""
     import torch
     stream = torch.cuda.current_stream().cuda_stream
-    big, small = fa.kuhn_mesh(48, jitter=0.2, seed=1), fa.kuhn_mesh(30)
+    big, small = fa.kuhn_mesh(48, jitter=0.2, seed=1), fa.kuhn_mesh(40)
     phi_big = fa.phi_field("spheres", big.xyz, seed=2)
     phi_small = fa.phi_field("star", small.xyz)
     rb, rs = fa.Reinit(0), fa.Reinit(0)
