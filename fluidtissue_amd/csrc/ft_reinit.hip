@@ -2382,23 +2382,41 @@ __device__ __forceinline__ int fem_lower_bound(const int32_t* lst, int cnt, int3
     return lo;
 }
 
+// The tets a0 + u (u < kFemGather, those below a1) of a row's list, their
+// list entries and then the tets loaded together: kFemGather of each load in
+// flight per lane instead of one dependent pair at a time.
+constexpr int kFemGather = 4;
+__device__ __forceinline__ void fem_gather_tets(const int32_t* n2t_ids, const int4* tets, uint64_t a0,
+                                                uint64_t a1, int4 (&q)[kFemGather])
+{
+    int32_t id[kFemGather];
+#pragma unroll
+    for (int u = 0; u < kFemGather; u++) id[u] = a0 + u < a1 ? n2t_ids[a0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < kFemGather; u++) q[u] = id[u] >= 0 ? tets[id[u]] : make_int4(0, 0, 0, 0);
+}
+
 // Sorted unique columns of row i (the node and every node of its tets) into
 // lst; returns the count, or -1 past kFemMaxCols.
 __device__ int fem_row_cols(const uint64_t* n2t_off, const int32_t* n2t_ids, const int4* tets, int64_t i,
                             int32_t* lst)
 {
     int cnt = 0;
-    for (uint64_t a = n2t_off[i]; a < n2t_off[i + 1]; a++) {
-        const int4 q = tets[n2t_ids[a]];
-        const int v4[4] = {q.x, q.y, q.z, q.w};
-        for (int j = 0; j < 4; j++) {
-            const int32_t v = v4[j];
-            const int pos = fem_lower_bound(lst, cnt, v);
-            if (pos < cnt && lst[pos] == v) continue;
-            if (cnt == kFemMaxCols) return -1;
-            for (int k = cnt; k > pos; k--) lst[k] = lst[k - 1];
-            lst[pos] = v;
-            cnt++;
+    const uint64_t a1 = n2t_off[i + 1];
+    for (uint64_t a0 = n2t_off[i]; a0 < a1; a0 += kFemGather) {
+        int4 q[kFemGather];
+        fem_gather_tets(n2t_ids, tets, a0, a1, q);
+        for (int u = 0; u < kFemGather && a0 + u < a1; u++) {
+            const int v4[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+            for (int j = 0; j < 4; j++) {
+                const int32_t v = v4[j];
+                const int pos = fem_lower_bound(lst, cnt, v);
+                if (pos < cnt && lst[pos] == v) continue;
+                if (cnt == kFemMaxCols) return -1;
+                for (int k = cnt; k > pos; k--) lst[k] = lst[k - 1];
+                lst[pos] = v;
+                cnt++;
+            }
         }
     }
     return cnt;
@@ -2440,16 +2458,20 @@ __global__ __launch_bounds__(kFemSlice) void k_fem_rowfill(const uint64_t* n2t_o
     const int cnt = fem_row_cols(n2t_off, n2t_ids, tets, i, l);
     const int W = (int)width[blockIdx.x];
     for (int k = 0; k < W; k++) cols[ell_at(e, i, k)] = (k < cnt) ? l[k] : (int32_t)i;
-    for (uint64_t a = n2t_off[i]; a < n2t_off[i + 1]; a++) {
-        const int4 q = tets[n2t_ids[a]];
-        const int v4[4] = {q.x, q.y, q.z, q.w};
-        uint32_t m = 0;
-        for (int j = 0; j < 4; j++) {
-            // every node of the row's tets is in its list (cnt >= 1 here)
-            const int k = fem_lower_bound(l, cnt, v4[j]);
-            m |= (uint32_t)(k < cnt ? k : cnt - 1) << (8 * j);
+    const uint64_t a1 = n2t_off[i + 1];
+    for (uint64_t a0 = n2t_off[i]; a0 < a1; a0 += kFemGather) {
+        int4 q[kFemGather];
+        fem_gather_tets(n2t_ids, tets, a0, a1, q);
+        for (int u = 0; u < kFemGather && a0 + u < a1; u++) {
+            const int v4[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+            uint32_t m = 0;
+            for (int j = 0; j < 4; j++) {
+                // every node of the row's tets is in its list (cnt >= 1 here)
+                const int k = fem_lower_bound(l, cnt, v4[j]);
+                m |= (uint32_t)(k < cnt ? k : cnt - 1) << (8 * j);
+            }
+            fmap[a0 + u] = m;
         }
-        fmap[a] = m;
     }
 }
 
