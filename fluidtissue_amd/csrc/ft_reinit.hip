@@ -2182,6 +2182,8 @@ __global__ __launch_bounds__(kBS) void k_tet_grad(const int4* tets, int64_t T, c
 
 // Assembler::grad_nodes: per node, the float mean of its tets' gradients,
 // summed in increasing tet id.
+constexpr int kNodeGather = 8;
+
 __global__ __launch_bounds__(kBS) void k_node_normals(const uint64_t* off, const int32_t* ids,
                                                       const float* grad_t, int64_t N, float* out,
                                                       const uint64_t* sc)
@@ -2190,11 +2192,23 @@ __global__ __launch_bounds__(kBS) void k_node_normals(const uint64_t* off, const
     if (i >= N || halted(sc)) return;
     const uint64_t b = off[i], e = off[i + 1];
     float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
-    for (uint64_t j = b; j < e; j++) {
-        const int64_t t = ids[j];
-        n0 += grad_t[3 * t];
-        n1 += grad_t[3 * t + 1];
-        n2 += grad_t[3 * t + 2];
+    // in list order; the list entries, then their gradients, kNodeGather at a
+    // time (loads in flight together instead of one dependent pair per tet)
+    for (uint64_t j0 = b; j0 < e; j0 += kNodeGather) {
+        int64_t t[kNodeGather];
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++) t[u] = j0 + u < e ? (int64_t)ids[j0 + u] : -1;
+        float g[kNodeGather][3];
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++)
+            for (int c = 0; c < 3; c++) g[u][c] = t[u] >= 0 ? grad_t[3 * t[u] + c] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++)
+            if (j0 + u < e) {
+                n0 += g[u][0];
+                n1 += g[u][1];
+                n2 += g[u][2];
+            }
     }
     const float cnt = (float)(e - b);
     out[3 * i] = n0 / cnt;
@@ -2228,7 +2242,17 @@ __global__ __launch_bounds__(kBS) void k_node_grad_comp(const uint64_t* off, con
     if (i >= N) return;
     const uint64_t b = off[i], e = off[i + 1];
     float s = 0.0f;
-    for (uint64_t j = b; j < e; j++) s += grad_t[3 * (int64_t)ids[j] + comp];
+    for (uint64_t j0 = b; j0 < e; j0 += kNodeGather) {      // in list order, as k_node_normals
+        int64_t t[kNodeGather];
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++) t[u] = j0 + u < e ? (int64_t)ids[j0 + u] : -1;
+        float g[kNodeGather];
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++) g[u] = t[u] >= 0 ? grad_t[3 * t[u] + comp] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++)
+            if (j0 + u < e) s += g[u];
+    }
     const float m = s / (float)(e - b);
     k[i] = first ? m : k[i] + m;
 }
