@@ -157,6 +157,62 @@ def run_advect(args):
     return out
 
 
+def run_normals(args):
+    """One step = the reinit of the 256^3 workload with the level-set normals
+    (LevelSet::resetPhi_ to its end, LevelSet.cpp:429-455 with
+    Assembler::grad_nodes), then the interface curvature
+    (LevelSet::getInterfaceCurvature, LevelSet.cpp:215-253), phi restored from
+    a device copy before each step; both timed, each also on its own.  One
+    surface-tension call (Flow::surfaceTension_) follows, untimed: its force
+    arrays are the caller's host arrays."""
+    import torch
+    import fluidtissue_amd as fa
+    wl = WORKLOADS["spheres_256"]
+    m = fa.kuhn_mesh(wl["n"], jitter=wl["jitter"], seed=1)
+    phi = fa.phi_field(wl["phi"], m.xyz, seed=0)
+    r = fa.Reinit(0)
+    r.set_mesh(m)
+    d0 = torch.from_numpy(phi).cuda()
+    d = torch.empty_like(d0)
+    kw = dict(mode="triangles", bin_scale=1.25, query_block=64, want_normals=True)
+
+    def reinit():
+        d.copy_(d0)
+        r.reset_phi_device(d.data_ptr(), None, **kw)
+
+    for _ in range(args.warmup):
+        reinit()
+        r.curvature()
+    torch.cuda.synchronize()
+    t_re = t_cu = 0.0
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        reinit()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        k = r.curvature()
+        t2 = time.perf_counter()
+        t_re += t1 - t0
+        t_cu += t2 - t1
+    fx, fy, fz = r.surface_tension(0.1, 1)
+    ms_re, ms_cu = t_re * 1e3 / args.steps, t_cu * 1e3 / args.steps
+    N = m.n_nodes
+    log(f"normals 256^3: reinit with normals {ms_re:.3f} ms, curvature {ms_cu:.3f} ms "
+        f"({k.shape[0]} crossing points)")
+    out = {
+        "metric": "Mnodes/s reinit with normals + interface curvature at 256^3",
+        "value": N / ((ms_re + ms_cu) * 1e-3) / 1e6, "unit": "Mnodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_re + ms_cu, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 search; f64 gradients", "data": "synthetic",
+        "config": {"workload": "normals: the 256^3 reinit workload with want_normals, then the curvature",
+                   "nodes": N, "crossing_points": int(k.shape[0])},
+        "reinit_with_normals_ms": ms_re, "curvature_ms": ms_cu,
+        "surface_tension_force_max": float(max(np.abs(fx).max(), np.abs(fy).max(), np.abs(fz).max())),
+    }
+    r.close()
+    return out
+
+
 def run_batched(args):
     """512 fields, N ~ U{24..96}, each a union of 1-16 ellipsoids: one batched
     reinit (autotuned bin size and workgroup size) against 512 serial plain
@@ -788,7 +844,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="spheres_256", choices=sorted(WORKLOADS) + ["batched", "serving", "advect"])
+    ap.add_argument("--workload", default="spheres_256", choices=sorted(WORKLOADS) + ["batched", "serving", "advect", "normals"])
     ap.add_argument("--advect-n", type=int, default=256)
     ap.add_argument("--fields", type=int, default=512)
     ap.add_argument("--min-n", type=int, default=24, help="batched: smallest field side")
@@ -831,6 +887,9 @@ def main():
         return
     if args.workload == "advect":
         print(json.dumps(run_advect(args)), flush=True)
+        return
+    if args.workload == "normals":
+        print(json.dumps(run_normals(args)), flush=True)
         return
     wl = WORKLOADS[args.workload]
     t0 = time.perf_counter()
