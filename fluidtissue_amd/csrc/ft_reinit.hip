@@ -2234,27 +2234,75 @@ __global__ __launch_bounds__(kBS) void k_unit_split(const float* g, int64_t N, d
 // One component of grad_nodes (the float mean over the node's tets of that
 // component) added into k: k = mean on the first call, k += mean after, so
 // k = gg_x.x + gg_y.y + gg_z.z in that order (:237-238).
-__global__ __launch_bounds__(kBS) void k_node_grad_comp(const uint64_t* off, const int32_t* ids,
-                                                        const float* grad_t, int64_t N, int comp,
-                                                        int first, float* k)
+// The curvature's gradients (LevelSet.cpp:233-235, the entries :238 uses):
+// for c = x, y, z the c-th entry of
+// the P1 gradient of g_c, in k_tet_grad's frame and operation order, with one
+// frame per tet for the three fields instead of one pass per field.
+__global__ __launch_bounds__(kBS) void k_tet_grad_diag(const int4* tets, int64_t T, const float* xyz,
+                                                       const double* gx, const double* gy, const double* gz,
+                                                       float* out, const uint64_t* sc)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x;
+    if (t >= T || halted(sc)) return;
+    const int4 q = tets[t];
+    const int qi[4] = {q.x, q.y, q.z, q.w};
+    double x[4][3];
+    for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 3; k++) x[j][k] = (double)xyz[3 * (int64_t)qi[j] + k];
+    double B[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) B[r][c] = x[c + 1][r] - x[0][r];
+    double C[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            C[r][c] = B[r1][c1] * B[r2][c2] - B[r1][c2] * B[r2][c1];
+        }
+    const double det = C[0][0] * B[0][0] + (C[1][0] * B[1][0] + C[2][0] * B[2][0]);
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    if (det != 0.0) {
+        const double inv = 1.0 / det;
+        const double* uf[3] = {gx, gy, gz};
+        for (int r = 0; r < 3; r++) {
+            const double u0 = uf[r][qi[0]], u1 = uf[r][qi[1]], u2 = uf[r][qi[2]], u3 = uf[r][qi[3]];
+            const double b0 = C[r][0] * inv, b1 = C[r][1] * inv, b2 = C[r][2] * inv;
+            const double c0 = (b0 * -1.0 + b1 * -1.0) + b2 * -1.0;
+            g[r] = (float)(((u0 * c0 + u1 * b0) + u2 * b1) + u3 * b2);
+        }
+    }
+    out[3 * t] = g[0];
+    out[3 * t + 1] = g[1];
+    out[3 * t + 2] = g[2];
+}
+
+// Node curvature (LevelSet.cpp:237-239): per component the float mean over the node's
+// tets in increasing id (grad_nodes), then the sum x + y + z in that order.
+__global__ __launch_bounds__(kBS) void k_node_div(const uint64_t* off, const int32_t* ids, const float* gd,
+                                                  int64_t N, float* k)
 {
     const int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x;
     if (i >= N) return;
     const uint64_t b = off[i], e = off[i + 1];
-    float s = 0.0f;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
     for (uint64_t j0 = b; j0 < e; j0 += kNodeGather) {      // in list order, as k_node_normals
         int64_t t[kNodeGather];
 #pragma unroll
         for (int u = 0; u < kNodeGather; u++) t[u] = j0 + u < e ? (int64_t)ids[j0 + u] : -1;
-        float g[kNodeGather];
-#pragma unroll
-        for (int u = 0; u < kNodeGather; u++) g[u] = t[u] >= 0 ? grad_t[3 * t[u] + comp] : 0.0f;
+        float g[kNodeGather][3];
 #pragma unroll
         for (int u = 0; u < kNodeGather; u++)
-            if (j0 + u < e) s += g[u];
+            for (int c = 0; c < 3; c++) g[u][c] = t[u] >= 0 ? gd[3 * t[u] + c] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kNodeGather; u++)
+            if (j0 + u < e) {
+                s0 += g[u][0];
+                s1 += g[u][1];
+                s2 += g[u][2];
+            }
     }
-    const float m = s / (float)(e - b);
-    k[i] = first ? m : k[i] + m;
+    const float cnt = (float)(e - b);
+    const float m0 = s0 / cnt, m1 = s1 / cnt, m2 = s2 / cnt;
+    k[i] = (m0 + m1) + m2;
 }
 
 // Curvature at the crossing points (:243-250): t = float(phi0 / (phi0 - phi1))
@@ -5244,14 +5292,11 @@ int enqueue_curvature(ft_ctx* h, hipStream_t s)
         const uint64_t* sc = h->scalars.as<uint64_t>();
         hipLaunchKernelGGL(k_unit_split, dim3(grid1d(N)), dim3(kBS), 0, s, h->normals.as<float>(), N, g,
                            g + N, g + 2 * N);
-        for (int c = 0; c < 3; c++) {
-            if (T > 0)
-                hipLaunchKernelGGL(k_tet_grad, dim3(grid1d(T)), dim3(kBS), 0, s, h->tets.as<int4>(), T,
-                                   h->xyz.as<float>(), g + c * N, h->grad_t.as<float>(), sc);
-            hipLaunchKernelGGL(k_node_grad_comp, dim3(grid1d(N)), dim3(kBS), 0, s,
-                               h->n2t_off.as<uint64_t>(), h->n2t_ids.as<int32_t>(), h->grad_t.as<float>(),
-                               N, c, c == 0 ? 1 : 0, h->kmesh.as<float>());
-        }
+        if (T > 0)
+            hipLaunchKernelGGL(k_tet_grad_diag, dim3(grid1d(T)), dim3(kBS), 0, s, h->tets.as<int4>(), T,
+                               h->xyz.as<float>(), g, g + N, g + 2 * N, h->grad_t.as<float>(), sc);
+        hipLaunchKernelGGL(k_node_div, dim3(grid1d(N)), dim3(kBS), 0, s, h->n2t_off.as<uint64_t>(),
+                           h->n2t_ids.as<int32_t>(), h->grad_t.as<float>(), N, h->kmesh.as<float>());
         if (P > 0)
             hipLaunchKernelGGL(k_curv_points, dim3(grid1d(P)), dim3(kBS), 0, s, h->ifp2e.as<int32_t>(), P,
                                h->edges.as<int2>(), h->last_phi, h->kmesh.as<float>(), h->k_if.as<float>());
