@@ -1671,7 +1671,8 @@ __device__ __forceinline__ void search_tile(const QueryArgs& a, int64_t base, in
 // Query schedule of a launch: deep tiles first (kDeepTileH), the rest in
 // XCD-chunked Morton order (xcd_block).  Static: one tile per wave, the
 // deep tiles on the first workgroups (which the dispatcher deals over all
-// XCDs), the others after them.  Work queue (a.queue): a persistent grid
+// XCDs; a very deep tile on kSplitParts of them, a.split_out), the others
+// after them.  Work queue (a.queue; every tile whole): a persistent grid
 // whose waves take tiles from counters -- the deep tiles from one shared
 // counter, then the rest from the counter of their XCD (workgroup b runs on
 // XCD b % 8), each XCD's share being its chunks of the static order; a wave
