@@ -436,3 +436,29 @@ def test_bins_schedule_equals_tree_walk(name, band_h):
     with pytest.raises(fa._lib.FtError):
         r.reset_phi(phi, mode="triangles", schedule="bins")
     r.close()
+
+
+def test_bins_schedule_on_a_batch():
+    """The bins kernel on a batch: each tile reads its own field's bin grid
+    and bin range, and the results equal the tree walk's bit for bit."""
+    import torch
+    r = fa.Reinit(0)
+    r.set_kuhn_batch([24, 40, (33, 28, 30), 17], jitter=0.15, seed=3)
+    m = r.mesh()
+    no, _, _ = r.batch_info()
+    kinds = ["star", "spheres", "ellipsoids", "sphere_wave"]
+    phi = np.concatenate([fa.phi_field(kinds[k], m.xyz[no[k]:no[k + 1]], seed=k + 7) for k in range(len(no) - 1)])
+    outs = []
+    for sch in ("static", "bins"):
+        d = torch.from_numpy(phi.copy()).cuda()
+        di = torch.empty_like(d)
+        cnts = r.reset_phi_batch_device(d.data_ptr(), di.data_ptr(), mode="triangles", blend="update",
+                                        band=0.1, want_nearest=True, schedule=sch)
+        torch.cuda.synchronize()
+        outs.append((d.cpu().numpy(), di.cpu().numpy(), r.nearest(), [c.n_clamped for c in cnts]))
+    (o1, i1, n1, c1), (o2, i2, n2, c2) = outs
+    assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64))
+    assert np.array_equal(i1.view(np.uint64), i2.view(np.uint64))
+    assert np.array_equal(n1, n2)
+    assert c1 == c2 and sum(c1) > 0
+    r.close()
