@@ -7,6 +7,12 @@ of every node with the blend fused in (LevelSet::resetPhi_ + LevelSet::init's
 reset, src/fem/LevelSet.cpp:73-74, 429-455).  Each step first restores the
 same input phi with a device copy (included in the timed region).
 
+Other workloads (--workload): star_128 and spheres_256_band6h (narrow
+bands), batched (512 fields in one captured graph), serving (the served
+stream against serial calls), advect (LevelSet::update's advection step) and
+normals (the reinit with the level-set normals, then the interface
+curvature).
+
 Prints one JSON line (rank 0).  --gpus N > 1 is not supported: the workload
 is one field on one GPU.
 """
